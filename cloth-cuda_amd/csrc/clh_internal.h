@@ -1,0 +1,61 @@
+#pragma once
+// Internal declarations shared by the translation units of libclothhip.
+#include "clothhip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace clh {
+
+// Records `msg` as this thread's last error and returns `status`.
+int fail(int status, const std::string& msg);
+// Maps a HIP error to CLH_ERR_DEVICE with the call text; CLH_OK on success.
+int hip_status(hipError_t e, const char* what);
+
+#define CLH_HIP(call)                                                  \
+    do {                                                               \
+        const int _s = ::clh::hip_status((call), #call);               \
+        if (_s != CLH_OK) return _s;                                   \
+    } while (0)
+
+#define CLH_TRY(call)                                                  \
+    do {                                                               \
+        const int _s = (call);                                         \
+        if (_s != CLH_OK) return _s;                                   \
+    } while (0)
+
+// Device buffer owned by the solver.
+template <typename T>
+struct DevBuf {
+    T*     ptr   = nullptr;
+    size_t count = 0;
+    int resize(size_t n)
+    {
+        if (n <= count && ptr) return CLH_OK;
+        release();
+        if (n == 0) return CLH_OK;
+        void* p = nullptr;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess)
+            return fail(CLH_ERR_ALLOC, "hipMalloc of solver scratch failed");
+        ptr = static_cast<T*>(p);
+        count = n;
+        return CLH_OK;
+    }
+    int upload(const std::vector<T>& h)
+    {
+        CLH_TRY(resize(h.size()));
+        if (!h.empty())
+            CLH_HIP(hipMemcpy(ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return CLH_OK;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+};
+
+}  // namespace clh

@@ -1,0 +1,158 @@
+#pragma once
+// Device-side element math shared by the solver kernels.
+//
+// Two stretch projections:
+//  * ref_cuda_rotation reproduces the reference kernel
+//    tri_stretch_project_kernel (src/pd_solver.cu:48-208) branch for branch:
+//    Gram-Schmidt frame of F, eigen-decomposition of F2d^T F2d with the same
+//    1e-10 guards, and the 2-D rotation it assembles as V W^T
+//    (pd_solver.cu:184-194), which is the transpose of the polar factor.
+//  * polar_rotation is the closest rotation to F on the Stiefel manifold:
+//    same Gram-Schmidt frame, then the 2-D rotation maximising tr(R^T F2d),
+//    (cos, sin) proportional to (F2d00 + F2d11, F2d10 - F2d01). For
+//    det(F2d) > 0 this is the polar factor W V^T that the reference's CPU
+//    path takes from an SVD (src/stretch_reference.cpp:38-68); for det < 0
+//    it is the sign-corrected factor of the same code (lines 61-65).
+#include <hip/hip_runtime.h>
+
+namespace clh {
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 ld3(const float* __restrict__ p, size_t i)
+{
+    return f3{p[i * 3 + 0], p[i * 3 + 1], p[i * 3 + 2]};
+}
+__device__ __forceinline__ void st3(float* __restrict__ p, size_t i, f3 v)
+{
+    p[i * 3 + 0] = v.x;
+    p[i * 3 + 1] = v.y;
+    p[i * 3 + 2] = v.z;
+}
+
+// Gram-Schmidt frame (u1, u2) of the columns of F with the reference GPU
+// kernel's fallbacks (src/pd_solver.cu:79-107).
+__device__ __forceinline__ void frame_ref_cuda(f3 F0, f3 F1, f3& u1, f3& u2)
+{
+    const float l0 = sqrtf(dot(F0, F0));
+    u1 = (l0 > 1e-10f) ? mk(F0.x / l0, F0.y / l0, F0.z / l0) : mk(1.0f, 0.0f, 0.0f);
+    const float a = dot(F1, u1);
+    const f3 c = mk(F1.x - a * u1.x, F1.y - a * u1.y, F1.z - a * u1.z);
+    const float lc = sqrtf(dot(c, c));
+    if (lc > 1e-10f) {
+        u2 = mk(c.x / lc, c.y / lc, c.z / lc);
+    } else {
+        u2 = (fabsf(u1.x) > 0.9f) ? mk(-u1.y, u1.x, 0.0f) : mk(0.0f, -u1.z, u1.y);
+        const float l2 = sqrtf(dot(u2, u2));
+        u2 = mk(u2.x / l2, u2.y / l2, u2.z / l2);
+    }
+}
+
+// Rotation (R0, R1) = columns of U * R2d as the reference GPU kernel forms it.
+__device__ __forceinline__ void ref_cuda_rotation(f3 F0, f3 F1, f3& R0, f3& R1)
+{
+    f3 u1, u2;
+    frame_ref_cuda(F0, F1, u1, u2);
+    const float a00 = dot(u1, F0), a01 = dot(u1, F1);
+    const float a10 = dot(u2, F0), a11 = dot(u2, F1);
+
+    // eigen-decomposition of A = F2d^T F2d
+    const float A00 = a00 * a00 + a10 * a10;
+    const float A01 = a00 * a01 + a10 * a11;
+    const float A11 = a01 * a01 + a11 * a11;
+    const float tr = (A00 + A11) * 0.5f;
+    const float disc = sqrtf(fmaxf(0.0f, (A00 - A11) * (A00 - A11) * 0.25f + A01 * A01));
+    const float lam1 = tr + disc, lam2 = tr - disc;
+
+    float v1x = 1.0f, v1y = 0.0f, v2x = 0.0f, v2y = 1.0f;
+    if (!(disc < 1e-10f || fabsf(A01) < 1e-10f)) {
+        const float q = lam1 - A00;
+        const float len = sqrtf(A01 * A01 + q * q);
+        if (len > 1e-10f) {
+            v1x = A01 / len;
+            v1y = q / len;
+        } else {
+            v1x = 1.0f;
+            v1y = 0.0f;
+        }
+        v2x = -v1y;
+        v2y = v1x;
+    }
+
+    const float s1 = sqrtf(fmaxf(0.0f, lam1));
+    const float s2 = sqrtf(fmaxf(0.0f, lam2));
+    float w1x, w1y, w2x, w2y;
+    if (s1 > 1e-10f) {
+        w1x = (a00 * v1x + a01 * v1y) / s1;
+        w1y = (a10 * v1x + a11 * v1y) / s1;
+    } else {
+        w1x = 1.0f;
+        w1y = 0.0f;
+    }
+    if (s2 > 1e-10f) {
+        w2x = (a00 * v2x + a01 * v2y) / s2;
+        w2y = (a10 * v2x + a11 * v2y) / s2;
+    } else {
+        const float fx = a00 * v2x + a01 * v2y;
+        const float fy = a10 * v2x + a11 * v2y;
+        const float d = w1x * fx + w1y * fy;
+        const float gx = fx - d * w1x, gy = fy - d * w1y;
+        const float lg = sqrtf(gx * gx + gy * gy);
+        if (lg > 1e-10f) {
+            w2x = gx / lg;
+            w2y = gy / lg;
+        } else {
+            w2x = -w1y;
+            w2y = w1x;
+        }
+    }
+    // R2d[i][j] = V[i,:] . W[j,:]
+    const float r00 = v1x * w1x + v2x * w2x;
+    const float r01 = v1x * w1y + v2x * w2y;
+    const float r10 = v1y * w1x + v2y * w2x;
+    const float r11 = v1y * w1y + v2y * w2y;
+    R0 = mk(u1.x * r00 + u2.x * r10, u1.y * r00 + u2.y * r10, u1.z * r00 + u2.z * r10);
+    R1 = mk(u1.x * r01 + u2.x * r11, u1.y * r01 + u2.y * r11, u1.z * r01 + u2.z * r11);
+}
+
+// Closest rotation on the Stiefel manifold (the frame fallbacks follow the
+// CPU path, src/stretch_reference.cpp:43-54).
+__device__ __forceinline__ void polar_rotation(f3 F0, f3 F1, f3& R0, f3& R1)
+{
+    const float l0 = sqrtf(dot(F0, F0));
+    const f3 u1 = (l0 > 1e-10f) ? mk(F0.x / l0, F0.y / l0, F0.z / l0) : mk(1.0f, 0.0f, 0.0f);
+    const float a = dot(F1, u1);
+    const f3 c = mk(F1.x - a * u1.x, F1.y - a * u1.y, F1.z - a * u1.z);
+    const float lc = sqrtf(dot(c, c));
+    f3 u2;
+    if (lc > 1e-10f) {
+        u2 = mk(c.x / lc, c.y / lc, c.z / lc);
+    } else {
+        u2 = mk(-u1.y, u1.x, 0.0f);
+        if (sqrtf(dot(u2, u2)) < 1e-10f) u2 = mk(0.0f, -u1.z, u1.y);
+        const float l2 = sqrtf(dot(u2, u2));
+        u2 = mk(u2.x / l2, u2.y / l2, u2.z / l2);
+    }
+    const float a00 = dot(u1, F0), a01 = dot(u1, F1);
+    const float a10 = dot(u2, F0), a11 = dot(u2, F1);
+    float cs = a00 + a11, sn = a10 - a01;
+    const float n = sqrtf(cs * cs + sn * sn);
+    if (n > 1e-30f) {
+        cs = cs / n;
+        sn = sn / n;
+    } else {
+        cs = 1.0f;
+        sn = 0.0f;
+    }
+    // R2d = [[cs, -sn], [sn, cs]]; R = [u1 u2] R2d
+    R0 = mk(u1.x * cs + u2.x * sn, u1.y * cs + u2.y * sn, u1.z * cs + u2.z * sn);
+    R1 = mk(u2.x * cs - u1.x * sn, u2.y * cs - u1.y * sn, u2.z * cs - u1.z * sn);
+}
+
+}  // namespace clh

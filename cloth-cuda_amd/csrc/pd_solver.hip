@@ -1,0 +1,999 @@
+// Solver object of the C ABI: plan (per-vertex adjacency), the frame
+// pipeline of both modes, hipGraph capture of fixed-length frames, FP64
+// residual statistics and the measurement entry points.
+//
+// Frame pipeline (one HIP stream):
+//   prologue     x_prev = x_n, y = x_n + h v + h^2 a, first iterate, pin map
+//
+// PD mode iterates on displacements d_k = x_k - x_n (x_n kept in x_prev) and
+// forms edge vectors as (x_n,j - x_n,i) + (d_j - d_i). The global system is
+// stiff (stiffness / mass ~ 1e5 at 256x256), so rounding absolute FP32
+// positions (~6e-8 m) alone leaves a relative residual near 1.5e-4 at
+// 256x256; with displacements the floor falls by about two orders of
+// magnitude while storage and arithmetic stay FP32. Reference mode keeps
+// absolute positions, as the reference does.
+//   per iteration k:
+//     element kernels  triangles -> corner forces, hinges -> bend vectors
+//     vertex kernel    deterministic gather over the vertex's corners and
+//                      hinges in ascending element order, Jacobi update,
+//                      Chebyshev blend, pins, FP64 residual partials
+//   epilogue     x_{n+1}, v_{n+1} = (x_{n+1} - x_n)(1 - d)/h, pins
+// No atomics: every sum has a fixed order, so a frame is bitwise
+// reproducible run to run.
+#include "clh_internal.h"
+#include "pd_kernels.h"
+#include "pd_fused.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace clh;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxIterations = 1 << 20;
+
+inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// x_prev = x, first iterate, pin map cleared. Reference mode: y = x + h v +
+// h^2 a and x0 = x (or the guess). PD mode: y holds dy = h v + h^2 a and x0
+// holds the displacement of the guess (0 without one).
+__global__ void __launch_bounds__(kBlock)
+k_prologue(const float* __restrict__ pos, const float* __restrict__ vel,
+           const float* __restrict__ guess, float* __restrict__ xprev, float* __restrict__ y,
+           float* __restrict__ x0, int* __restrict__ pin_map, int N, float dt, float h2a0,
+           float h2a1, float h2a2, int disp)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const f3 x = ld3(pos, v);
+    const f3 u = ld3(vel, v);
+    st3(xprev, v, x);
+    const f3 dy = mk(dt * u.x + h2a0, dt * u.y + h2a1, dt * u.z + h2a2);
+    if (disp) {
+        st3(y, v, dy);
+        st3(x0, v, guess ? ld3(guess, v) - x : mk(0.0f, 0.0f, 0.0f));
+    } else {
+        st3(y, v, mk(x.x + dt * u.x + h2a0, x.y + dt * u.y + h2a1, x.z + dt * u.z + h2a2));
+        st3(x0, v, guess ? ld3(guess, v) : x);
+    }
+    pin_map[v] = -1;
+}
+
+// pin_map[idx[i]] = i; in PD mode the first displacement of a pinned vertex
+// takes it to its target.
+__global__ void __launch_bounds__(kBlock)
+k_pin_scatter(const int* __restrict__ idx, const float* __restrict__ target,
+              const float* __restrict__ xprev, int* __restrict__ pin_map,
+              float* __restrict__ x0, int P, int disp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const int v = idx[i];
+    pin_map[v] = i;
+    if (disp) st3(x0, v, ld3(target, i) - ld3(xprev, v));
+}
+
+// Per triangle: deformation gradient, rotation, and the three corner forces.
+// Reference mode stores h^2 wA D^T R (src/pd_solver.cu:256-298, with the
+// rotation of pd_solver.cu:48-208). PD mode stores h^2 wA D^T (R - F) in
+// `fd` (residual) and h^2 wA D^T R in `fr` (right-hand side).
+// PD mode reads positions as base + displacement x.
+template <int kMode>
+__global__ void __launch_bounds__(kBlock)
+k_triangles(const float* __restrict__ base, const float* __restrict__ x,
+            const int* __restrict__ tris, const float* __restrict__ dminv,
+            const float* __restrict__ area, const float* __restrict__ kst, float* __restrict__ fr,
+            float* __restrict__ fd, int T, float h2)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const int i0 = tris[(size_t)t * 3 + 0], i1 = tris[(size_t)t * 3 + 1],
+              i2 = tris[(size_t)t * 3 + 2];
+    const f3 p0 = ld3(x, i0), p1 = ld3(x, i1), p2 = ld3(x, i2);
+    f3 d0 = p1 - p0, d1 = p2 - p0;
+    if (kMode == CLH_MODE_PD) {
+        const f3 b0 = ld3(base, i0);
+        d0 = (ld3(base, i1) - b0) + d0;
+        d1 = (ld3(base, i2) - b0) + d1;
+    }
+    const float g00 = dminv[(size_t)t * 4 + 0], g10 = dminv[(size_t)t * 4 + 1];
+    const float g01 = dminv[(size_t)t * 4 + 2], g11 = dminv[(size_t)t * 4 + 3];
+    const f3 F0 = mk(d0.x * g00 + d1.x * g10, d0.y * g00 + d1.y * g10, d0.z * g00 + d1.z * g10);
+    const f3 F1 = mk(d0.x * g01 + d1.x * g11, d0.y * g01 + d1.y * g11, d0.z * g01 + d1.z * g11);
+    const float wA = kst[t] * area[t] * h2;
+    const float s0 = g00 + g10, s1 = g01 + g11;
+    f3 R0, R1;
+    if (kMode == CLH_MODE_REF_CUDA) ref_cuda_rotation(F0, F1, R0, R1);
+    else                            polar_rotation(F0, F1, R0, R1);
+
+    float* o = fr + (size_t)t * 9;
+    st3(o, 0, mk(-(s0 * R0.x + s1 * R1.x) * wA, -(s0 * R0.y + s1 * R1.y) * wA,
+                 -(s0 * R0.z + s1 * R1.z) * wA));
+    st3(o, 1, mk((g00 * R0.x + g01 * R1.x) * wA, (g00 * R0.y + g01 * R1.y) * wA,
+                 (g00 * R0.z + g01 * R1.z) * wA));
+    st3(o, 2, mk((g10 * R0.x + g11 * R1.x) * wA, (g10 * R0.y + g11 * R1.y) * wA,
+                 (g10 * R0.z + g11 * R1.z) * wA));
+    if (kMode == CLH_MODE_PD) {
+        const f3 E0 = R0 - F0, E1 = R1 - F1;
+        float* q = fd + (size_t)t * 9;
+        st3(q, 0, mk(-(s0 * E0.x + s1 * E1.x) * wA, -(s0 * E0.y + s1 * E1.y) * wA,
+                     -(s0 * E0.z + s1 * E1.z) * wA));
+        st3(q, 1, mk((g00 * E0.x + g01 * E1.x) * wA, (g00 * E0.y + g01 * E1.y) * wA,
+                     (g00 * E0.z + g01 * E1.z) * wA));
+        st3(q, 2, mk((g10 * E0.x + g11 * E1.x) * wA, (g10 * E0.y + g11 * E1.y) * wA,
+                     (g10 * E0.z + g11 * E1.z) * wA));
+    }
+}
+
+// Per hinge: curvature vector e = sum w_i x_i and its projection
+// p = e * n_rest / |e| (0 when |e| or n_rest <= 1e-10), as in
+// src/pd_solver.cu:304-335. Stores p, and in PD mode also p - e, with e
+// evaluated from base + displacement as
+// sum_{i>0} w_i ((b_i - b_0) + (x_i - x_0)) + (sum_i w_i)(b_0 + x_0).
+template <int kMode>
+__global__ void __launch_bounds__(kBlock)
+k_hinges(const float* __restrict__ base, const float* __restrict__ x,
+         const int* __restrict__ quads, const float* __restrict__ w,
+         const float* __restrict__ nrest, float* __restrict__ bp, float* __restrict__ bd, int E)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    f3 acc = mk(0.0f, 0.0f, 0.0f);
+    if (kMode == CLH_MODE_PD) {
+        const int q0 = quads[(size_t)e * 4];
+        const f3 b0 = ld3(base, q0), x0 = ld3(x, q0);
+#pragma unroll
+        for (int i = 1; i < 4; ++i) {
+            const int qi = quads[(size_t)e * 4 + i];
+            const float wi = w[(size_t)e * 4 + i];
+            const f3 d = (ld3(base, qi) - b0) + (ld3(x, qi) - x0);
+            acc.x += wi * d.x;
+            acc.y += wi * d.y;
+            acc.z += wi * d.z;
+        }
+        const float W = ((w[(size_t)e * 4] + w[(size_t)e * 4 + 1]) + w[(size_t)e * 4 + 2]) +
+                        w[(size_t)e * 4 + 3];
+        const f3 p0 = b0 + x0;
+        acc.x += W * p0.x;
+        acc.y += W * p0.y;
+        acc.z += W * p0.z;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float wi = w[(size_t)e * 4 + i];
+            const f3 p = ld3(x, quads[(size_t)e * 4 + i]);
+            acc.x += wi * p.x;
+            acc.y += wi * p.y;
+            acc.z += wi * p.z;
+        }
+    }
+    const float nr = nrest[e];
+    const float len = sqrtf(dot(acc, acc));
+    f3 p = mk(0.0f, 0.0f, 0.0f);
+    if (len > 1e-10f && nr > 1e-10f) p = acc * (nr / len);
+    st3(bp, e, p);
+    if (kMode == CLH_MODE_PD) st3(bd, e, p - acc);
+}
+
+struct VertexArgs {
+    const float* base;    // x_n (PD: iterates are displacements from it)
+    const float* x;       // x_k
+    const float* xkm1;    // x_{k-1} (Chebyshev)
+    float*       out;     // x_{k+1}
+    const float* y;       // y (reference mode) or y - x_n (PD)
+    const float* mass;
+    const float* diag;
+    const int*   pin_map;
+    const float* pin_target;
+    const int*   vt_ptr;  // vertex -> triangle corners (t*3+c), ascending
+    const int*   vt_idx;
+    const int*   ve_ptr;  // vertex -> hinge slots (e*4+i), ascending
+    const int*   ve_idx;
+    const float* fr;      // corner forces of R   [T*9]
+    const float* fd;      // corner forces of R-F [T*9] (PD)
+    const float* bp;      // hinge projections p  [E*3]
+    const float* bd;      // hinge p - e          [E*3] (PD)
+    const float* bk;      // hinge stiffness      [E]
+    const float* bw;      // hinge weights        [E*4]
+    double*      partials;  // [gridDim.x * 2] per-block sum |r|^2, |b|^2 (PD, when non-null)
+    int   N;
+    int   use_stretch;
+    int   use_bend;
+    float h2;
+    float omega;          // Chebyshev weight of this update (PD)
+    int   blend;          // apply the Chebyshev blend (PD)
+};
+
+// Reference mode: rhs = sum corner forces + sum k h^2 w p + m y,
+// x = rhs / diag (0 when diag <= 1e-10), pins (src/pd_solver.cu:364-398,
+// 414-428).
+__global__ void __launch_bounds__(kBlock) k_vertex_ref(VertexArgs a)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.N) return;
+    const int slot = a.pin_map[v];
+    if (slot >= 0) {
+        st3(a.out, v, ld3(a.pin_target, slot));
+        return;
+    }
+    f3 acc = mk(0.0f, 0.0f, 0.0f);
+    if (a.use_stretch)
+        for (int j = a.vt_ptr[v]; j < a.vt_ptr[v + 1]; ++j) acc = acc + ld3(a.fr, a.vt_idx[j]);
+    if (a.use_bend)
+        for (int j = a.ve_ptr[v]; j < a.ve_ptr[v + 1]; ++j) {
+            const int s = a.ve_idx[j];
+            const float kw = (a.bk[s >> 2] * a.h2) * a.bw[s];
+            const f3 p = ld3(a.bp, s >> 2);
+            acc = acc + mk(kw * p.x, kw * p.y, kw * p.z);
+        }
+    const float m = a.mass[v];
+    const f3 yv = ld3(a.y, v);
+    const f3 rhs = mk(acc.x + m * yv.x, acc.y + m * yv.y, acc.z + m * yv.z);
+    const float d = a.diag[v];
+    const float inv = (d > 1e-10f) ? (1.0f / d) : 0.0f;
+    st3(a.out, v, rhs * inv);
+}
+
+// Block-wide sum of two doubles; thread 0 writes them to out[2*blockIdx.x].
+__device__ void block_sum2(double r, double b, double* out)
+{
+    __shared__ double sr[kBlock / 64], sb[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        r += __shfl_down(r, off, 64);
+        b += __shfl_down(b, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        sr[wave] = r;
+        sb[wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tr = 0.0, tb = 0.0;
+        for (int i = 0; i < kBlock / 64; ++i) {
+            tr += sr[i];
+            tb += sb[i];
+        }
+        out[2 * blockIdx.x + 0] = tr;
+        out[2 * blockIdx.x + 1] = tb;
+    }
+}
+
+// PD mode on displacements: r = m (dy - x) + sum h^2 wA D^T (R - F) +
+// sum h^2 k w (p - e), x_J = x + r / diag, x_{k+1} = omega (x_J - x_{k-1}) +
+// x_{k-1} when blending, pinned vertices held at target - base.
+__global__ void __launch_bounds__(kBlock) k_vertex_pd(VertexArgs a)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0.0, bb = 0.0;
+    if (v < a.N) {
+        const int slot = a.pin_map[v];
+        const f3 bv = ld3(a.base, v);
+        if (slot >= 0) {
+            st3(a.out, v, ld3(a.pin_target, slot) - bv);
+        } else {
+            f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
+            if (a.use_stretch)
+                for (int j = a.vt_ptr[v]; j < a.vt_ptr[v + 1]; ++j) {
+                    const int s = a.vt_idx[j];
+                    ar = ar + ld3(a.fd, s);
+                    ab = ab + ld3(a.fr, s);
+                }
+            if (a.use_bend)
+                for (int j = a.ve_ptr[v]; j < a.ve_ptr[v + 1]; ++j) {
+                    const int s = a.ve_idx[j];
+                    const float kw = (a.bk[s >> 2] * a.h2) * a.bw[s];
+                    const f3 d = ld3(a.bd, s >> 2), p = ld3(a.bp, s >> 2);
+                    ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+                    ab = ab + mk(kw * p.x, kw * p.y, kw * p.z);
+                }
+            const float m = a.mass[v];
+            const f3 xv = ld3(a.x, v), yv = ld3(a.y, v);
+            const f3 r = mk(ar.x + m * (yv.x - xv.x), ar.y + m * (yv.y - xv.y),
+                            ar.z + m * (yv.z - xv.z));
+            const float d = a.diag[v];
+            const float inv = (d > 1e-10f) ? (1.0f / d) : 0.0f;
+            f3 xn = mk(xv.x + r.x * inv, xv.y + r.y * inv, xv.z + r.z * inv);
+            if (a.blend) {
+                const f3 xp = ld3(a.xkm1, v);
+                xn = mk(a.omega * (xn.x - xp.x) + xp.x, a.omega * (xn.y - xp.y) + xp.y,
+                        a.omega * (xn.z - xp.z) + xp.z);
+            }
+            st3(a.out, v, xn);
+            if (a.partials) {
+                const f3 ya = bv + yv;
+                const f3 b = mk(ab.x + m * ya.x, ab.y + m * ya.y, ab.z + m * ya.z);
+                rr = (double)r.x * r.x + (double)r.y * r.y + (double)r.z * r.z;
+                bb = (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
+            }
+        }
+    }
+    if (a.partials) block_sum2(rr, bb, a.partials);
+}
+
+// Sums the per-block partials in a fixed order into hist[2*slot .. +1].
+__global__ void __launch_bounds__(kBlock)
+k_reduce_partials(const double* __restrict__ partials, int nblocks, double* __restrict__ hist,
+                  int slot)
+{
+    double r = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += kBlock) {
+        r += partials[2 * i + 0];
+        b += partials[2 * i + 1];
+    }
+    __shared__ double sr[kBlock], sb[kBlock];
+    sr[threadIdx.x] = r;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sr[threadIdx.x] += sr[threadIdx.x + s];
+            sb[threadIdx.x] += sb[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        hist[2 * slot + 0] = sr[0];
+        hist[2 * slot + 1] = sb[0];
+    }
+}
+
+// x_{n+1} and v = (x_{n+1} - x_n)(1 - d)/h; pinned vertices at target with
+// zero velocity (src/pd_solver.cu:400-412, 602-618). In PD mode xf holds the
+// displacement d, so x_{n+1} = x_n + d and v = d (1 - d)/h.
+__global__ void __launch_bounds__(kBlock)
+k_epilogue(const float* __restrict__ xf, const float* __restrict__ xprev,
+           const int* __restrict__ pin_map, const float* __restrict__ target,
+           float* __restrict__ pos, float* __restrict__ vel, int N, float scale, int disp)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const int slot = pin_map[v];
+    if (slot >= 0) {
+        st3(pos, v, ld3(target, slot));
+        st3(vel, v, mk(0.0f, 0.0f, 0.0f));
+        return;
+    }
+    const f3 x = ld3(xf, v), xp = ld3(xprev, v);
+    if (disp) {
+        st3(pos, v, xp + x);
+        st3(vel, v, x * scale);
+    } else {
+        st3(pos, v, x);
+        st3(vel, v, mk((x.x - xp.x) * scale, (x.y - xp.y) * scale, (x.z - xp.z) * scale));
+    }
+}
+
+// Absolute positions of an iterate into a trace slot.
+__global__ void __launch_bounds__(kBlock)
+k_trace(const float* __restrict__ it, const float* __restrict__ xprev, float* __restrict__ out,
+        int N, int disp)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    st3(out, v, disp ? ld3(xprev, v) + ld3(it, v) : ld3(it, v));
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+
+// CSR of vertex -> incident element slots (element * arity + corner), slots in
+// ascending order.
+void build_incidence(const std::vector<int>& elems, int arity, int N, std::vector<int>& ptr,
+                     std::vector<int>& idx)
+{
+    const size_t ne = elems.size() / (size_t)arity;
+    ptr.assign((size_t)N + 1, 0);
+    for (size_t s = 0; s < ne * (size_t)arity; ++s) ptr[(size_t)elems[s] + 1]++;
+    for (int v = 0; v < N; ++v) ptr[(size_t)v + 1] += ptr[(size_t)v];
+    idx.assign(ne * (size_t)arity, 0);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t s = 0; s < ne * (size_t)arity; ++s) idx[(size_t)fill[(size_t)elems[s]]++] = (int)s;
+}
+
+bool same_args(const clh_step_args& a, const clh_step_args& b)
+{
+    return std::memcmp(&a, &b, sizeof(a)) == 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// solver object
+// ---------------------------------------------------------------------------
+
+struct clh_solver {
+    clh_config cfg;
+    int N = 0, T = 0, E = 0;
+    hipStream_t stream = nullptr;      // stream work is enqueued on
+    hipStream_t own_stream = nullptr;  // created by the solver
+    bool user_stream = false;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
+
+    // plan
+    const int* plan_tris = nullptr;
+    const int* plan_quads = nullptr;
+    bool plan_ready = false;
+    DevBuf<int> vt_ptr, vt_idx, ve_ptr, ve_idx;
+    FusedPlan fused;
+    int variant = CLH_KERNEL_UNFUSED;
+
+    // scratch
+    DevBuf<float> y, xprev, it[3], fr, fd, bp, bd;
+    DevBuf<int> pin_map;
+    DevBuf<double> partials, hist;
+    int nblocks_v = 0;
+
+    // pinned host mirror of the residual history
+    double* h_hist = nullptr;
+    int h_hist_cap = 0;
+
+    // graph of a fixed-length frame
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    clh_step_args graph_args = {};
+    bool graph_valid = false;
+
+    // last pin list whose indices were range-checked
+    const int* pins_checked_ptr = nullptr;
+    int pins_checked_n = -1;
+
+    // statistics of the last frame
+    int last_iters = 0;
+    int last_checks = 0;
+    int converged = 0;
+    int frames = 0;
+};
+
+namespace {
+
+int ensure_plan(clh_solver* s, const clh_step_args& a)
+{
+    if (s->plan_ready && s->plan_tris == a.d_tris && s->plan_quads == a.d_bend_quads)
+        return CLH_OK;
+    const int N = s->N, T = s->T;
+    const int E = a.d_bend_quads ? s->E : 0;
+    std::vector<int> tris((size_t)T * 3), quads((size_t)E * 4);
+    if (T > 0) {
+        if (!a.d_tris) return fail(CLH_ERR_INVALID, "clh_solver_step: null d_tris with num_tris > 0");
+        CLH_HIP(hipMemcpy(tris.data(), a.d_tris, tris.size() * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    if (E > 0)
+        CLH_HIP(hipMemcpy(quads.data(), a.d_bend_quads, quads.size() * sizeof(int),
+                          hipMemcpyDeviceToHost));
+    for (int v : tris)
+        if (v < 0 || v >= N) return fail(CLH_ERR_INVALID, "triangle vertex index out of range");
+    for (int v : quads)
+        if (v < 0 || v >= N) return fail(CLH_ERR_INVALID, "hinge vertex index out of range");
+
+    std::vector<int> ptr, idx;
+    build_incidence(tris, 3, N, ptr, idx);
+    CLH_TRY(s->vt_ptr.upload(ptr));
+    CLH_TRY(s->vt_idx.upload(idx));
+    build_incidence(quads, 4, N, ptr, idx);
+    CLH_TRY(s->ve_ptr.upload(ptr));
+    CLH_TRY(s->ve_idx.upload(idx));
+
+    s->variant = CLH_KERNEL_UNFUSED;
+    if (s->cfg.kernel_variant != CLH_KERNEL_UNFUSED) {
+        const int st = fused_plan_build(s->fused, N, tris, quads, s->cfg.mode);
+        if (st == CLH_OK) s->variant = CLH_KERNEL_FUSED;
+        else if (s->cfg.kernel_variant == CLH_KERNEL_FUSED) return st;
+    }
+
+    s->plan_tris = a.d_tris;
+    s->plan_quads = a.d_bend_quads;
+    s->plan_ready = true;
+    s->graph_valid = false;
+    return CLH_OK;
+}
+
+int ensure_scratch(clh_solver* s)
+{
+    const size_t N = (size_t)s->N, T = (size_t)s->T, E = (size_t)s->E;
+    CLH_TRY(s->y.resize(N * 3));
+    CLH_TRY(s->xprev.resize(N * 3));
+    for (auto& b : s->it) CLH_TRY(b.resize(N * 3));
+    CLH_TRY(s->pin_map.resize(N));
+    CLH_TRY(s->fr.resize(T * 9));
+    CLH_TRY(s->bp.resize(E * 3));
+    if (s->cfg.mode == CLH_MODE_PD) {
+        CLH_TRY(s->fd.resize(T * 9));
+        CLH_TRY(s->bd.resize(E * 3));
+    }
+    s->nblocks_v = (int)grid_for(N);
+    CLH_TRY(s->partials.resize((size_t)std::max(s->nblocks_v, fused_max_blocks(s->fused)) * 2));
+    const int max_checks = s->cfg.max_iterations + 1;
+    CLH_TRY(s->hist.resize((size_t)max_checks * 2));
+    if (s->h_hist_cap < max_checks) {
+        if (s->h_hist) (void)hipHostFree(s->h_hist);
+        s->h_hist = nullptr;
+        CLH_HIP(hipHostMalloc((void**)&s->h_hist, (size_t)max_checks * 2 * sizeof(double)));
+        s->h_hist_cap = max_checks;
+    }
+    return CLH_OK;
+}
+
+// Chebyshev weight of the update producing x_{k+1} (Wang 2015 with delay S):
+// 1 before S, 2/(2 - rho^2) at S, 4/(4 - rho^2 w) after.
+float chebyshev_omega(const clh_config& c, int k, float prev)
+{
+    const int S = std::max(1, c.cheb_delay);
+    const float r2 = c.rho * c.rho;
+    if (k < S) return 1.0f;
+    if (k == S) return 2.0f / (2.0f - r2);
+    return 4.0f / (4.0f - r2 * prev);
+}
+
+bool is_check_iteration(const clh_config& c, int k)
+{
+    if (c.mode != CLH_MODE_PD) return false;
+    if (k == c.max_iterations - 1) return true;
+    return c.tolerance > 0.0f && c.check_every > 0 && ((k + 1) % c.check_every) == 0;
+}
+
+// Enqueues the prologue and pin scatter.
+int enqueue_prologue(clh_solver* s, const clh_step_args& a)
+{
+    const clh_config& c = s->cfg;
+    const float h2 = c.dt * c.dt;
+    const float acc[3] = {c.ext_accel[0], c.gravity + c.ext_accel[1], c.ext_accel[2]};
+    const float h2a0 = h2 * acc[0], h2a1 = h2 * acc[1], h2a2 = h2 * acc[2];
+    if (s->N > 0)
+        hipLaunchKernelGGL(k_prologue, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream, a.d_pos,
+                           a.d_vel, a.d_init_guess, s->xprev.ptr, s->y.ptr, s->it[0].ptr,
+                           s->pin_map.ptr, s->N, c.dt, h2a0, h2a1, h2a2,
+                           c.mode == CLH_MODE_PD ? 1 : 0);
+    if (a.num_pinned > 0)
+        hipLaunchKernelGGL(k_pin_scatter, dim3(grid_for(a.num_pinned)), dim3(kBlock), 0,
+                           s->stream, a.d_pin_idx, a.d_pin_target, s->xprev.ptr, s->pin_map.ptr,
+                           s->it[0].ptr, a.num_pinned, c.mode == CLH_MODE_PD ? 1 : 0);
+    return hip_status(hipGetLastError(), "prologue launch");
+}
+
+// Enqueues iteration k (x_k in it[k%3] -> x_{k+1} in it[(k+1)%3]).
+int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega, bool check,
+                      int check_slot)
+{
+    const clh_config& c = s->cfg;
+    const float h2 = c.dt * c.dt;
+    const int pd = (c.mode == CLH_MODE_PD);
+    const float* xk = s->it[k % 3].ptr;
+    const bool use_stretch = s->T > 0 && a.d_stretch_k != nullptr;
+    const bool use_bend = s->E > 0 && a.d_bend_quads != nullptr;
+    const bool blend = pd && c.use_chebyshev && k > 0 && omega != 1.0f;
+
+    if (s->variant == CLH_KERNEL_FUSED) {
+        FusedIterArgs f;
+        f.x = xk;
+        f.xkm1 = s->it[(k + 2) % 3].ptr;
+        f.out = s->it[(k + 1) % 3].ptr;
+        f.y = s->y.ptr;
+        f.mass = a.d_mass;
+        f.diag = a.d_jacobi_diag;
+        f.pin_map = s->pin_map.ptr;
+        f.pin_target = a.d_pin_target;
+        f.partials = check ? s->partials.ptr : nullptr;
+        f.h2 = h2;
+        f.omega = omega;
+        f.blend = blend ? 1 : 0;
+        f.use_stretch = use_stretch ? 1 : 0;
+        f.use_bend = use_bend ? 1 : 0;
+        CLH_TRY(fused_launch_iteration(s->fused, f, c.mode, s->stream));
+        if (check)
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->stream,
+                               s->partials.ptr, s->fused.num_tiles, s->hist.ptr, check_slot);
+        return hip_status(hipGetLastError(), "fused iteration launch");
+    }
+
+    if (use_stretch) {
+        if (pd)
+            hipLaunchKernelGGL(k_triangles<CLH_MODE_PD>, dim3(grid_for(s->T)), dim3(kBlock), 0,
+                               s->stream, s->xprev.ptr, xk, a.d_tris, a.d_Dm_inv, a.d_rest_area, a.d_stretch_k,
+                               s->fr.ptr, s->fd.ptr, s->T, h2);
+        else
+            hipLaunchKernelGGL(k_triangles<CLH_MODE_REF_CUDA>, dim3(grid_for(s->T)), dim3(kBlock),
+                               0, s->stream, s->xprev.ptr, xk, a.d_tris, a.d_Dm_inv, a.d_rest_area,
+                               a.d_stretch_k, s->fr.ptr, s->fd.ptr, s->T, h2);
+    }
+    if (use_bend) {
+        if (pd)
+            hipLaunchKernelGGL(k_hinges<CLH_MODE_PD>, dim3(grid_for(s->E)), dim3(kBlock), 0,
+                               s->stream, s->xprev.ptr, xk, a.d_bend_quads, a.d_bend_w, a.d_bend_n, s->bp.ptr,
+                               s->bd.ptr, s->E);
+        else
+            hipLaunchKernelGGL(k_hinges<CLH_MODE_REF_CUDA>, dim3(grid_for(s->E)), dim3(kBlock), 0,
+                               s->stream, s->xprev.ptr, xk, a.d_bend_quads, a.d_bend_w, a.d_bend_n, s->bp.ptr,
+                               s->bd.ptr, s->E);
+    }
+    VertexArgs va;
+    va.base = s->xprev.ptr;
+    va.x = xk;
+    va.xkm1 = s->it[(k + 2) % 3].ptr;
+    va.out = s->it[(k + 1) % 3].ptr;
+    va.y = s->y.ptr;
+    va.mass = a.d_mass;
+    va.diag = a.d_jacobi_diag;
+    va.pin_map = s->pin_map.ptr;
+    va.pin_target = a.d_pin_target;
+    va.vt_ptr = s->vt_ptr.ptr;
+    va.vt_idx = s->vt_idx.ptr;
+    va.ve_ptr = s->ve_ptr.ptr;
+    va.ve_idx = s->ve_idx.ptr;
+    va.fr = s->fr.ptr;
+    va.fd = s->fd.ptr;
+    va.bp = s->bp.ptr;
+    va.bd = s->bd.ptr;
+    va.bk = a.d_bend_k;
+    va.bw = a.d_bend_w;
+    va.partials = check ? s->partials.ptr : nullptr;
+    va.N = s->N;
+    va.use_stretch = use_stretch ? 1 : 0;
+    va.use_bend = use_bend ? 1 : 0;
+    va.h2 = h2;
+    va.omega = omega;
+    va.blend = blend ? 1 : 0;
+    if (s->N > 0) {
+        if (pd)
+            hipLaunchKernelGGL(k_vertex_pd, dim3(s->nblocks_v), dim3(kBlock), 0, s->stream, va);
+        else
+            hipLaunchKernelGGL(k_vertex_ref, dim3(s->nblocks_v), dim3(kBlock), 0, s->stream, va);
+    }
+    if (check && s->N > 0)
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->stream, s->partials.ptr,
+                           s->nblocks_v, s->hist.ptr, check_slot);
+    return hip_status(hipGetLastError(), "iteration launch");
+}
+
+int enqueue_epilogue(clh_solver* s, const clh_step_args& a, int iters)
+{
+    const clh_config& c = s->cfg;
+    const float scale = (1.0f - c.damping) / c.dt;
+    if (s->N > 0)
+        hipLaunchKernelGGL(k_epilogue, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream,
+                           s->it[iters % 3].ptr, s->xprev.ptr, s->pin_map.ptr, a.d_pin_target,
+                           a.d_pos, a.d_vel, s->N, scale, c.mode == CLH_MODE_PD ? 1 : 0);
+    return hip_status(hipGetLastError(), "epilogue launch");
+}
+
+// Absolute positions of iterate k+1 into slot k of the trace.
+int enqueue_trace(clh_solver* s, const clh_step_args& a, int k)
+{
+    if (s->N > 0)
+        hipLaunchKernelGGL(k_trace, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream,
+                           s->it[(k + 1) % 3].ptr, s->xprev.ptr,
+                           a.d_trace + (size_t)k * s->N * 3, s->N,
+                           s->cfg.mode == CLH_MODE_PD ? 1 : 0);
+    return hip_status(hipGetLastError(), "trace launch");
+}
+
+// Fixed-length frame: every launch is known up front, so it can be captured.
+int enqueue_fixed_frame(clh_solver* s, const clh_step_args& a)
+{
+    const int K = s->cfg.max_iterations;
+    CLH_TRY(enqueue_prologue(s, a));
+    float omega = 1.0f;
+    int checks = 0;
+    for (int k = 0; k < K; ++k) {
+        omega = chebyshev_omega(s->cfg, k, omega);
+        const bool check = is_check_iteration(s->cfg, k);
+        CLH_TRY(enqueue_iteration(s, a, k, omega, check, check ? checks : 0));
+        if (check) ++checks;
+        if (a.d_trace) CLH_TRY(enqueue_trace(s, a, k));
+    }
+    CLH_TRY(enqueue_epilogue(s, a, K));
+    s->last_iters = K;
+    s->last_checks = checks;
+    s->converged = 0;
+    return CLH_OK;
+}
+
+int validate_args(const clh_solver* s, const clh_step_args* a)
+{
+    if (!a) return fail(CLH_ERR_INVALID, "clh_solver_step: null args");
+    if (s->N > 0 && (!a->d_pos || !a->d_vel || !a->d_mass || !a->d_jacobi_diag))
+        return fail(CLH_ERR_INVALID, "clh_solver_step: null pos, vel, mass or jacobi_diag");
+    if (a->d_stretch_k && (!a->d_tris || !a->d_Dm_inv || !a->d_rest_area))
+        return fail(CLH_ERR_INVALID, "clh_solver_step: stretch enabled with null triangle data");
+    if (a->d_bend_quads && (!a->d_bend_w || !a->d_bend_n || !a->d_bend_k))
+        return fail(CLH_ERR_INVALID, "clh_solver_step: bending enabled with null hinge data");
+    if (a->num_pinned < 0 || (a->num_pinned > 0 && (!a->d_pin_idx || !a->d_pin_target)))
+        return fail(CLH_ERR_INVALID, "clh_solver_step: bad pin arguments");
+    return CLH_OK;
+}
+
+int step_impl(clh_solver* s, const clh_step_args& a)
+{
+    CLH_TRY(ensure_plan(s, a));
+    CLH_TRY(ensure_scratch(s));
+    if (a.num_pinned > 0 &&
+        (s->pins_checked_ptr != a.d_pin_idx || s->pins_checked_n != a.num_pinned)) {
+        // pin indices address the vertex arrays: check each new list once
+        std::vector<int> idx((size_t)a.num_pinned);
+        CLH_HIP(hipMemcpy(idx.data(), a.d_pin_idx, idx.size() * sizeof(int),
+                          hipMemcpyDeviceToHost));
+        for (int v : idx)
+            if (v < 0 || v >= s->N) return fail(CLH_ERR_INVALID, "pinned index out of range");
+        s->pins_checked_ptr = a.d_pin_idx;
+        s->pins_checked_n = a.num_pinned;
+    }
+    const clh_config& c = s->cfg;
+    const bool fixed = !(c.mode == CLH_MODE_PD && c.tolerance > 0.0f);
+
+    if (fixed && c.use_graph && !a.d_trace) {
+        if (!s->graph_valid || !same_args(s->graph_args, a)) {
+            if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+            if (s->graph) (void)hipGraphDestroy(s->graph);
+            s->graph_exec = nullptr;
+            s->graph = nullptr;
+            CLH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            const int st = enqueue_fixed_frame(s, a);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(s->stream, &g);
+            if (st != CLH_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            CLH_HIP(e);
+            s->graph = g;
+            CLH_HIP(hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
+            s->graph_args = a;
+            s->graph_valid = true;
+        }
+        CLH_HIP(hipGraphLaunch(s->graph_exec, s->stream));
+        s->last_iters = c.max_iterations;
+        s->last_checks = (c.mode == CLH_MODE_PD && c.max_iterations > 0) ? 1 : 0;
+        s->converged = 0;
+        return CLH_OK;
+    }
+    if (fixed) return enqueue_fixed_frame(s, a);
+
+    // PD mode with a tolerance: residual tests every check_every iterations,
+    // read back to decide whether to continue.
+    CLH_TRY(enqueue_prologue(s, a));
+    float omega = 1.0f;
+    int checks = 0, k = 0;
+    s->converged = 0;
+    const double tol2 = (double)c.tolerance * (double)c.tolerance;
+    for (; k < c.max_iterations; ++k) {
+        omega = chebyshev_omega(c, k, omega);
+        const bool check = is_check_iteration(c, k);
+        CLH_TRY(enqueue_iteration(s, a, k, omega, check, checks));
+        if (a.d_trace) CLH_TRY(enqueue_trace(s, a, k));
+        if (check) {
+            CLH_HIP(hipMemcpyAsync(s->h_hist + 2 * checks, s->hist.ptr + 2 * checks,
+                                   2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+            CLH_HIP(hipStreamSynchronize(s->stream));
+            const double r2 = s->h_hist[2 * checks], b2 = s->h_hist[2 * checks + 1];
+            ++checks;
+            if (!std::isfinite(r2)) return fail(CLH_ERR_NUMERIC, "non-finite residual");
+            if (r2 <= tol2 * b2) {
+                s->converged = 1;
+                ++k;
+                break;
+            }
+        }
+    }
+    s->last_iters = k;
+    s->last_checks = checks;
+    return enqueue_epilogue(s, a, k);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" int clh_config_default(clh_config* c)
+{
+    if (!c) return fail(CLH_ERR_INVALID, "clh_config_default: null pointer");
+    std::memset(c, 0, sizeof(*c));
+    c->mode = CLH_MODE_REF_CUDA;
+    c->max_iterations = 50;
+    c->use_chebyshev = 1;
+    c->rho = 0.9f;
+    c->gravity = -9.8f;
+    c->dt = 0.01f;
+    c->damping = 0.0f;
+    c->tolerance = 0.0f;
+    c->check_every = 10;
+    c->cheb_delay = 0;
+    c->kernel_variant = CLH_KERNEL_AUTO;
+    c->use_graph = 1;
+    return CLH_OK;
+}
+
+extern "C" int clh_solver_create(const clh_config* cfg, const clh_mesh_desc* mesh,
+                                 clh_solver** out)
+{
+    if (!cfg || !mesh || !out) return fail(CLH_ERR_INVALID, "clh_solver_create: null pointer");
+    *out = nullptr;
+    if (mesh->num_verts < 0 || mesh->num_tris < 0 || mesh->num_bend < 0)
+        return fail(CLH_ERR_INVALID, "clh_solver_create: negative count");
+    if (cfg->mode != CLH_MODE_REF_CUDA && cfg->mode != CLH_MODE_PD)
+        return fail(CLH_ERR_INVALID, "clh_solver_create: unknown mode");
+    if (cfg->max_iterations < 0 || cfg->max_iterations > kMaxIterations)
+        return fail(CLH_ERR_INVALID, "clh_solver_create: max_iterations out of range");
+    if (!(cfg->dt > 0.0f)) return fail(CLH_ERR_INVALID, "clh_solver_create: dt must be > 0");
+    if (cfg->kernel_variant < CLH_KERNEL_AUTO || cfg->kernel_variant > CLH_KERNEL_FUSED)
+        return fail(CLH_ERR_INVALID, "clh_solver_create: unknown kernel variant");
+    clh_solver* s = new clh_solver();
+    s->cfg = *cfg;
+    if (s->cfg.check_every <= 0) s->cfg.check_every = 1;
+    s->N = mesh->num_verts;
+    s->T = mesh->num_tris;
+    s->E = mesh->num_bend;
+    int st = hip_status(hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking),
+                        "hipStreamCreate");
+    if (st == CLH_OK) st = hip_status(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming), "hipEventCreate");
+    if (st == CLH_OK) st = hip_status(hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming), "hipEventCreate");
+    if (st == CLH_OK) st = hip_status(hipEventCreate(&s->ev_t0), "hipEventCreate");
+    if (st == CLH_OK) st = hip_status(hipEventCreate(&s->ev_t1), "hipEventCreate");
+    if (st != CLH_OK) {
+        clh_solver_destroy(s);
+        return st;
+    }
+    s->stream = s->own_stream;
+    if (mesh->d_tris || mesh->d_bend_quads) {
+        clh_step_args a = {};
+        a.d_tris = mesh->d_tris;
+        a.d_bend_quads = mesh->d_bend_quads;
+        st = ensure_plan(s, a);
+        if (st != CLH_OK) {
+            clh_solver_destroy(s);
+            return st;
+        }
+    }
+    *out = s;
+    return CLH_OK;
+}
+
+extern "C" int clh_solver_destroy(clh_solver* s)
+{
+    if (!s) return CLH_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    for (hipEvent_t e : {s->ev_in, s->ev_out, s->ev_t0, s->ev_t1})
+        if (e) (void)hipEventDestroy(e);
+    if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
+    if (s->h_hist) (void)hipHostFree(s->h_hist);
+    fused_plan_release(s->fused);
+    delete s;
+    return CLH_OK;
+}
+
+extern "C" int clh_solver_reset(clh_solver* s)
+{
+    if (!s) return fail(CLH_ERR_INVALID, "clh_solver_reset: null solver");
+    CLH_HIP(hipStreamSynchronize(s->stream));
+    s->plan_ready = false;
+    s->graph_valid = false;
+    s->pins_checked_ptr = nullptr;
+    s->pins_checked_n = -1;
+    s->frames = 0;
+    return CLH_OK;
+}
+
+extern "C" int clh_solver_set_stream(clh_solver* s, void* stream)
+{
+    if (!s) return fail(CLH_ERR_INVALID, "clh_solver_set_stream: null solver");
+    CLH_HIP(hipStreamSynchronize(s->stream));
+    s->user_stream = (stream != nullptr);
+    s->stream = stream ? static_cast<hipStream_t>(stream) : s->own_stream;
+    s->graph_valid = false;
+    return CLH_OK;
+}
+
+extern "C" int clh_solver_step(clh_solver* s, const clh_step_args* a)
+{
+    if (!s) return fail(CLH_ERR_INVALID, "clh_solver_step: null solver");
+    CLH_TRY(validate_args(s, a));
+    if (!s->user_stream) {
+        // order after earlier work on the null stream, and make later
+        // null-stream work (blocking copies) wait for this frame
+        CLH_HIP(hipEventRecord(s->ev_in, 0));
+        CLH_HIP(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+    }
+    CLH_TRY(step_impl(s, *a));
+    if (!s->user_stream) {
+        CLH_HIP(hipEventRecord(s->ev_out, s->stream));
+        CLH_HIP(hipStreamWaitEvent(0, s->ev_out, 0));
+    }
+    ++s->frames;
+    return CLH_OK;
+}
+
+extern "C" int clh_solver_get_stats(clh_solver* s, clh_step_stats* out)
+{
+    if (!s || !out) return fail(CLH_ERR_INVALID, "clh_solver_get_stats: null pointer");
+    CLH_HIP(hipStreamSynchronize(s->stream));
+    std::memset(out, 0, sizeof(*out));
+    out->iterations = s->last_iters;
+    out->converged = s->converged;
+    out->frames = s->frames;
+    if (s->cfg.mode == CLH_MODE_PD && s->last_checks > 0 && s->hist.ptr) {
+        std::vector<double> h((size_t)s->last_checks * 2);
+        CLH_HIP(hipMemcpy(h.data(), s->hist.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        out->residual0 = std::sqrt(h[0]);
+        out->residual = std::sqrt(h[h.size() - 2]);
+        out->rhs_norm = std::sqrt(h[h.size() - 1]);
+    }
+    return CLH_OK;
+}
+
+extern "C" int clh_solver_residual_history(clh_solver* s, double* out, int max_count, int* count)
+{
+    if (!s || !count) return fail(CLH_ERR_INVALID, "clh_solver_residual_history: null pointer");
+    CLH_HIP(hipStreamSynchronize(s->stream));
+    *count = (s->cfg.mode == CLH_MODE_PD) ? s->last_checks : 0;
+    const int n = std::min(*count, std::max(max_count, 0));
+    if (n > 0 && out) {
+        std::vector<double> h((size_t)n * 2);
+        CLH_HIP(hipMemcpy(h.data(), s->hist.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i)
+            out[i] = h[2 * i + 1] > 0.0 ? std::sqrt(h[2 * i] / h[2 * i + 1]) : 0.0;
+    }
+    return CLH_OK;
+}
+
+extern "C" int clh_solver_run_frames(clh_solver* s, const clh_step_args* a, int num_frames,
+                                     float* elapsed_ms)
+{
+    if (!s || !elapsed_ms || num_frames <= 0)
+        return fail(CLH_ERR_INVALID, "clh_solver_run_frames: bad arguments");
+    CLH_TRY(validate_args(s, a));
+    CLH_HIP(hipDeviceSynchronize());
+    CLH_HIP(hipEventRecord(s->ev_t0, s->stream));
+    for (int f = 0; f < num_frames; ++f) {
+        CLH_TRY(step_impl(s, *a));
+        ++s->frames;
+    }
+    CLH_HIP(hipEventRecord(s->ev_t1, s->stream));
+    CLH_HIP(hipEventSynchronize(s->ev_t1));
+    CLH_HIP(hipEventElapsedTime(elapsed_ms, s->ev_t0, s->ev_t1));
+    return CLH_OK;
+}
+
+extern "C" int clh_solver_time_iteration_kernel(clh_solver* s, const clh_step_args* a,
+                                                int num_launches, float* avg_ms)
+{
+    if (!s || !avg_ms || num_launches <= 0)
+        return fail(CLH_ERR_INVALID, "clh_solver_time_iteration_kernel: bad arguments");
+    CLH_TRY(validate_args(s, a));
+    CLH_TRY(ensure_plan(s, *a));
+    CLH_TRY(ensure_scratch(s));
+    // a fresh prologue leaves a consistent iterate in it[0]; the launches
+    // below re-run iteration 0 -> 1 without touching the mesh state
+    CLH_TRY(enqueue_prologue(s, *a));
+    CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, 0));  // warm-up
+    CLH_HIP(hipStreamSynchronize(s->stream));
+    CLH_HIP(hipEventRecord(s->ev_t0, s->stream));
+    for (int i = 0; i < num_launches; ++i) CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, 0));
+    CLH_HIP(hipEventRecord(s->ev_t1, s->stream));
+    CLH_HIP(hipEventSynchronize(s->ev_t1));
+    float ms = 0.0f;
+    CLH_HIP(hipEventElapsedTime(&ms, s->ev_t0, s->ev_t1));
+    *avg_ms = ms / (float)num_launches;
+    return CLH_OK;
+}
+
+extern "C" int clh_solver_plan_info(clh_solver* s, clh_plan_info* out)
+{
+    if (!s || !out) return fail(CLH_ERR_INVALID, "clh_solver_plan_info: null pointer");
+    std::memset(out, 0, sizeof(*out));
+    out->variant = s->plan_ready ? s->variant : CLH_KERNEL_AUTO;
+    if (s->plan_ready && s->variant == CLH_KERNEL_FUSED) fused_plan_info(s->fused, out);
+    return CLH_OK;
+}

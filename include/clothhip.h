@@ -1,0 +1,221 @@
+#pragma once
+/*
+ * clothhip.h — C ABI of libclothhip.so, the MI355X (gfx950) projective-
+ * dynamics cloth solver.
+ *
+ * Every entry point returns an int status (CLH_OK == 0); on failure
+ * clh_last_error() returns a message for the calling thread. Only plain
+ * pointers, sizes and POD structs cross this boundary. Device pointers are
+ * pointers returned by clh_alloc (or any other allocator of device memory).
+ *
+ * Each entry point names the reference interface it replaces
+ * (Chongyao/cloth-cuda, file:line).
+ */
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+    CLH_OK              = 0,
+    CLH_ERR_INVALID     = 1,  /* bad argument (null pointer, size, count) */
+    CLH_ERR_DEVICE      = 2,  /* the HIP runtime reported an error       */
+    CLH_ERR_ALLOC       = 3,  /* device allocation failed                 */
+    CLH_ERR_UNSUPPORTED = 4,  /* request outside what the library handles */
+    CLH_ERR_NUMERIC     = 5   /* non-finite state detected                */
+};
+
+/* ---- copy directions (same numbering as the runtime's memcpy kinds) -- */
+enum { CLH_H2H = 0, CLH_H2D = 1, CLH_D2H = 2, CLH_D2D = 3 };
+
+/* ---- solver modes ------------------------------------------------------
+ * CLH_MODE_REF_CUDA  reproduces PDSolver::step of the CUDA reference
+ *                    (src/pd_solver.cu:496-621): rhs = M y + h^2 sum w A^T p,
+ *                    x = rhs / diag, the 2-D rotation of the stretch
+ *                    projection as that kernel forms it (pd_solver.cu:184-194),
+ *                    no Chebyshev blend (pd_solver.cu:623-634).
+ * CLH_MODE_PD        projective dynamics with a true Jacobi sweep
+ *                    x <- x + D^-1 (b(x) - K x), the polar rotation of F,
+ *                    Chebyshev blending and an FP64 relative residual
+ *                    ||b - K x|| / ||b|| used for early termination.
+ */
+enum { CLH_MODE_REF_CUDA = 0, CLH_MODE_PD = 1 };
+
+/* ---- kernel variants --------------------------------------------------- */
+enum {
+    CLH_KERNEL_AUTO    = 0,  /* fused when available, else unfused          */
+    CLH_KERNEL_UNFUSED = 1,  /* element kernels + per-vertex gather          */
+    CLH_KERNEL_FUSED   = 2   /* one LDS-tiled kernel per iteration           */
+};
+
+/* ---- runtime ------------------------------------------------------------ */
+
+/* Message describing the last failure on this thread ("" if none). */
+const char* clh_last_error(void);
+
+/* Replaces cuda_device_available() (src/utils/cuda_helper.h:19-22). */
+int clh_device_count(int* count);
+
+typedef struct clh_device_info {
+    char   name[256];
+    char   arch[64];          /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+    int    compute_units;
+    int    clock_khz;
+    int    mem_clock_khz;
+    int    mem_bus_width;
+    size_t total_mem;
+    size_t l2_cache_bytes;
+} clh_device_info;
+
+/* Replaces print_device_info() (src/utils/cuda_helper.h:24-37). */
+int clh_get_device_info(int device, clh_device_info* out);
+
+/* Device memory used by the upload / free bodies of ClothMesh, SimConstraints
+ * and Constraints (src/cloth_mesh.cpp:18-83, src/sim_constraints.cpp:187-247,
+ * src/constraints.cpp:77-101), which call cudaMalloc / cudaMemcpy /
+ * cudaMemset / cudaFree directly. clh_memcpy and clh_memset are blocking with
+ * respect to the host, like the runtime calls they replace. */
+int clh_alloc(void** ptr, size_t bytes);
+int clh_free(void* ptr);
+int clh_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int clh_memset(void* ptr, int value, size_t bytes);
+int clh_synchronize(void);
+
+/* pos[idx[i]] = target[i]; vel[idx[i]] = 0 for i < num_pinned.
+ * Replaces launch_apply_constraints_kernel (include/pd_solver.h:74-77,
+ * src/pd_solver.cu:414-439), the kernel behind Constraints::apply_gpu
+ * (src/constraints.cpp:112-119). Runs on the null stream. */
+int clh_apply_constraints(float* d_pos, float* d_vel, const int* d_indices,
+                          const float* d_target, int num_pinned);
+
+/* ---- solver ------------------------------------------------------------- */
+
+typedef struct clh_solver clh_solver;
+
+/* Mirrors PDSolverConfig (include/pd_solver.h:4-15) plus appended fields. */
+typedef struct clh_config {
+    int   mode;             /* CLH_MODE_*                                       */
+    int   max_iterations;   /* local-global iterations per frame                */
+    int   use_chebyshev;    /* CLH_MODE_PD: blend with Chebyshev weights        */
+    float rho;              /* spectral radius estimate for the weights         */
+    float gravity;          /* acceleration along component 1 (the y axis)      */
+    float dt;               /* time step                                        */
+    float damping;          /* velocity damping in [0, 1]                       */
+    float tolerance;        /* CLH_MODE_PD: stop at ||r||/||b|| <= tolerance; 0 = fixed iterations */
+    int   check_every;      /* CLH_MODE_PD: residual test period (iterations)   */
+    int   cheb_delay;       /* CLH_MODE_PD: iterations before blending starts   */
+    float ext_accel[3];     /* uniform external acceleration (wind), m/s^2      */
+    int   kernel_variant;   /* CLH_KERNEL_*                                     */
+    int   use_graph;        /* capture fixed-length frames in a hipGraph        */
+} clh_config;
+
+/* Fills *cfg with the defaults of PDSolverConfig (include/pd_solver.h:5-14)
+ * and CLH_MODE_REF_CUDA. */
+int clh_config_default(clh_config* cfg);
+
+/* Topology fixed for the lifetime of a solver. The library copies what it
+ * needs; the pointers need not outlive the call. */
+typedef struct clh_mesh_desc {
+    int        num_verts;
+    int        num_tris;
+    int        num_bend;
+    const int* d_tris;        /* [T*3] device */
+    const int* d_bend_quads;  /* [E*4] device, may be NULL when num_bend == 0 */
+} clh_mesh_desc;
+
+/* Replaces PDSolver::PDSolver (src/pd_solver.cu:445-460), which caches N, T
+ * and E and allocates the solver scratch (pd_solver.cu:470-482). */
+int clh_solver_create(const clh_config* cfg, const clh_mesh_desc* mesh,
+                      clh_solver** out);
+/* Replaces PDSolver::~PDSolver (src/pd_solver.cu:462-468). */
+int clh_solver_destroy(clh_solver* s);
+/* Replaces PDSolver::reset (src/pd_solver.cu:491-494). Also drops cached
+ * per-element constants so that values changed in place are re-read. */
+int clh_solver_reset(clh_solver* s);
+
+/* Per-frame arguments; the same arrays PDSolver::step reads from ClothMesh,
+ * SimConstraints and Constraints (src/pd_solver.cu:496-621). A NULL
+ * d_stretch_k disables stretch and a NULL d_bend_quads disables bending,
+ * as in the reference (pd_solver.cu:534, 542). */
+typedef struct clh_step_args {
+    float*       d_pos;          /* [N*3] in/out                                */
+    float*       d_vel;          /* [N*3] in/out                                */
+    const float* d_mass;         /* [N]                                         */
+    const int*   d_tris;         /* [T*3]                                       */
+    const float* d_Dm_inv;       /* [T*4] column-major [m00, m10, m01, m11]     */
+    const float* d_rest_area;    /* [T]                                         */
+    const float* d_stretch_k;    /* [T]                                         */
+    const float* d_jacobi_diag;  /* [N]                                         */
+    const int*   d_bend_quads;   /* [E*4]                                       */
+    const float* d_bend_w;       /* [E*4]                                       */
+    const float* d_bend_n;       /* [E]                                         */
+    const float* d_bend_k;       /* [E]                                         */
+    const int*   d_pin_idx;      /* [P]                                         */
+    const float* d_pin_target;   /* [P*3] compact targets                       */
+    int          num_pinned;
+    const float* d_init_guess;   /* optional [N*3] first iterate (warm start); NULL = x_n */
+    float*       d_trace;        /* optional [max_iterations*N*3] every iterate */
+} clh_step_args;
+
+/* Replaces PDSolver::step (src/pd_solver.cu:496-621). Work is enqueued on the
+ * solver's stream (the null stream unless clh_solver_set_stream was called),
+ * so a following blocking copy observes the result, as in the reference. */
+int clh_solver_step(clh_solver* s, const clh_step_args* a);
+
+typedef struct clh_step_stats {
+    int    iterations;   /* iterations run in the last frame                */
+    int    converged;    /* CLH_MODE_PD with tolerance > 0: reached it       */
+    double residual;     /* last FP64 ||b - K x|| (CLH_MODE_PD)              */
+    double residual0;    /* first FP64 ||b - K x|| of the frame              */
+    double rhs_norm;     /* FP64 ||b|| at the last residual evaluation        */
+    int    frames;       /* frames stepped since creation or reset           */
+} clh_step_stats;
+
+/* Synchronizes the solver's stream and returns statistics of the last frame. */
+int clh_solver_get_stats(clh_solver* s, clh_step_stats* out);
+/* Copies up to max_count FP64 relative residuals ||r_k||/||b_k|| of the last
+ * frame (one per residual evaluation) into host memory; *count gets the
+ * number available. */
+int clh_solver_residual_history(clh_solver* s, double* out, int max_count,
+                                int* count);
+/* Sets the stream (a hipStream_t) that step and run enqueue on. */
+int clh_solver_set_stream(clh_solver* s, void* stream);
+
+/* ---- measurement ---------------------------------------------------------- */
+
+/* Steps num_frames frames back to back on the solver's stream and returns the
+ * elapsed device time between events recorded before and after them. */
+int clh_solver_run_frames(clh_solver* s, const clh_step_args* a,
+                          int num_frames, float* elapsed_ms);
+
+/* Launches the dominant per-iteration kernel num_launches times back to back
+ * on the solver's stream on the state left by the last frame, bracketed by
+ * events, and returns the average time per launch. The mesh state is restored
+ * afterwards. */
+int clh_solver_time_iteration_kernel(clh_solver* s, const clh_step_args* a,
+                                     int num_launches, float* avg_ms);
+
+/* Bytes the plan of the fused kernel reads per iteration beyond the
+ * algorithmic count (halo indices, adjacency, duplicated halo elements), and
+ * the tile statistics, for the roofline report. */
+typedef struct clh_plan_info {
+    int    variant;          /* CLH_KERNEL_* actually used */
+    int    num_tiles;
+    int    tile_verts;
+    double halo_tri_factor;  /* triangles processed / T */
+    double halo_bend_factor; /* hinges processed / E    */
+    double bytes_per_iter;   /* bytes the kernel streams per iteration (model) */
+} clh_plan_info;
+int clh_solver_plan_info(clh_solver* s, clh_plan_info* out);
+
+/* Streams bytes of device memory through a copy kernel num_launches times and
+ * returns the average time per launch (measured copy peak). */
+int clh_measure_copy(void* d_dst, const void* d_src, size_t bytes,
+                     int num_launches, float* avg_ms);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif

@@ -1,0 +1,45 @@
+"""Shared helpers of the test suite: building scenes through the C++ API and
+running the same frame on the HIP solver (C ABI) and on the oracle."""
+import numpy as np
+
+import clothhip as ch
+import pyoracle as po
+
+ARRAYS = ["pos0", "rest_pos", "tris", "Dm_inv", "rest_area", "mass", "stretch_k", "jacobi_diag",
+          "bend_quads", "bend_w", "bend_n", "bend_k", "pin_idx", "pin_target"]
+
+# The five workloads (reference axes: the cloth lies in x-z, gravity on y;
+# a z-up "wind along +y" is reference +z). Wind 0.5 N/m^2 over 0.1 kg/m^2
+# is an acceleration of 5 m/s^2.
+CFG1 = dict(nrows=32, ncols=32, size=1 / 31, type=0, seed=1, density=0.1, stretch_k=1e3,
+            bend_k=1e-2, pin_mode=ch.PIN_CORNERS, pos_jitter=1e-4, dt=1 / 60)
+CFG2 = dict(nrows=256, ncols=256, size=1 / 255, type=0, seed=2, density=0.1, stretch_k=1e3,
+            bend_k=1e-2, pin_mode=ch.PIN_TOP_ROW, pos_jitter=1e-3, dt=1 / 60)
+CFG3 = dict(nrows=1024, ncols=1024, size=1 / 1023, type=ch.MESH_IRREGULAR, rest_jitter=0.3,
+            seed=3, density=0.1, stretch_k=5e2, stretch_k_max=5e3, bend_k=1e-2,
+            pin_mode=ch.PIN_TOP_ROW, pos_jitter=0.0, dt=1 / 60)
+CFG2_WIND = (0.0, 0.0, 5.0)
+
+
+def scene_arrays(scene):
+    out = {}
+    for k in ARRAYS:
+        out[k] = scene.get(k)
+    return out
+
+
+def oracle_mesh(arrays, stretch=True, bend=True):
+    return po.Mesh(arrays, stretch=stretch, bend=bend)
+
+
+def rel_err(a, b, scale=None):
+    """max |a - b| over the scale (default: max |b|, at least 1)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    s = scale if scale is not None else max(np.abs(b).max(), 1.0)
+    return float(np.abs(a - b).max() / s)
+
+
+def extent(pos):
+    pos = np.asarray(pos).reshape(-1, 3)
+    return float(np.max(pos.max(0) - pos.min(0)))

@@ -1,0 +1,334 @@
+"""Parity of the HIP solver (through the C ABI of libclothhip) with the
+oracle on the same seeded inputs: per iteration in reference mode, per
+iteration and per frame in PD mode, multi-frame against FP64, at the
+workloads' full sizes, plus determinism and edge cases."""
+import numpy as np
+import pytest
+
+import clothhip as ch
+import pyoracle as po
+from helpers import CFG1, CFG2, CFG2_WIND, CFG3, extent, oracle_mesh, rel_err, scene_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def make(spec, **kw):
+    d = dict(spec)
+    d.update(kw)
+    s = ch.Scene(**d)
+    s.upload()
+    return s, scene_arrays(s)
+
+
+def solver_for(s, **cfg):
+    c = ch.config(**cfg)
+    return ch.Solver(c, s.N, s.T, s.E)
+
+
+def gpu_frame(s, solver, args, trace_iters=0):
+    tr = None
+    if trace_iters:
+        tr = ch.DeviceArray(trace_iters * s.N * 3 * 4)
+        args.d_trace = tr.ptr
+    solver.step(args)
+    pos, vel = s.get_state()
+    out = tr.download(np.float32, (trace_iters, s.N, 3)) if tr else None
+    args.d_trace = None
+    return pos, vel, out
+
+
+def test_device_is_gfx950():
+    assert ch.device_count() >= 1
+    info = ch.device_info(0)
+    assert info.arch.decode().startswith("gfx950")
+
+
+def test_apply_constraints():
+    n = 50
+    pos = np.random.default_rng(0).normal(size=(n, 3)).astype(np.float32)
+    vel = np.ones((n, 3), np.float32)
+    idx = np.array([3, 17, 42], np.int32)
+    tgt = np.arange(9, dtype=np.float32).reshape(3, 3)
+    dp, dv = ch.DeviceArray.from_host(pos), ch.DeviceArray.from_host(vel)
+    di, dt = ch.DeviceArray.from_host(idx), ch.DeviceArray.from_host(tgt)
+    ch.check(ch.hip().clh_apply_constraints(dp.ptr, dv.ptr, di.ptr, dt.ptr, 3), "apply")
+    ch.check(ch.hip().clh_synchronize(), "sync")
+    p2, v2 = dp.download(np.float32, (n, 3)), dv.download(np.float32, (n, 3))
+    exp_p, exp_v = pos.copy(), vel.copy()
+    exp_p[idx] = tgt
+    exp_v[idx] = 0
+    np.testing.assert_array_equal(p2, exp_p)
+    np.testing.assert_array_equal(v2, exp_v)
+
+
+REF_CASES = {
+    "cfg1": dict(CFG1),
+    "irregular16": dict(nrows=16, ncols=16, size=1 / 15, type=ch.MESH_IRREGULAR, rest_jitter=0.3,
+                        seed=11, stretch_k=5e2, stretch_k_max=5e3, bend_k=1e-2,
+                        pin_mode=ch.PIN_TOP_ROW, pos_jitter=1e-3, dt=1 / 60),
+    "cross10": dict(nrows=10, ncols=10, size=0.1, type=3, stretch_k=100.0, bend_k=1.0,
+                    pin_mode=ch.PIN_CORNERS, pos_jitter=1e-3, dt=0.01),
+}
+
+
+@pytest.mark.parametrize("case", list(REF_CASES))
+def test_ref_mode_per_iteration(case):
+    """Reference mode, iteration by iteration from identical inputs: the HIP
+    iterate k+1 against the oracle's one-iteration update of the HIP iterate
+    k (the reference's dynamics amplify rounding, so errors must not be
+    allowed to accumulate before comparing). Tolerance 1e-5 relative."""
+    spec = REF_CASES[case]
+    s, arr = make(spec)
+    K = 12
+    dt = spec["dt"]
+    sol = solver_for(s, mode=ch.MODE_REF_CUDA, max_iterations=K, gravity=-9.81, dt=dt,
+                     use_graph=0)
+    pos0, vel0 = s.get_state()
+    vel0 = np.random.default_rng(1).normal(scale=0.05, size=vel0.shape).astype(np.float32)
+    s.set_state(pos0, vel0)
+    _, _, tr = gpu_frame(s, sol, s.step_args(), trace_iters=K)
+    m = oracle_mesh(arr)
+    p = po.params(mode=po.REF_CUDA, iterations=1, gravity=-9.81, dt=dt)
+    prev = pos0
+    for k in range(K):
+        _, _, otr, _, _ = po.step(m, p, pos0, vel0, "f32", trace=True, init=prev)
+        err = rel_err(tr[k], otr[0], scale=max(extent(pos0), 1.0))
+        assert err < 1e-5, (case, k, err)
+        prev = tr[k]
+
+
+def pd_cfg(K, dt, **kw):
+    c = dict(mode=ch.MODE_PD, max_iterations=K, gravity=-9.81, dt=dt, rho=0.99, cheb_delay=1)
+    c.update(kw)
+    return c
+
+
+def test_pd_mode_per_iteration_cfg1():
+    """PD mode: every iterate of one frame against the FP32 oracle started
+    from the same state (1e-5 relative) and the FP64 oracle (1e-5)."""
+    s, arr = make(CFG1)
+    K = 20
+    sol = solver_for(s, **pd_cfg(K, CFG1["dt"], use_graph=0))
+    pos0, vel0 = s.get_state()
+    _, _, tr = gpu_frame(s, sol, s.step_args(), trace_iters=K)
+    m = oracle_mesh(arr)
+    p = po.params(mode=po.PD, iterations=K, gravity=-9.81, dt=CFG1["dt"], rho=0.99, cheb_delay=1)
+    _, _, t32, _, _ = po.step(m, p, pos0, vel0, "f32", trace=True)
+    _, _, t64, _, _ = po.step(m, p, pos0, vel0, "f64", trace=True)
+    L = extent(pos0)
+    for k in range(K):
+        assert rel_err(tr[k], t32[k], L) < 1e-5, k
+        assert rel_err(tr[k], t64[k], L) < 1e-5, k
+
+
+def test_pd_mode_200_frames_vs_fp64():
+    """Config 1 for 200 frames (10 iterations, Chebyshev rho 0.99): the HIP
+    trajectory within 1e-4 of the cloth extent of the FP64 oracle's
+    independent trajectory; each frame within 1e-5 of the FP32 oracle's frame
+    from the same start."""
+    s, arr = make(CFG1)
+    sol = solver_for(s, **pd_cfg(10, CFG1["dt"]))
+    m = oracle_mesh(arr)
+    p = po.params(mode=po.PD, iterations=10, gravity=-9.81, dt=CFG1["dt"], rho=0.99, cheb_delay=1)
+    pos, vel = s.get_state()
+    x64, v64 = pos.astype(np.float64), vel.astype(np.float64)
+    args = s.step_args()
+    worst_frame = 0.0
+    for f in range(200):
+        if f % 20 == 0:
+            o32, *_ = po.step(m, p, pos, vel, "f32")
+        sol.step(args)
+        npos, nvel = s.get_state()
+        if f % 20 == 0:
+            worst_frame = max(worst_frame, rel_err(npos, o32, extent(pos)))
+        pos, vel = npos, nvel
+        x64, v64, *_ = po.step(m, p, x64, v64, "f64")
+    assert worst_frame < 1e-5
+    assert np.abs(pos - x64).max() < 1e-4 * extent(x64)
+
+
+def test_deterministic_and_graph_equivalent():
+    s, arr = make(CFG1)
+    outs = []
+    for use_graph in (1, 1, 0):
+        s.set_state(arr["pos0"], np.zeros_like(arr["pos0"]))
+        sol = solver_for(s, **pd_cfg(15, CFG1["dt"], use_graph=use_graph))
+        args = s.step_args()
+        for _ in range(5):
+            sol.step(args)
+        outs.append(s.get_state())
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o[0], outs[0][0])
+        np.testing.assert_array_equal(o[1], outs[0][1])
+
+
+def test_cpp_pdsolver_matches_c_abi():
+    """The C++ PDSolver class (include/pd_solver.h) and a direct clh_solver
+    give bitwise-identical frames."""
+    for mode in (ch.MODE_REF_CUDA, ch.MODE_PD):
+        s, arr = make(CFG1)
+        cfg = ch.pd_config(max_iterations=10, gravity=-9.81, dt=CFG1["dt"], rho=0.99,
+                           cheb_delay=1, mode=mode, tolerance=0.0)
+        cpp = ch.PDSolver(s, cfg)
+        for _ in range(3):
+            cpp.step()
+        a = s.get_state()
+        s.set_state(arr["pos0"], np.zeros_like(arr["pos0"]))
+        sol = solver_for(s, mode=mode, max_iterations=10, gravity=-9.81, dt=CFG1["dt"], rho=0.99,
+                         cheb_delay=1)
+        args = s.step_args()
+        for _ in range(3):
+            sol.step(args)
+        b = s.get_state()
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_free_fall_and_pins():
+    s, arr = make(CFG1, stretch_k=0.0, bend_k=0.0)
+    rng = np.random.default_rng(5)
+    vel = rng.normal(size=(s.N, 3)).astype(np.float32)
+    pos = arr["pos0"]
+    s.set_state(pos, vel)
+    for mode in (ch.MODE_PD, ch.MODE_REF_CUDA):
+        s.set_state(pos, vel)
+        sol = solver_for(s, mode=mode, max_iterations=5, gravity=-9.81, dt=CFG1["dt"])
+        sol.step(s.step_args())
+        x, v = s.get_state()
+        h = np.float32(CFG1["dt"])
+        y = pos + h * vel + (h * h) * np.array([0, np.float32(-9.81), 0], np.float32)
+        free = np.ones(s.N, bool)
+        free[arr["pin_idx"]] = False
+        np.testing.assert_allclose(x[free], y[free], rtol=0, atol=4e-7)
+        np.testing.assert_array_equal(x[arr["pin_idx"]], arr["pin_target"])
+        np.testing.assert_array_equal(v[arr["pin_idx"]], 0)
+
+
+def test_no_pins_no_bend_no_stretch_paths():
+    s, arr = make(CFG1, pin_mode=ch.PIN_NONE, bend_k=0.0)
+    m = oracle_mesh(arr, bend=False)
+    pos, vel = s.get_state()
+    sol = solver_for(s, **pd_cfg(8, CFG1["dt"]))
+    sol.step(s.step_args(bend=False))
+    x, _ = s.get_state()
+    o, *_ = po.step(m, po.params(mode=po.PD, iterations=8, gravity=-9.81, dt=CFG1["dt"], rho=0.99,
+                                 cheb_delay=1), pos, vel, "f32")
+    assert rel_err(x, o, extent(pos)) < 1e-5
+    # stretch disabled through a null stiffness pointer (Test 3 style)
+    s2, arr2 = make(CFG1, stretch_k=0.0, bend_k=1.0, pin_mode=ch.PIN_NONE)
+    m2 = oracle_mesh(arr2, stretch=False)
+    pos, vel = s2.get_state()
+    sol2 = solver_for(s2, mode=ch.MODE_REF_CUDA, max_iterations=8, gravity=0.0, dt=CFG1["dt"])
+    sol2.step(s2.step_args(stretch=False))
+    x, _ = s2.get_state()
+    o, *_ = po.step(m2, po.params(mode=po.REF_CUDA, iterations=8, gravity=0.0, dt=CFG1["dt"]),
+                    pos, vel, "f32")
+    assert rel_err(x, o, extent(pos)) < 1e-5
+
+
+def test_smallest_mesh():
+    s, arr = make(dict(CFG1, nrows=2, ncols=2, size=1.0, pin_mode=ch.PIN_NONE))
+    assert (s.N, s.T) == (4, 2)
+    m = oracle_mesh(arr)
+    pos, vel = s.get_state()
+    sol = solver_for(s, **pd_cfg(5, CFG1["dt"]))
+    sol.step(s.step_args())
+    x, _ = s.get_state()
+    o, *_ = po.step(m, po.params(mode=po.PD, iterations=5, gravity=-9.81, dt=CFG1["dt"], rho=0.99,
+                                 cheb_delay=1), pos, vel, "f32")
+    assert rel_err(x, o, 1.0) < 1e-6
+
+
+def test_tolerance_mode_reports_fp64_residual():
+    """PD mode with a tolerance stops once ||r||/||b|| (FP64 partial sums on
+    the GPU) is below it; the reported residual matches the FP64 oracle's
+    evaluation of the same iterate."""
+    s, arr = make(CFG1)
+    tol = 2e-4
+    sol = solver_for(s, **pd_cfg(400, CFG1["dt"], tolerance=tol, check_every=1))
+    pos, vel = s.get_state()
+    sol.step(s.step_args())
+    st = sol.stats()
+    hist = sol.residual_history()
+    assert st.converged == 1 and 1 <= st.iterations < 400
+    assert hist[-1] <= tol and len(hist) == st.iterations
+    # the FP32 oracle stops at the same iteration
+    m = oracle_mesh(arr)
+    p = po.params(mode=po.PD, iterations=400, gravity=-9.81, dt=CFG1["dt"], rho=0.99,
+                  cheb_delay=1, tolerance=tol, check_every=1)
+    _, _, _, rs, it = po.step(m, p, pos, vel, "f32", resid=True)
+    assert abs(it - st.iterations) <= 1
+    np.testing.assert_allclose(hist[:5], np.sqrt(rs[:5, 0] / rs[:5, 1]), rtol=1e-3)
+
+
+def test_invalid_indices_rejected():
+    s, arr = make(CFG1)
+    bad = arr["tris"].copy()
+    bad[7, 1] = s.N + 3
+    dbad = ch.DeviceArray.from_host(bad)
+    sol = solver_for(s, **pd_cfg(3, CFG1["dt"]))
+    args = s.step_args()
+    args.d_tris = dbad.ptr
+    with pytest.raises(ch.ClothHipError, match="out of range"):
+        sol.step(args)
+
+
+def test_cfg2_full_size_frame():
+    """256x256 (config 2) one frame, 20 iterations with wind: against the FP32
+    oracle (1e-5 relative) and the FP64 oracle (1e-5 of extent)."""
+    s, arr = make(CFG2)
+    sol = solver_for(s, **pd_cfg(20, CFG2["dt"], ext_accel=CFG2_WIND))
+    pos, vel = s.get_state()
+    sol.step(s.step_args())
+    x, v = s.get_state()
+    m = oracle_mesh(arr)
+    p = po.params(mode=po.PD, iterations=20, gravity=-9.81, dt=CFG2["dt"], rho=0.99,
+                  cheb_delay=1, ext_accel=CFG2_WIND, threads=8)
+    o32, *_ = po.step(m, p, pos, vel, "f32")
+    o64, *_ = po.step(m, p, pos, vel, "f64")
+    assert rel_err(x, o32, extent(pos)) < 1e-5
+    assert rel_err(x, o64, extent(pos)) < 1e-5
+
+
+def test_cfg3_full_size_frame():
+    """1024x1024 irregular mesh (config 3), 3 iterations of PD mode and one
+    of reference mode against the FP32 oracle; size-independent: pins exact,
+    velocities consistent with positions."""
+    s, arr = make(CFG3)
+    assert s.N == 1024 * 1024 and s.T == 2 * 1023 * 1023
+    pos, vel = s.get_state()
+    m = oracle_mesh(arr)
+    sol = solver_for(s, **pd_cfg(3, CFG3["dt"]))
+    sol.step(s.step_args())
+    x, v = s.get_state()
+    p = po.params(mode=po.PD, iterations=3, gravity=-9.81, dt=CFG3["dt"], rho=0.99, cheb_delay=1,
+                  threads=8)
+    o32, *_ = po.step(m, p, pos, vel, "f32")
+    assert rel_err(x, o32, extent(pos)) < 1e-5
+    np.testing.assert_array_equal(x[arr["pin_idx"]], arr["pin_target"])
+    h = np.float32(CFG3["dt"])
+    free = np.ones(s.N, bool)
+    free[arr["pin_idx"]] = False
+    np.testing.assert_allclose(v[free] * h, (x - pos)[free], atol=2e-7)
+    s.set_state(pos, vel)
+    sol_r = solver_for(s, mode=ch.MODE_REF_CUDA, max_iterations=1, gravity=-9.81, dt=CFG3["dt"])
+    sol_r.step(s.step_args())
+    xr, _ = s.get_state()
+    orr, *_ = po.step(m, po.params(mode=po.REF_CUDA, iterations=1, gravity=-9.81, dt=CFG3["dt"],
+                                   threads=8), pos, vel, "f32")
+    assert rel_err(xr, orr, extent(pos)) < 1e-5
+
+
+def test_reference_bounds_through_cpp_api():
+    """The reference's Test 1 (test_pd_constraints.cpp:151-187) through the C++
+    PDSolver in reference mode on the GPU."""
+    s, arr = make(dict(nrows=10, ncols=10, size=0.1, type=3, stretch_k=100.0, bend_k=1.0,
+                       pin_mode=ch.PIN_CORNERS, pos_jitter=0.0, dt=0.01))
+    s.set_state(arr["rest_pos"], np.zeros_like(arr["rest_pos"]))
+    cfg = ch.pd_config(dt=0.01, gravity=0.0, max_iterations=100)
+    sol = ch.PDSolver(s, cfg)
+    for _ in range(100):
+        sol.step()
+    x, v = s.get_state()
+    ke = 0.5 * (arr["mass"] * (v.astype(np.float64) ** 2).sum(1)).sum()
+    assert np.linalg.norm(v, axis=1).max() < 10 and ke < 10
