@@ -66,7 +66,7 @@ class StepStats(C.Structure):
 class PlanInfo(C.Structure):
     _fields_ = [("variant", C.c_int), ("num_tiles", C.c_int), ("tile_verts", C.c_int),
                 ("halo_tri_factor", C.c_double), ("halo_bend_factor", C.c_double),
-                ("bytes_per_iter", C.c_double)]
+                ("bytes_per_iter", C.c_double), ("lds_bytes", C.c_double)]
 
 
 class SceneSpec(C.Structure):
@@ -92,7 +92,7 @@ CLH_SYMBOLS = [
     "clh_config_default", "clh_solver_create", "clh_solver_destroy", "clh_solver_reset",
     "clh_solver_step", "clh_solver_get_stats", "clh_solver_residual_history",
     "clh_solver_set_stream", "clh_solver_run_frames", "clh_solver_time_iteration_kernel",
-    "clh_solver_plan_info", "clh_measure_copy",
+    "clh_solver_plan_info", "clh_measure_copy", "clh_fused_plan_check",
 ]
 
 _hip = None
@@ -129,6 +129,8 @@ def _load():
     hip.clh_solver_time_iteration_kernel.argtypes = [_vp, C.POINTER(StepArgs), C.c_int, _fp]
     hip.clh_solver_plan_info.argtypes = [_vp, C.POINTER(PlanInfo)]
     hip.clh_measure_copy.argtypes = [_vp, _vp, C.c_size_t, C.c_int, _fp]
+    hip.clh_fused_plan_check.argtypes = [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int,
+                                         C.c_int, C.POINTER(PlanInfo)]
     hip.clh_get_device_info.argtypes = [C.c_int, C.POINTER(DeviceInfo)]
     hip.clh_device_count.argtypes = [_ip]
 
@@ -460,6 +462,19 @@ class PDSolver:
             self.close()
         except Exception:
             pass
+
+
+def fused_plan_check(N, tris, quads, pos, tile_verts=128, mode=MODE_PD) -> PlanInfo:
+    """Builds and validates the fused tile plan on the host (no device)."""
+    tris = np.ascontiguousarray(tris, np.int32)
+    quads = None if quads is None or len(quads) == 0 else np.ascontiguousarray(quads, np.int32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    info = PlanInfo()
+    check(hip().clh_fused_plan_check(N, len(tris), 0 if quads is None else len(quads),
+                                     _ptr(tris), None if quads is None else _ptr(quads),
+                                     _ptr(pos), tile_verts, mode, C.byref(info)),
+          "clh_fused_plan_check")
+    return info
 
 
 def measure_copy(nbytes: int, launches: int = 20) -> float:

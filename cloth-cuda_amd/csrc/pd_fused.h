@@ -1,5 +1,17 @@
 #pragma once
 // Fused LDS-tiled iteration kernel: plan and launch interface.
+//
+// Vertices are renumbered into tiles (Morton order of the positions at plan
+// time, cut into runs of tile_verts); every per-vertex array of the
+// iteration lives in that tile order. A tile owns its vertices and carries
+// a private copy of every triangle and hinge touching them, with local
+// (16-bit) vertex indices into its LDS position table (own vertices first,
+// then the halo). One workgroup per tile per iteration:
+//   1. stage own + halo positions in LDS,
+//   2. triangles -> corner forces in LDS, hinges -> bend vectors in LDS,
+//   3. each own vertex gathers its corners and hinges (ascending element
+//      id, the order of the unfused pipeline), then Jacobi / Chebyshev /
+//      pins and FP64 residual partials; x_{k+1} goes to HBM.
 #include "clh_internal.h"
 
 #include <vector>
@@ -7,16 +19,16 @@
 namespace clh {
 
 struct FusedIterArgs {
-    const float* x;        // x_k, original vertex order [N*3]
-    const float* xkm1;     // x_{k-1}
-    float*       out;      // x_{k+1}
-    const float* y;
-    const float* mass;
-    const float* diag;
-    const int*   pin_map;
+    const float* x;        // iterate k (tile order)
+    const float* xkm1;     // iterate k-1
+    float*       out;      // iterate k+1
+    const float* base;     // x_n (tile order; PD iterates are displacements)
+    const float* y;        // y (reference mode) or y - x_n (PD), tile order
+    const float* mass;     // tile order
+    const float* diag;     // tile order
+    const int*   pin_map;  // tile order -> pin slot or -1
     const float* pin_target;
     double*      partials; // per-tile |r|^2, |b|^2 when non-null
-    float h2;
     float omega;
     int   blend;
     int   use_stretch;
@@ -24,17 +36,49 @@ struct FusedIterArgs {
 };
 
 struct FusedPlan {
+    int N = 0;
     int num_tiles = 0;
     int tile_verts = 0;
+    int lmax = 0, tmax = 0, hmax = 0, vmax = 0;  // per-tile maxima (LDS sizing)
     double tri_factor = 0.0, bend_factor = 0.0, bytes_per_iter = 0.0;
+    int xcd_remap = 1;
+
+    DevBuf<int>   perm, inv_perm;             // tile order <-> original
+    DevBuf<int>   tile_info;                  // [num_tiles * 8]
+    DevBuf<int>   halo;                       // tile-order ids of halo vertices
+    DevBuf<int>   tri_gid, hin_gid;           // original element ids
+    DevBuf<uint2> tri_idx, hin_idx;           // packed u16 local vertex ids
+    DevBuf<float4> tri_c0, hin_w;             // Dm_inv | bend weights
+    DevBuf<float> tri_c1;                     // h^2 k A
+    DevBuf<float2> hin_c;                     // (n_rest, h^2 k)
+    DevBuf<int>   gl_ptr;                     // [N+1] gather list offsets (tile order)
+    DevBuf<unsigned short> gl_idx;            // local slots: tri*3+c, then hinge*4+i
+    DevBuf<unsigned char>  gl_ntri;           // [N] leading triangle entries
+
+    // packed-constant provenance (repack when any pointer changes)
+    const void* packed_src[8] = {};
+    float packed_h2 = -1.0f;
 };
 
 int  fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
-                      const std::vector<int>& quads, int mode);
+                      const std::vector<int>& quads, const std::vector<float>& pos,
+                      int tile_verts);
+int  fused_pack_constants(FusedPlan& p, const clh_step_args& a, float h2, hipStream_t stream);
 int  fused_max_blocks(const FusedPlan& p);
+size_t fused_lds_bytes(const FusedPlan& p, int mode, bool check);
 int  fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
                             hipStream_t stream);
 void fused_plan_info(const FusedPlan& p, clh_plan_info* out);
 void fused_plan_release(FusedPlan& p);
+
+// Frame prologue / epilogue in tile order (gather from and scatter to the
+// caller's original-order arrays).
+int fused_prologue(const FusedPlan& p, const clh_step_args& a, float* xprev, float* y, float* x0,
+                   float* mass, float* diag, int* pin_map, float dt, const float h2a[3],
+                   int disp, hipStream_t stream);
+int fused_epilogue(const FusedPlan& p, const clh_step_args& a, const float* xf, const float* xprev,
+                   const int* pin_map, float scale, int disp, hipStream_t stream);
+int fused_trace(const FusedPlan& p, const float* it, const float* xprev, float* out, int disp,
+                hipStream_t stream);
 
 }  // namespace clh

@@ -1,18 +1,831 @@
-// Fused LDS-tiled iteration kernel (not yet available: the unfused
-// pipeline is used).
+// Fused LDS-tiled iteration kernel: plan construction (host), constant
+// packing, the per-iteration kernel, and the tile-order frame prologue /
+// epilogue.
 #include "pd_fused.h"
+#include "pd_kernels.h"
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace clh {
+namespace {
+
+constexpr int kFB = 256;  // threads per fused workgroup
+constexpr int kTileInfo = 8;  // vbeg, nown, halo_beg, nhalo, tri_beg, ntri, hin_beg, nhin
+
+// ---------------------------------------------------------------------------
+// host: plan
+// ---------------------------------------------------------------------------
+
+uint64_t spread21(uint64_t v)
+{
+    v &= 0x1FFFFF;
+    v = (v | (v << 32)) & 0x1F00000000FFFFull;
+    v = (v | (v << 16)) & 0x1F0000FF0000FFull;
+    v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+    v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+// Vertex order: 3-D Morton code of the positions (21 bits per axis over the
+// bounding cube), ties by index.
+std::vector<int> morton_order(int N, const std::vector<float>& pos)
+{
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int v = 0; v < N; ++v)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], pos[(size_t)v * 3 + k]);
+            hi[k] = std::max(hi[k], pos[(size_t)v * 3 + k]);
+        }
+    // one scale for all axes (the largest extent), so that a nearly flat
+    // cloth orders by its two in-plane axes
+    double ext = 0.0;
+    for (int k = 0; k < 3; ++k) ext = std::max(ext, (double)hi[k] - (double)lo[k]);
+    std::vector<uint64_t> key((size_t)N);
+    for (int v = 0; v < N; ++v) {
+        uint64_t q[3];
+        for (int k = 0; k < 3; ++k) {
+            double f = ext > 0 ? ((double)pos[(size_t)v * 3 + k] - lo[k]) / ext : 0.0;
+            f = std::min(std::max(f, 0.0), 1.0);
+            q[k] = (uint64_t)(f * 2097151.0);
+        }
+        key[(size_t)v] = spread21(q[0]) | (spread21(q[1]) << 1) | (spread21(q[2]) << 2);
+    }
+    std::vector<int> order((size_t)N);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return key[(size_t)a] < key[(size_t)b]; });
+    return order;
+}
+
+void incidence(const std::vector<int>& elems, int arity, int N, std::vector<int>& ptr,
+               std::vector<int>& ids)
+{
+    const size_t ne = elems.size() / (size_t)arity;
+    ptr.assign((size_t)N + 1, 0);
+    for (size_t s = 0; s < elems.size(); ++s) ptr[(size_t)elems[s] + 1]++;
+    for (int v = 0; v < N; ++v) ptr[(size_t)v + 1] += ptr[(size_t)v];
+    ids.assign(ptr[(size_t)N], 0);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t e = 0; e < ne; ++e)
+        for (int c = 0; c < arity; ++c) {
+            const int v = elems[e * arity + c];
+            // one entry per element even if the vertex repeats in it
+            if (fill[(size_t)v] > ptr[(size_t)v] && ids[(size_t)fill[(size_t)v] - 1] == (int)e)
+                continue;
+            ids[(size_t)fill[(size_t)v]++] = (int)e;
+        }
+    // compact away the holes left by repeated vertices
+    std::vector<int> nptr((size_t)N + 1, 0), nids;
+    nids.reserve(ids.size());
+    for (int v = 0; v < N; ++v) {
+        for (int j = ptr[(size_t)v]; j < fill[(size_t)v]; ++j) nids.push_back(ids[(size_t)j]);
+        nptr[(size_t)v + 1] = (int)nids.size();
+    }
+    ptr.swap(nptr);
+    ids.swap(nids);
+}
+
+// vertex -> incident slots (element * arity + corner), ascending
+void build_slots(const std::vector<int>& elems, int arity, int N, std::vector<int>& ptr,
+                 std::vector<int>& ids)
+{
+    ptr.assign((size_t)N + 1, 0);
+    for (size_t s = 0; s < elems.size(); ++s) ptr[(size_t)elems[s] + 1]++;
+    for (int v = 0; v < N; ++v) ptr[(size_t)v + 1] += ptr[(size_t)v];
+    ids.assign(elems.size(), 0);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t s = 0; s < elems.size(); ++s) ids[(size_t)fill[(size_t)elems[s]]++] = (int)s;
+    (void)arity;
+}
+
+struct HostPlan {
+    FusedPlan meta;  // sizes only; device buffers unused
+    std::vector<int> perm, inv, info, halo, tri_gid, hin_gid, gl_ptr;
+    std::vector<uint2> tri_idx, hin_idx;
+    std::vector<unsigned short> gl_idx;
+    std::vector<unsigned char> gl_ntri;
+};
+
+// ---------------------------------------------------------------------------
+// device
+// ---------------------------------------------------------------------------
+
+struct FusedKernelArgs {
+    FusedIterArgs it;
+    const int*            tile_info;
+    const int*            halo;
+    const uint2*          tri_idx;
+    const float4*         tri_c0;
+    const float*          tri_c1;
+    const uint2*          hin_idx;
+    const float4*         hin_w;
+    const float2*         hin_c;
+    const int*            gl_ptr;
+    const unsigned short* gl_idx;
+    const unsigned char*  gl_ntri;
+    int lmax, tmax, hmax;
+    int num_tiles;
+    int xcd_remap;
+};
+
+__device__ __forceinline__ f3 lds3(const float* s, int i)
+{
+    return f3{s[3 * i], s[3 * i + 1], s[3 * i + 2]};
+}
+__device__ __forceinline__ void sts3(float* s, int i, f3 v)
+{
+    s[3 * i] = v.x;
+    s[3 * i + 1] = v.y;
+    s[3 * i + 2] = v.z;
+}
+
+__device__ void fused_block_sum2(double r, double b, double* out, int slot)
+{
+    __shared__ double sr[kFB / 64], sb[kFB / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        r += __shfl_down(r, off, 64);
+        b += __shfl_down(b, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        sr[wave] = r;
+        sb[wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tr = 0.0, tb = 0.0;
+        for (int i = 0; i < kFB / 64; ++i) {
+            tr += sr[i];
+            tb += sb[i];
+        }
+        out[2 * slot + 0] = tr;
+        out[2 * slot + 1] = tb;
+    }
+}
+
+// One local-global iteration of one tile. kMode selects the reference
+// update (absolute positions, rhs / diag) or PD (displacements, Jacobi +
+// Chebyshev); kCheck adds the right-hand-side norm for the FP64 residual.
+template <int kMode, bool kCheck>
+__global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
+{
+    constexpr bool PD = (kMode == CLH_MODE_PD);
+    extern __shared__ float smem[];
+    const FusedIterArgs& a = A.it;
+
+    int tile = blockIdx.x;
+    if (A.xcd_remap) {
+        // consecutive tiles (a spatial region) on one XCD: workgroup b runs
+        // on XCD b % 8
+        const int per = A.num_tiles >> 3;
+        tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    }
+    const int* ti = A.tile_info + (size_t)tile * kTileInfo;
+    const int vbeg = ti[0], nown = ti[1], hbeg = ti[2], nhalo = ti[3];
+    const int tbeg = ti[4], ntri = ti[5], ebeg = ti[6], nhin = ti[7];
+
+    float* sb = smem;                                  // base     [lmax*3] (PD)
+    float* sx = sb + (PD ? A.lmax * 3 : 0);            // x / disp [lmax*3]
+    float* st = sx + A.lmax * 3;                       // tri out  [tmax*9]
+    float* st2 = st + A.tmax * 9;                      // tri R    [tmax*9] (PD, check)
+    float* skw = st2 + ((PD && kCheck) ? A.tmax * 9 : 0);  // hinge k h^2 w_i [hmax*4]
+    float* sq = skw + A.hmax * 4;                      // hinge p-e (PD) or p [hmax*3]
+    float* sp = sq + A.hmax * 3;                       // hinge p [hmax*3] (PD, check)
+
+    // 1. positions of own and halo vertices
+    for (int i = threadIdx.x; i < nown; i += kFB) {
+        sts3(sx, i, ld3(a.x, (size_t)vbeg + i));
+        if (PD) sts3(sb, i, ld3(a.base, (size_t)vbeg + i));
+    }
+    for (int i = threadIdx.x; i < nhalo; i += kFB) {
+        const int g = A.halo[hbeg + i];
+        sts3(sx, nown + i, ld3(a.x, g));
+        if (PD) sts3(sb, nown + i, ld3(a.base, g));
+    }
+    __syncthreads();
+
+    // 2a. triangles -> corner forces (src/pd_solver.cu:48-298)
+    if (a.use_stretch) {
+        for (int j = threadIdx.x; j < ntri; j += kFB) {
+            const uint2 id = A.tri_idx[tbeg + j];
+            const int l0 = id.x & 0xFFFF, l1 = id.x >> 16, l2 = id.y & 0xFFFF;
+            f3 d0 = lds3(sx, l1) - lds3(sx, l0);
+            f3 d1 = lds3(sx, l2) - lds3(sx, l0);
+            if (PD) {
+                const f3 b0 = lds3(sb, l0);
+                d0 = (lds3(sb, l1) - b0) + d0;
+                d1 = (lds3(sb, l2) - b0) + d1;
+            }
+            const float4 G = A.tri_c0[tbeg + j];
+            const float g00 = G.x, g10 = G.y, g01 = G.z, g11 = G.w;
+            const float wA = A.tri_c1[tbeg + j];
+            const f3 F0 = mk(d0.x * g00 + d1.x * g10, d0.y * g00 + d1.y * g10, d0.z * g00 + d1.z * g10);
+            const f3 F1 = mk(d0.x * g01 + d1.x * g11, d0.y * g01 + d1.y * g11, d0.z * g01 + d1.z * g11);
+            f3 R0, R1;
+            if (PD) polar_rotation(F0, F1, R0, R1);
+            else    ref_cuda_rotation(F0, F1, R0, R1);
+            const float s0 = g00 + g10, s1 = g01 + g11;
+            f3 A0 = R0, A1 = R1;
+            if (PD) {
+                A0 = R0 - F0;
+                A1 = R1 - F1;
+            }
+            float* o = st + (size_t)j * 9;
+            sts3(o, 0, mk(-(s0 * A0.x + s1 * A1.x) * wA, -(s0 * A0.y + s1 * A1.y) * wA,
+                          -(s0 * A0.z + s1 * A1.z) * wA));
+            sts3(o, 1, mk((g00 * A0.x + g01 * A1.x) * wA, (g00 * A0.y + g01 * A1.y) * wA,
+                          (g00 * A0.z + g01 * A1.z) * wA));
+            sts3(o, 2, mk((g10 * A0.x + g11 * A1.x) * wA, (g10 * A0.y + g11 * A1.y) * wA,
+                          (g10 * A0.z + g11 * A1.z) * wA));
+            if (PD && kCheck) {
+                float* q = st2 + (size_t)j * 9;
+                sts3(q, 0, mk(-(s0 * R0.x + s1 * R1.x) * wA, -(s0 * R0.y + s1 * R1.y) * wA,
+                              -(s0 * R0.z + s1 * R1.z) * wA));
+                sts3(q, 1, mk((g00 * R0.x + g01 * R1.x) * wA, (g00 * R0.y + g01 * R1.y) * wA,
+                              (g00 * R0.z + g01 * R1.z) * wA));
+                sts3(q, 2, mk((g10 * R0.x + g11 * R1.x) * wA, (g10 * R0.y + g11 * R1.y) * wA,
+                              (g10 * R0.z + g11 * R1.z) * wA));
+            }
+        }
+    }
+    // 2b. hinges -> bend vectors (src/pd_solver.cu:304-358)
+    if (a.use_bend) {
+        for (int j = threadIdx.x; j < nhin; j += kFB) {
+            const uint2 id = A.hin_idx[ebeg + j];
+            const int q0 = id.x & 0xFFFF, q1 = id.x >> 16, q2 = id.y & 0xFFFF, q3 = id.y >> 16;
+            const float4 w = A.hin_w[ebeg + j];
+            const float2 c = A.hin_c[ebeg + j];  // (n_rest, k h^2)
+            f3 acc = mk(0.0f, 0.0f, 0.0f);
+            if (PD) {
+                const f3 b0 = lds3(sb, q0), x0 = lds3(sx, q0);
+                const int qs[3] = {q1, q2, q3};
+                const float ws[3] = {w.y, w.z, w.w};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const f3 d = (lds3(sb, qs[i]) - b0) + (lds3(sx, qs[i]) - x0);
+                    acc.x += ws[i] * d.x;
+                    acc.y += ws[i] * d.y;
+                    acc.z += ws[i] * d.z;
+                }
+                const float W = ((w.x + w.y) + w.z) + w.w;
+                const f3 p0 = b0 + x0;
+                acc.x += W * p0.x;
+                acc.y += W * p0.y;
+                acc.z += W * p0.z;
+            } else {
+                const int qs[4] = {q0, q1, q2, q3};
+                const float ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const f3 p = lds3(sx, qs[i]);
+                    acc.x += ws[i] * p.x;
+                    acc.y += ws[i] * p.y;
+                    acc.z += ws[i] * p.z;
+                }
+            }
+            const float len = sqrtf(dot(acc, acc));
+            f3 p = mk(0.0f, 0.0f, 0.0f);
+            if (len > 1e-10f && c.x > 1e-10f) p = acc * (c.x / len);
+            skw[j * 4 + 0] = c.y * w.x;
+            skw[j * 4 + 1] = c.y * w.y;
+            skw[j * 4 + 2] = c.y * w.z;
+            skw[j * 4 + 3] = c.y * w.w;
+            sts3(sq, j, PD ? (p - acc) : p);
+            if (PD && kCheck) sts3(sp, j, p);
+        }
+    }
+    __syncthreads();
+
+    // 3. own vertices: gather, update, pins, residual
+    double rr = 0.0, bb = 0.0;
+    for (int i = threadIdx.x; i < nown; i += kFB) {
+        const size_t g = (size_t)vbeg + i;
+        const int slot = a.pin_map[g];
+        if (slot >= 0) {
+            const f3 tgt = ld3(a.pin_target, slot);
+            st3(a.out, g, PD ? tgt - lds3(sb, i) : tgt);
+            continue;
+        }
+        const int j0 = A.gl_ptr[g], j1 = A.gl_ptr[g + 1];
+        const int jt = j0 + A.gl_ntri[g];
+        f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
+        if (a.use_stretch)
+            for (int j = j0; j < jt; ++j) {
+                const int s = A.gl_idx[j];
+                ar = ar + lds3(st, s);
+                if (PD && kCheck) ab = ab + lds3(st2, s);
+            }
+        if (a.use_bend)
+            for (int j = jt; j < j1; ++j) {
+                const int s = A.gl_idx[j];
+                const float kw = skw[s];
+                const f3 d = lds3(sq, s >> 2);
+                ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+                if (PD && kCheck) {
+                    const f3 p = lds3(sp, s >> 2);
+                    ab = ab + mk(kw * p.x, kw * p.y, kw * p.z);
+                }
+            }
+        const float m = a.mass[g];
+        const float dg = a.diag[g];
+        const float inv = (dg > 1e-10f) ? (1.0f / dg) : 0.0f;
+        const f3 yv = ld3(a.y, g);
+        if (!PD) {
+            const f3 rhs = mk(ar.x + m * yv.x, ar.y + m * yv.y, ar.z + m * yv.z);
+            st3(a.out, g, rhs * inv);
+            continue;
+        }
+        const f3 xv = lds3(sx, i);
+        const f3 r = mk(ar.x + m * (yv.x - xv.x), ar.y + m * (yv.y - xv.y), ar.z + m * (yv.z - xv.z));
+        f3 xn = mk(xv.x + r.x * inv, xv.y + r.y * inv, xv.z + r.z * inv);
+        if (a.blend) {
+            const f3 xp = ld3(a.xkm1, g);
+            xn = mk(a.omega * (xn.x - xp.x) + xp.x, a.omega * (xn.y - xp.y) + xp.y,
+                    a.omega * (xn.z - xp.z) + xp.z);
+        }
+        st3(a.out, g, xn);
+        if (kCheck) {
+            const f3 ya = lds3(sb, i) + yv;
+            const f3 b = mk(ab.x + m * ya.x, ab.y + m * ya.y, ab.z + m * ya.z);
+            rr += (double)r.x * r.x + (double)r.y * r.y + (double)r.z * r.z;
+            bb += (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
+        }
+    }
+    if (kCheck) fused_block_sum2(rr, bb, a.partials, tile);
+}
+
+// ---- tile-order frame kernels ------------------------------------------------
+
+__global__ void __launch_bounds__(256)
+k_fused_prologue(const int* __restrict__ perm, const float* __restrict__ pos,
+                 const float* __restrict__ vel, const float* __restrict__ guess,
+                 const float* __restrict__ mass_in, const float* __restrict__ diag_in,
+                 float* __restrict__ xprev, float* __restrict__ y, float* __restrict__ x0,
+                 float* __restrict__ mass, float* __restrict__ diag, int* __restrict__ pin_map,
+                 int N, float dt, float h2a0, float h2a1, float h2a2, int disp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int o = perm[i];
+    const f3 x = ld3(pos, o), u = ld3(vel, o);
+    st3(xprev, i, x);
+    const f3 dy = mk(dt * u.x + h2a0, dt * u.y + h2a1, dt * u.z + h2a2);
+    if (disp) {
+        st3(y, i, dy);
+        st3(x0, i, guess ? ld3(guess, o) - x : mk(0.0f, 0.0f, 0.0f));
+    } else {
+        st3(y, i, mk(x.x + dt * u.x + h2a0, x.y + dt * u.y + h2a1, x.z + dt * u.z + h2a2));
+        st3(x0, i, guess ? ld3(guess, o) : x);
+    }
+    mass[i] = mass_in[o];
+    diag[i] = diag_in[o];
+    pin_map[i] = -1;
+}
+
+__global__ void __launch_bounds__(256)
+k_fused_pins(const int* __restrict__ inv_perm, const int* __restrict__ idx,
+             const float* __restrict__ target, const float* __restrict__ xprev,
+             int* __restrict__ pin_map, float* __restrict__ x0, int P, int disp)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P) return;
+    const int i = inv_perm[idx[k]];
+    pin_map[i] = k;
+    if (disp) st3(x0, i, ld3(target, k) - ld3(xprev, i));
+}
+
+__global__ void __launch_bounds__(256)
+k_fused_epilogue(const int* __restrict__ perm, const float* __restrict__ xf,
+                 const float* __restrict__ xprev, const int* __restrict__ pin_map,
+                 const float* __restrict__ target, float* __restrict__ pos,
+                 float* __restrict__ vel, int N, float scale, int disp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int o = perm[i];
+    const int slot = pin_map[i];
+    if (slot >= 0) {
+        st3(pos, o, ld3(target, slot));
+        st3(vel, o, mk(0.0f, 0.0f, 0.0f));
+        return;
+    }
+    const f3 x = ld3(xf, i), xp = ld3(xprev, i);
+    if (disp) {
+        st3(pos, o, xp + x);
+        st3(vel, o, x * scale);
+    } else {
+        st3(pos, o, x);
+        st3(vel, o, mk((x.x - xp.x) * scale, (x.y - xp.y) * scale, (x.z - xp.z) * scale));
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_fused_trace(const int* __restrict__ perm, const float* __restrict__ it,
+              const float* __restrict__ xprev, float* __restrict__ out, int N, int disp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    st3(out, perm[i], disp ? ld3(xprev, i) + ld3(it, i) : ld3(it, i));
+}
+
+__global__ void __launch_bounds__(256)
+k_pack_tris(const int* __restrict__ gid, const float* __restrict__ dminv,
+            const float* __restrict__ area, const float* __restrict__ kst,
+            float4* __restrict__ c0, float* __restrict__ c1, int n, float h2)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const size_t t = (size_t)gid[j];
+    c0[j] = make_float4(dminv[t * 4 + 0], dminv[t * 4 + 1], dminv[t * 4 + 2], dminv[t * 4 + 3]);
+    c1[j] = kst ? kst[t] * area[t] * h2 : 0.0f;
+}
+
+__global__ void __launch_bounds__(256)
+k_pack_hinges(const int* __restrict__ gid, const float* __restrict__ w,
+              const float* __restrict__ nrest, const float* __restrict__ bk,
+              float4* __restrict__ hw, float2* __restrict__ hc, int n, float h2)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const size_t e = (size_t)gid[j];
+    hw[j] = make_float4(w[e * 4 + 0], w[e * 4 + 1], w[e * 4 + 2], w[e * 4 + 3]);
+    hc[j] = make_float2(nrest[e], bk[e] * h2);
+}
+
+size_t lds_bytes(const FusedPlan& p, int mode, bool check)
+{
+    const bool pd = (mode == CLH_MODE_PD);
+    size_t f = (size_t)p.lmax * 3 * (pd ? 2 : 1) + (size_t)p.tmax * 9 * ((pd && check) ? 2 : 1) +
+               (size_t)p.hmax * (4 + 3 + ((pd && check) ? 3 : 0));
+    return f * sizeof(float);
+}
+
+unsigned blocks_for(int n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
+                    const std::vector<int>& quads, const std::vector<float>& pos, int tile_verts)
+{
+    if (N <= 0) return fail(CLH_ERR_UNSUPPORTED, "fused plan: empty mesh");
+    const int T = (int)(tris.size() / 3), E = (int)(quads.size() / 4);
+    const int V = std::max(32, std::min(tile_verts, 1024));
+
+    std::vector<int> perm = morton_order(N, pos);
+    std::vector<int> inv((size_t)N);
+    for (int i = 0; i < N; ++i) inv[(size_t)perm[(size_t)i]] = i;
+
+    std::vector<int> vt_ptr, vt_ids, ve_ptr, ve_ids;
+    incidence(tris, 3, N, vt_ptr, vt_ids);
+    incidence(quads, 4, N, ve_ptr, ve_ids);
+
+    const int ntiles = (N + V - 1) / V;
+    std::vector<int> info((size_t)ntiles * kTileInfo), halo, tri_gid, hin_gid;
+    std::vector<uint2> tri_idx, hin_idx;
+    std::vector<int> gl_ptr((size_t)N + 1, 0);
+    std::vector<unsigned short> gl_idx;
+    std::vector<unsigned char> gl_ntri((size_t)N, 0);
+    std::vector<int> loc((size_t)N, -1);  // tile-order id -> local id in the current tile
+    int lmax = 0, tmax = 0, hmax = 0;
+
+    for (int t = 0; t < ntiles; ++t) {
+        const int vbeg = t * V, nown = std::min(N, vbeg + V) - vbeg;
+        std::vector<int> tl, hl, touched;
+        for (int i = 0; i < nown; ++i) {
+            const int o = perm[(size_t)vbeg + i];
+            for (int j = vt_ptr[(size_t)o]; j < vt_ptr[(size_t)o + 1]; ++j) tl.push_back(vt_ids[(size_t)j]);
+            for (int j = ve_ptr[(size_t)o]; j < ve_ptr[(size_t)o + 1]; ++j) hl.push_back(ve_ids[(size_t)j]);
+        }
+        std::sort(tl.begin(), tl.end());
+        tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
+        std::sort(hl.begin(), hl.end());
+        hl.erase(std::unique(hl.begin(), hl.end()), hl.end());
+
+        for (int i = 0; i < nown; ++i) {
+            loc[(size_t)vbeg + i] = i;
+            touched.push_back(vbeg + i);
+        }
+        int nl = nown;
+        const int hbeg = (int)halo.size();
+        auto local_of = [&](int orig) {
+            const int g = inv[(size_t)orig];
+            if (loc[(size_t)g] < 0) {
+                loc[(size_t)g] = nl++;
+                halo.push_back(g);
+                touched.push_back(g);
+            }
+            return loc[(size_t)g];
+        };
+        const int tbeg = (int)tri_gid.size();
+        for (int e : tl) {
+            const int a = local_of(tris[(size_t)e * 3]), b = local_of(tris[(size_t)e * 3 + 1]),
+                      c = local_of(tris[(size_t)e * 3 + 2]);
+            tri_gid.push_back(e);
+            tri_idx.push_back(make_uint2((unsigned)a | ((unsigned)b << 16), (unsigned)c | 0xFFFF0000u));
+        }
+        const int ebeg = (int)hin_gid.size();
+        for (int e : hl) {
+            const int a = local_of(quads[(size_t)e * 4]), b = local_of(quads[(size_t)e * 4 + 1]),
+                      c = local_of(quads[(size_t)e * 4 + 2]), d = local_of(quads[(size_t)e * 4 + 3]);
+            hin_gid.push_back(e);
+            hin_idx.push_back(make_uint2((unsigned)a | ((unsigned)b << 16), (unsigned)c | ((unsigned)d << 16)));
+        }
+        if (nl > 65535 || (int)tl.size() * 3 > 65535 || (int)hl.size() * 4 > 65535)
+            return fail(CLH_ERR_UNSUPPORTED, "fused plan: tile exceeds 16-bit local indexing");
+
+        // gather lists: triangle corners then hinge slots, ascending element id
+        for (int i = 0; i < nown; ++i) {
+            const int g = vbeg + i, o = perm[(size_t)g];
+            int nt = 0;
+            for (int j = vt_ptr[(size_t)o]; j < vt_ptr[(size_t)o + 1]; ++j) {
+                const int e = vt_ids[(size_t)j];
+                const int slot = (int)(std::lower_bound(tl.begin(), tl.end(), e) - tl.begin());
+                for (int c = 0; c < 3; ++c)
+                    if (tris[(size_t)e * 3 + c] == o) {
+                        gl_idx.push_back((unsigned short)(slot * 3 + c));
+                        ++nt;
+                    }
+            }
+            if (nt > 255) return fail(CLH_ERR_UNSUPPORTED, "fused plan: vertex valence above 255");
+            gl_ntri[(size_t)g] = (unsigned char)nt;
+            for (int j = ve_ptr[(size_t)o]; j < ve_ptr[(size_t)o + 1]; ++j) {
+                const int e = ve_ids[(size_t)j];
+                const int slot = (int)(std::lower_bound(hl.begin(), hl.end(), e) - hl.begin());
+                for (int c = 0; c < 4; ++c)
+                    if (quads[(size_t)e * 4 + c] == o) gl_idx.push_back((unsigned short)(slot * 4 + c));
+            }
+            gl_ptr[(size_t)g + 1] = (int)gl_idx.size();
+        }
+        int* ti = &info[(size_t)t * kTileInfo];
+        ti[0] = vbeg; ti[1] = nown; ti[2] = hbeg; ti[3] = nl - nown;
+        ti[4] = tbeg; ti[5] = (int)tl.size(); ti[6] = ebeg; ti[7] = (int)hl.size();
+        lmax = std::max(lmax, nl);
+        tmax = std::max(tmax, (int)tl.size());
+        hmax = std::max(hmax, (int)hl.size());
+        for (int g : touched) loc[(size_t)g] = -1;
+    }
+
+    FusedPlan& p = h.meta;
+    p.N = N;
+    p.num_tiles = ntiles;
+    p.tile_verts = V;
+    p.lmax = lmax;
+    p.tmax = std::max(tmax, 1);
+    p.hmax = std::max(hmax, 1);
+    p.vmax = V;
+    p.tri_factor = T > 0 ? (double)tri_gid.size() / T : 0.0;
+    p.bend_factor = E > 0 ? (double)hin_gid.size() / E : 0.0;
+    p.xcd_remap = (ntiles % 8 == 0) ? 1 : 0;
+    // bytes the iteration streams: per vertex x_k, x_{k-1}, base, y, out (12 B
+    // each), mass, diag, pin map, gather offset and count; halo ids and
+    // positions; packed triangle (28 B) and hinge (32 B) records; gather
+    // entries (2 B); tile descriptors
+    p.bytes_per_iter = (double)N * (5 * 12 + 4 + 4 + 4 + 4 + 1) + (double)halo.size() * (4 + 24) +
+                       (double)tri_gid.size() * 28 + (double)hin_gid.size() * 32 +
+                       (double)gl_idx.size() * 2 + (double)ntiles * kTileInfo * 4;
+
+    if (halo.empty()) halo.push_back(0);
+    if (tri_gid.empty()) { tri_gid.push_back(0); tri_idx.push_back(make_uint2(0, 0)); }
+    if (hin_gid.empty()) { hin_gid.push_back(0); hin_idx.push_back(make_uint2(0, 0)); }
+    if (gl_idx.empty()) gl_idx.push_back(0);
+    h.perm.swap(perm);
+    h.inv.swap(inv);
+    h.info.swap(info);
+    h.halo.swap(halo);
+    h.tri_gid.swap(tri_gid);
+    h.hin_gid.swap(hin_gid);
+    h.tri_idx.swap(tri_idx);
+    h.hin_idx.swap(hin_idx);
+    h.gl_ptr.swap(gl_ptr);
+    h.gl_idx.swap(gl_idx);
+    h.gl_ntri.swap(gl_ntri);
+    return CLH_OK;
+}
+
+// Checks every index of a host plan against the mesh: local ids inside the
+// tile's table, halo ids outside the own range, each local slot naming the
+// element's vertex, and each own vertex's gather list equal to its
+// incident (element, corner) slots in ascending order (the order of the
+// unfused pipeline and the oracle).
+int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::vector<int>& quads)
+{
+    const FusedPlan& p = h.meta;
+    const int N = p.N;
+    if ((int)h.perm.size() != N || (int)h.inv.size() != N) return fail(CLH_ERR_INVALID, "plan: perm size");
+    for (int i = 0; i < N; ++i)
+        if (h.perm[(size_t)i] < 0 || h.perm[(size_t)i] >= N || h.inv[(size_t)h.perm[(size_t)i]] != i)
+            return fail(CLH_ERR_INVALID, "plan: perm is not a permutation");
+    std::vector<int> vt_ptr, vt_ids, ve_ptr, ve_ids;
+    build_slots(tris, 3, N, vt_ptr, vt_ids);
+    build_slots(quads, 4, N, ve_ptr, ve_ids);
+    const int T = (int)(tris.size() / 3), E = (int)(quads.size() / 4);
+    int covered = 0;
+    for (int t = 0; t < p.num_tiles; ++t) {
+        const int* ti = &h.info[(size_t)t * kTileInfo];
+        const int vbeg = ti[0], nown = ti[1], hbeg = ti[2], nhalo = ti[3];
+        const int tbeg = ti[4], ntri = ti[5], ebeg = ti[6], nhin = ti[7];
+        const int nl = nown + nhalo;
+        if (vbeg != covered || nown <= 0 || nown > p.tile_verts) return fail(CLH_ERR_INVALID, "plan: own range");
+        covered += nown;
+        if (nl > p.lmax || ntri > p.tmax || nhin > p.hmax) return fail(CLH_ERR_INVALID, "plan: tile maxima");
+        auto tile_id = [&](int l) { return l < nown ? vbeg + l : h.halo[(size_t)hbeg + l - nown]; };
+        for (int l = nown; l < nl; ++l) {
+            const int g = tile_id(l);
+            if (g < 0 || g >= N || (g >= vbeg && g < vbeg + nown)) return fail(CLH_ERR_INVALID, "plan: halo id");
+        }
+        for (int j = 0; j < ntri; ++j) {
+            const uint2 id = h.tri_idx[(size_t)tbeg + j];
+            const int e = h.tri_gid[(size_t)tbeg + j];
+            const int l[3] = {(int)(id.x & 0xFFFF), (int)(id.x >> 16), (int)(id.y & 0xFFFF)};
+            if (e < 0 || e >= T) return fail(CLH_ERR_INVALID, "plan: triangle id");
+            for (int c = 0; c < 3; ++c)
+                if (l[c] >= nl || h.perm[(size_t)tile_id(l[c])] != tris[(size_t)e * 3 + c])
+                    return fail(CLH_ERR_INVALID, "plan: triangle local index");
+        }
+        for (int j = 0; j < nhin; ++j) {
+            const uint2 id = h.hin_idx[(size_t)ebeg + j];
+            const int e = h.hin_gid[(size_t)ebeg + j];
+            const int l[4] = {(int)(id.x & 0xFFFF), (int)(id.x >> 16), (int)(id.y & 0xFFFF), (int)(id.y >> 16)};
+            if (e < 0 || e >= E) return fail(CLH_ERR_INVALID, "plan: hinge id");
+            for (int c = 0; c < 4; ++c)
+                if (l[c] >= nl || h.perm[(size_t)tile_id(l[c])] != quads[(size_t)e * 4 + c])
+                    return fail(CLH_ERR_INVALID, "plan: hinge local index");
+        }
+        for (int i = 0; i < nown; ++i) {
+            const int g = vbeg + i, o = h.perm[(size_t)g];
+            const int j0 = h.gl_ptr[(size_t)g], j1 = h.gl_ptr[(size_t)g + 1], nt = h.gl_ntri[(size_t)g];
+            std::vector<int> got;
+            for (int j = j0; j < j1; ++j) {
+                const int sl = h.gl_idx[(size_t)j];
+                if (j < j0 + nt) {
+                    if (sl / 3 >= ntri) return fail(CLH_ERR_INVALID, "plan: gather triangle slot");
+                    got.push_back(h.tri_gid[(size_t)tbeg + sl / 3] * 3 + sl % 3);
+                } else {
+                    if (sl / 4 >= nhin) return fail(CLH_ERR_INVALID, "plan: gather hinge slot");
+                    got.push_back(-1 - (h.hin_gid[(size_t)ebeg + sl / 4] * 4 + sl % 4));
+                }
+            }
+            std::vector<int> want;
+            for (int j = vt_ptr[(size_t)o]; j < vt_ptr[(size_t)o + 1]; ++j) want.push_back(vt_ids[(size_t)j]);
+            for (int j = ve_ptr[(size_t)o]; j < ve_ptr[(size_t)o + 1]; ++j) want.push_back(-1 - ve_ids[(size_t)j]);
+            if (got != want) return fail(CLH_ERR_INVALID, "plan: gather list differs from element order");
+        }
+    }
+    if (covered != N) return fail(CLH_ERR_INVALID, "plan: tiles do not cover the vertices");
+    return CLH_OK;
+}
+
+int fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
+                     const std::vector<int>& quads, const std::vector<float>& pos, int tile_verts)
+{
+    fused_plan_release(p);
+    HostPlan h;
+    CLH_TRY(host_plan_build(h, N, tris, quads, pos, tile_verts));
+    const FusedPlan& m = h.meta;
+    p.N = m.N; p.num_tiles = m.num_tiles; p.tile_verts = m.tile_verts;
+    p.lmax = m.lmax; p.tmax = m.tmax; p.hmax = m.hmax; p.vmax = m.vmax;
+    p.tri_factor = m.tri_factor; p.bend_factor = m.bend_factor; p.xcd_remap = m.xcd_remap;
+    p.bytes_per_iter = m.bytes_per_iter;
+    CLH_TRY(p.perm.upload(h.perm));
+    CLH_TRY(p.inv_perm.upload(h.inv));
+    CLH_TRY(p.tile_info.upload(h.info));
+    CLH_TRY(p.halo.upload(h.halo));
+    CLH_TRY(p.tri_gid.upload(h.tri_gid));
+    CLH_TRY(p.tri_idx.upload(h.tri_idx));
+    CLH_TRY(p.hin_gid.upload(h.hin_gid));
+    CLH_TRY(p.hin_idx.upload(h.hin_idx));
+    CLH_TRY(p.tri_c0.resize(h.tri_gid.size()));
+    CLH_TRY(p.tri_c1.resize(h.tri_gid.size()));
+    CLH_TRY(p.hin_w.resize(h.hin_gid.size()));
+    CLH_TRY(p.hin_c.resize(h.hin_gid.size()));
+    CLH_TRY(p.gl_ptr.upload(h.gl_ptr));
+    CLH_TRY(p.gl_idx.upload(h.gl_idx));
+    CLH_TRY(p.gl_ntri.upload(h.gl_ntri));
+    std::memset(p.packed_src, 0, sizeof(p.packed_src));
+    p.packed_h2 = -1.0f;
+    return CLH_OK;
+}
+
+int fused_pack_constants(FusedPlan& p, const clh_step_args& a, float h2, hipStream_t stream)
+{
+    const void* src[8] = {a.d_Dm_inv, a.d_rest_area, a.d_stretch_k, a.d_bend_w,
+                          a.d_bend_n,  a.d_bend_k,    a.d_tris,      a.d_bend_quads};
+    if (p.packed_h2 == h2 && std::memcmp(src, p.packed_src, sizeof(src)) == 0) return CLH_OK;
+    const int nt = (int)p.tri_gid.count, nh = (int)p.hin_gid.count;
+    if (a.d_Dm_inv && a.d_rest_area)
+        hipLaunchKernelGGL(k_pack_tris, dim3(blocks_for(nt)), dim3(256), 0, stream, p.tri_gid.ptr,
+                           a.d_Dm_inv, a.d_rest_area, a.d_stretch_k, p.tri_c0.ptr, p.tri_c1.ptr, nt,
+                           h2);
+    if (a.d_bend_w && a.d_bend_n && a.d_bend_k)
+        hipLaunchKernelGGL(k_pack_hinges, dim3(blocks_for(nh)), dim3(256), 0, stream,
+                           p.hin_gid.ptr, a.d_bend_w, a.d_bend_n, a.d_bend_k, p.hin_w.ptr,
+                           p.hin_c.ptr, nh, h2);
+    CLH_HIP(hipGetLastError());
+    std::memcpy(p.packed_src, src, sizeof(src));
+    p.packed_h2 = h2;
+    return CLH_OK;
+}
+
+int fused_max_blocks(const FusedPlan& p) { return p.num_tiles; }
+
+int fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
+                           hipStream_t stream)
+{
+    FusedKernelArgs A;
+    A.it = a;
+    A.tile_info = p.tile_info.ptr;
+    A.halo = p.halo.ptr;
+    A.tri_idx = p.tri_idx.ptr;
+    A.tri_c0 = p.tri_c0.ptr;
+    A.tri_c1 = p.tri_c1.ptr;
+    A.hin_idx = p.hin_idx.ptr;
+    A.hin_w = p.hin_w.ptr;
+    A.hin_c = p.hin_c.ptr;
+    A.gl_ptr = p.gl_ptr.ptr;
+    A.gl_idx = p.gl_idx.ptr;
+    A.gl_ntri = p.gl_ntri.ptr;
+    A.lmax = p.lmax;
+    A.tmax = p.tmax;
+    A.hmax = p.hmax;
+    A.num_tiles = p.num_tiles;
+    A.xcd_remap = p.xcd_remap;
+    const bool check = a.partials != nullptr;
+    const size_t lds = lds_bytes(p, mode, check);
+    const dim3 grid(p.num_tiles), block(kFB);
+    if (mode == CLH_MODE_PD) {
+        if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true>), grid, block, lds, stream, A);
+        else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false>), grid, block, lds, stream, A);
+    } else {
+        hipLaunchKernelGGL((k_fused_iter<CLH_MODE_REF_CUDA, false>), grid, block, lds, stream, A);
+    }
+    return hip_status(hipGetLastError(), "fused iteration launch");
+}
+
+size_t fused_lds_bytes(const FusedPlan& p, int mode, bool check) { return lds_bytes(p, mode, check); }
+
+int fused_prologue(const FusedPlan& p, const clh_step_args& a, float* xprev, float* y, float* x0,
+                   float* mass, float* diag, int* pin_map, float dt, const float h2a[3],
+                   int disp, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_fused_prologue, dim3(blocks_for(p.N)), dim3(256), 0, stream, p.perm.ptr,
+                       a.d_pos, a.d_vel, a.d_init_guess, a.d_mass, a.d_jacobi_diag, xprev, y, x0,
+                       mass, diag, pin_map, p.N, dt, h2a[0], h2a[1], h2a[2], disp);
+    if (a.num_pinned > 0)
+        hipLaunchKernelGGL(k_fused_pins, dim3(blocks_for(a.num_pinned)), dim3(256), 0, stream,
+                           p.inv_perm.ptr, a.d_pin_idx, a.d_pin_target, xprev, pin_map, x0,
+                           a.num_pinned, disp);
+    return hip_status(hipGetLastError(), "fused prologue launch");
+}
+
+int fused_epilogue(const FusedPlan& p, const clh_step_args& a, const float* xf, const float* xprev,
+                   const int* pin_map, float scale, int disp, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_fused_epilogue, dim3(blocks_for(p.N)), dim3(256), 0, stream, p.perm.ptr,
+                       xf, xprev, pin_map, a.d_pin_target, a.d_pos, a.d_vel, p.N, scale, disp);
+    return hip_status(hipGetLastError(), "fused epilogue launch");
+}
+
+int fused_trace(const FusedPlan& p, const float* it, const float* xprev, float* out, int disp,
+                hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_fused_trace, dim3(blocks_for(p.N)), dim3(256), 0, stream, p.perm.ptr, it,
+                       xprev, out, p.N, disp);
+    return hip_status(hipGetLastError(), "fused trace launch");
+}
+
+}  // namespace clh
+
+extern "C" int clh_fused_plan_check(int N, int T, int E, const int* tris, const int* quads,
+                                    const float* pos, int tile_verts, int mode,
+                                    clh_plan_info* out)
+{
+    using namespace clh;
+    if (N <= 0 || T < 0 || E < 0 || !pos || (T > 0 && !tris) || (E > 0 && !quads))
+        return fail(CLH_ERR_INVALID, "clh_fused_plan_check: bad arguments");
+    std::vector<int> t(tris, tris + (size_t)T * 3), q;
+    if (E > 0) q.assign(quads, quads + (size_t)E * 4);
+    for (int v : t) if (v < 0 || v >= N) return fail(CLH_ERR_INVALID, "triangle vertex index out of range");
+    for (int v : q) if (v < 0 || v >= N) return fail(CLH_ERR_INVALID, "hinge vertex index out of range");
+    std::vector<float> x(pos, pos + (size_t)N * 3);
+    HostPlan h;
+    CLH_TRY(host_plan_build(h, N, t, q, x, tile_verts));
+    CLH_TRY(host_plan_check(h, t, q));
+    if (out) {
+        std::memset(out, 0, sizeof(*out));
+        out->variant = CLH_KERNEL_FUSED;
+        fused_plan_info(h.meta, out);
+        out->lds_bytes = (double)fused_lds_bytes(h.meta, mode, true);
+    }
+    return CLH_OK;
+}
 
 namespace clh {
 
-int fused_plan_build(FusedPlan&, int, const std::vector<int>&, const std::vector<int>&, int)
-{
-    return fail(CLH_ERR_UNSUPPORTED, "fused kernel variant not available");
-}
-int fused_max_blocks(const FusedPlan& p) { return p.num_tiles; }
-int fused_launch_iteration(const FusedPlan&, const FusedIterArgs&, int, hipStream_t)
-{
-    return fail(CLH_ERR_UNSUPPORTED, "fused kernel variant not available");
-}
 void fused_plan_info(const FusedPlan& p, clh_plan_info* out)
 {
     out->num_tiles = p.num_tiles;
@@ -21,6 +834,25 @@ void fused_plan_info(const FusedPlan& p, clh_plan_info* out)
     out->halo_bend_factor = p.bend_factor;
     out->bytes_per_iter = p.bytes_per_iter;
 }
-void fused_plan_release(FusedPlan&) {}
+
+void fused_plan_release(FusedPlan& p)
+{
+    p.perm.release();
+    p.inv_perm.release();
+    p.tile_info.release();
+    p.halo.release();
+    p.tri_gid.release();
+    p.hin_gid.release();
+    p.tri_idx.release();
+    p.hin_idx.release();
+    p.tri_c0.release();
+    p.hin_w.release();
+    p.tri_c1.release();
+    p.hin_c.release();
+    p.gl_ptr.release();
+    p.gl_idx.release();
+    p.gl_ntri.release();
+    p.num_tiles = 0;
+}
 
 }  // namespace clh

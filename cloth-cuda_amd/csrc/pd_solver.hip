@@ -427,6 +427,8 @@ struct clh_solver {
     DevBuf<int> vt_ptr, vt_idx, ve_ptr, ve_idx;
     FusedPlan fused;
     int variant = CLH_KERNEL_UNFUSED;
+    int tile_verts = 128;
+    DevBuf<float> mass_t, diag_t;  // tile-order copies (fused variant)
 
     // scratch
     DevBuf<float> y, xprev, it[3], fr, fd, bp, bd;
@@ -485,8 +487,20 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
     CLH_TRY(s->ve_idx.upload(idx));
 
     s->variant = CLH_KERNEL_UNFUSED;
-    if (s->cfg.kernel_variant != CLH_KERNEL_UNFUSED) {
-        const int st = fused_plan_build(s->fused, N, tris, quads, s->cfg.mode);
+    if (s->cfg.kernel_variant != CLH_KERNEL_UNFUSED && N > 0 && a.d_pos) {
+        std::vector<float> pos((size_t)N * 3);
+        CLH_HIP(hipMemcpy(pos.data(), a.d_pos, pos.size() * sizeof(float), hipMemcpyDeviceToHost));
+        int dev = 0, lds_limit = 0;
+        CLH_HIP(hipGetDevice(&dev));
+        CLH_HIP(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        lds_limit = std::min(lds_limit, 64 * 1024);
+        int st = CLH_ERR_UNSUPPORTED;
+        for (int tv = s->tile_verts; tv >= 32; tv /= 2) {
+            st = fused_plan_build(s->fused, N, tris, quads, pos, tv);
+            if (st != CLH_OK) break;
+            if (fused_lds_bytes(s->fused, s->cfg.mode, true) <= (size_t)lds_limit) break;
+            st = fail(CLH_ERR_UNSUPPORTED, "fused plan: tile does not fit in LDS");
+        }
         if (st == CLH_OK) s->variant = CLH_KERNEL_FUSED;
         else if (s->cfg.kernel_variant == CLH_KERNEL_FUSED) return st;
     }
@@ -505,6 +519,10 @@ int ensure_scratch(clh_solver* s)
     CLH_TRY(s->xprev.resize(N * 3));
     for (auto& b : s->it) CLH_TRY(b.resize(N * 3));
     CLH_TRY(s->pin_map.resize(N));
+    if (s->variant == CLH_KERNEL_FUSED) {
+        CLH_TRY(s->mass_t.resize(N));
+        CLH_TRY(s->diag_t.resize(N));
+    }
     CLH_TRY(s->fr.resize(T * 9));
     CLH_TRY(s->bp.resize(E * 3));
     if (s->cfg.mode == CLH_MODE_PD) {
@@ -549,6 +567,13 @@ int enqueue_prologue(clh_solver* s, const clh_step_args& a)
     const float h2 = c.dt * c.dt;
     const float acc[3] = {c.ext_accel[0], c.gravity + c.ext_accel[1], c.ext_accel[2]};
     const float h2a0 = h2 * acc[0], h2a1 = h2 * acc[1], h2a2 = h2 * acc[2];
+    const int disp = c.mode == CLH_MODE_PD ? 1 : 0;
+    if (s->variant == CLH_KERNEL_FUSED) {
+        const float h2a[3] = {h2a0, h2a1, h2a2};
+        CLH_TRY(fused_pack_constants(s->fused, a, h2, s->stream));
+        return fused_prologue(s->fused, a, s->xprev.ptr, s->y.ptr, s->it[0].ptr, s->mass_t.ptr,
+                              s->diag_t.ptr, s->pin_map.ptr, c.dt, h2a, disp, s->stream);
+    }
     if (s->N > 0)
         hipLaunchKernelGGL(k_prologue, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream, a.d_pos,
                            a.d_vel, a.d_init_guess, s->xprev.ptr, s->y.ptr, s->it[0].ptr,
@@ -578,13 +603,13 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
         f.x = xk;
         f.xkm1 = s->it[(k + 2) % 3].ptr;
         f.out = s->it[(k + 1) % 3].ptr;
+        f.base = s->xprev.ptr;
         f.y = s->y.ptr;
-        f.mass = a.d_mass;
-        f.diag = a.d_jacobi_diag;
+        f.mass = s->mass_t.ptr;
+        f.diag = s->diag_t.ptr;
         f.pin_map = s->pin_map.ptr;
         f.pin_target = a.d_pin_target;
         f.partials = check ? s->partials.ptr : nullptr;
-        f.h2 = h2;
         f.omega = omega;
         f.blend = blend ? 1 : 0;
         f.use_stretch = use_stretch ? 1 : 0;
@@ -659,6 +684,9 @@ int enqueue_epilogue(clh_solver* s, const clh_step_args& a, int iters)
 {
     const clh_config& c = s->cfg;
     const float scale = (1.0f - c.damping) / c.dt;
+    if (s->variant == CLH_KERNEL_FUSED)
+        return fused_epilogue(s->fused, a, s->it[iters % 3].ptr, s->xprev.ptr, s->pin_map.ptr,
+                              scale, c.mode == CLH_MODE_PD ? 1 : 0, s->stream);
     if (s->N > 0)
         hipLaunchKernelGGL(k_epilogue, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream,
                            s->it[iters % 3].ptr, s->xprev.ptr, s->pin_map.ptr, a.d_pin_target,
@@ -669,6 +697,10 @@ int enqueue_epilogue(clh_solver* s, const clh_step_args& a, int iters)
 // Absolute positions of iterate k+1 into slot k of the trace.
 int enqueue_trace(clh_solver* s, const clh_step_args& a, int k)
 {
+    if (s->variant == CLH_KERNEL_FUSED)
+        return fused_trace(s->fused, s->it[(k + 1) % 3].ptr, s->xprev.ptr,
+                           a.d_trace + (size_t)k * s->N * 3, s->cfg.mode == CLH_MODE_PD ? 1 : 0,
+                           s->stream);
     if (s->N > 0)
         hipLaunchKernelGGL(k_trace, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream,
                            s->it[(k + 1) % 3].ptr, s->xprev.ptr,
@@ -845,16 +877,8 @@ extern "C" int clh_solver_create(const clh_config* cfg, const clh_mesh_desc* mes
         return st;
     }
     s->stream = s->own_stream;
-    if (mesh->d_tris || mesh->d_bend_quads) {
-        clh_step_args a = {};
-        a.d_tris = mesh->d_tris;
-        a.d_bend_quads = mesh->d_bend_quads;
-        st = ensure_plan(s, a);
-        if (st != CLH_OK) {
-            clh_solver_destroy(s);
-            return st;
-        }
-    }
+    // the plan is built at the first step, which also supplies the positions
+    // the fused variant orders its tiles by
     *out = s;
     return CLH_OK;
 }

@@ -208,8 +208,20 @@ typedef struct clh_plan_info {
     double halo_tri_factor;  /* triangles processed / T */
     double halo_bend_factor; /* hinges processed / E    */
     double bytes_per_iter;   /* bytes the kernel streams per iteration (model) */
+    double lds_bytes;        /* LDS per workgroup of the residual-check variant */
 } clh_plan_info;
 int clh_solver_plan_info(clh_solver* s, clh_plan_info* out);
+
+/* Builds the fused kernel's tile plan for a mesh from host arrays (no device
+ * needed) and checks every index in it: local ids inside each tile's LDS
+ * table, halo ids outside the tile, each local slot naming the element's
+ * vertex, and each vertex's gather list equal to its incident element slots
+ * in ascending order. Returns CLH_OK and the plan statistics, or an error
+ * naming the first violated property. Also the tile-size probe of the
+ * autotuner. */
+int clh_fused_plan_check(int num_verts, int num_tris, int num_bend, const int* tris,
+                         const int* bend_quads, const float* pos, int tile_verts, int mode,
+                         clh_plan_info* out);
 
 /* Streams bytes of device memory through a copy kernel num_launches times and
  * returns the average time per launch (measured copy peak). */

@@ -332,3 +332,25 @@ def test_reference_bounds_through_cpp_api():
     x, v = s.get_state()
     ke = 0.5 * (arr["mass"] * (v.astype(np.float64) ** 2).sum(1)).sum()
     assert np.linalg.norm(v, axis=1).max() < 10 and ke < 10
+
+
+@pytest.mark.parametrize("mode", [ch.MODE_REF_CUDA, ch.MODE_PD])
+@pytest.mark.parametrize("spec", ["cfg1", "irregular16", "cross10"])
+def test_fused_matches_unfused(mode, spec):
+    """The LDS-tiled kernel and the element + gather pipeline sum in the same
+    order; frames agree to 1e-6 relative (FMA contraction may differ)."""
+    s, arr = make(REF_CASES[spec])
+    pos, vel = s.get_state()
+    outs = []
+    for variant in (ch.KERNEL_UNFUSED, ch.KERNEL_FUSED):
+        s.set_state(pos, vel)
+        sol = solver_for(s, mode=mode, max_iterations=12, gravity=-9.81, dt=REF_CASES[spec]["dt"],
+                         rho=0.99, cheb_delay=1, kernel_variant=variant, tolerance=0.0)
+        args = s.step_args()
+        for _ in range(2):
+            sol.step(args)
+        assert sol.plan_info().variant == variant
+        outs.append(s.get_state())
+    L = max(extent(pos), 1.0)
+    assert rel_err(outs[1][0], outs[0][0], L) < 1e-6
+    assert rel_err(outs[1][1], outs[0][1], L / REF_CASES[spec]["dt"]) < 1e-6
