@@ -224,32 +224,27 @@ __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
             const float4 G = A.tri_c0[tbeg + j];
             const float g00 = G.x, g10 = G.y, g01 = G.z, g11 = G.w;
             const float wA = A.tri_c1[tbeg + j];
-            const f3 F0 = mk(d0.x * g00 + d1.x * g10, d0.y * g00 + d1.y * g10, d0.z * g00 + d1.z * g10);
-            const f3 F1 = mk(d0.x * g01 + d1.x * g11, d0.y * g01 + d1.y * g11, d0.z * g01 + d1.z * g11);
-            f3 R0, R1;
-            if (PD) polar_rotation(F0, F1, R0, R1);
-            else    ref_cuda_rotation(F0, F1, R0, R1);
-            const float s0 = g00 + g10, s1 = g01 + g11;
-            f3 A0 = R0, A1 = R1;
+            f3 F0, F1, R0, R1, c[3];
             if (PD) {
-                A0 = R0 - F0;
-                A1 = R1 - F1;
+                F0 = mk(d0.x * g00 + d1.x * g10, d0.y * g00 + d1.y * g10, d0.z * g00 + d1.z * g10);
+                F1 = mk(d0.x * g01 + d1.x * g11, d0.y * g01 + d1.y * g11, d0.z * g01 + d1.z * g11);
+                polar_rotation(F0, F1, R0, R1);
+                corner_forces(g00, g10, g01, g11, wA, R0 - F0, R1 - F1, c);
+            } else {
+                deformation_nc(d0, d1, g00, g10, g01, g11, F0, F1);
+                ref_cuda_rotation(F0, F1, R0, R1);
+                corner_forces_nc(g00, g10, g01, g11, wA, R0, R1, c);
             }
             float* o = st + (size_t)j * 9;
-            sts3(o, 0, mk(-(s0 * A0.x + s1 * A1.x) * wA, -(s0 * A0.y + s1 * A1.y) * wA,
-                          -(s0 * A0.z + s1 * A1.z) * wA));
-            sts3(o, 1, mk((g00 * A0.x + g01 * A1.x) * wA, (g00 * A0.y + g01 * A1.y) * wA,
-                          (g00 * A0.z + g01 * A1.z) * wA));
-            sts3(o, 2, mk((g10 * A0.x + g11 * A1.x) * wA, (g10 * A0.y + g11 * A1.y) * wA,
-                          (g10 * A0.z + g11 * A1.z) * wA));
+            sts3(o, 0, c[0]);
+            sts3(o, 1, c[1]);
+            sts3(o, 2, c[2]);
             if (PD && kCheck) {
+                corner_forces(g00, g10, g01, g11, wA, R0, R1, c);
                 float* q = st2 + (size_t)j * 9;
-                sts3(q, 0, mk(-(s0 * R0.x + s1 * R1.x) * wA, -(s0 * R0.y + s1 * R1.y) * wA,
-                              -(s0 * R0.z + s1 * R1.z) * wA));
-                sts3(q, 1, mk((g00 * R0.x + g01 * R1.x) * wA, (g00 * R0.y + g01 * R1.y) * wA,
-                              (g00 * R0.z + g01 * R1.z) * wA));
-                sts3(q, 2, mk((g10 * R0.x + g11 * R1.x) * wA, (g10 * R0.y + g11 * R1.y) * wA,
-                              (g10 * R0.z + g11 * R1.z) * wA));
+                sts3(q, 0, c[0]);
+                sts3(q, 1, c[1]);
+                sts3(q, 2, c[2]);
             }
         }
     }
@@ -278,19 +273,19 @@ __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
                 acc.y += W * p0.y;
                 acc.z += W * p0.z;
             } else {
-                const int qs[4] = {q0, q1, q2, q3};
-                const float ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const f3 p = lds3(sx, qs[i]);
-                    acc.x += ws[i] * p.x;
-                    acc.y += ws[i] * p.y;
-                    acc.z += ws[i] * p.z;
-                }
+                acc = madd_nc(acc, w.x, lds3(sx, q0));
+                acc = madd_nc(acc, w.y, lds3(sx, q1));
+                acc = madd_nc(acc, w.z, lds3(sx, q2));
+                acc = madd_nc(acc, w.w, lds3(sx, q3));
             }
-            const float len = sqrtf(dot(acc, acc));
-            f3 p = mk(0.0f, 0.0f, 0.0f);
-            if (len > 1e-10f && c.x > 1e-10f) p = acc * (c.x / len);
+            f3 p;
+            if (PD) {
+                const float len = sqrtf(dot(acc, acc));
+                p = mk(0.0f, 0.0f, 0.0f);
+                if (len > 1e-10f && c.x > 1e-10f) p = acc * (c.x / len);
+            } else {
+                p = hinge_projection_nc(acc, c.x);
+            }
             skw[j * 4 + 0] = c.y * w.x;
             skw[j * 4 + 1] = c.y * w.y;
             skw[j * 4 + 2] = c.y * w.z;
@@ -325,7 +320,8 @@ __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
                 const int s = A.gl_idx[j];
                 const float kw = skw[s];
                 const f3 d = lds3(sq, s >> 2);
-                ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+                if (PD) ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+                else    ar = madd_nc(ar, kw, d);
                 if (PD && kCheck) {
                     const f3 p = lds3(sp, s >> 2);
                     ab = ab + mk(kw * p.x, kw * p.y, kw * p.z);
@@ -336,8 +332,7 @@ __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
         const float inv = (dg > 1e-10f) ? (1.0f / dg) : 0.0f;
         const f3 yv = ld3(a.y, g);
         if (!PD) {
-            const f3 rhs = mk(ar.x + m * yv.x, ar.y + m * yv.y, ar.z + m * yv.z);
-            st3(a.out, g, rhs * inv);
+            st3(a.out, g, madd_nc(ar, m, yv) * inv);
             continue;
         }
         const f3 xv = lds3(sx, i);
@@ -379,7 +374,7 @@ k_fused_prologue(const int* __restrict__ perm, const float* __restrict__ pos,
         st3(y, i, dy);
         st3(x0, i, guess ? ld3(guess, o) - x : mk(0.0f, 0.0f, 0.0f));
     } else {
-        st3(y, i, mk(x.x + dt * u.x + h2a0, x.y + dt * u.y + h2a1, x.z + dt * u.z + h2a2));
+        st3(y, i, predict_nc(x, u, dt, h2a0, h2a1, h2a2));
         st3(x0, i, guess ? ld3(guess, o) : x);
     }
     mass[i] = mass_in[o];

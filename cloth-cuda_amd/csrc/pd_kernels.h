@@ -36,20 +36,87 @@ __device__ __forceinline__ void st3(float* __restrict__ p, size_t i, f3 v)
     p[i * 3 + 2] = v.z;
 }
 
+// ---------------------------------------------------------------------------
+// Reference-mode arithmetic. It is evaluated without FMA contraction,
+// operation for operation as written, so it rounds like the FP32
+// restatement in the oracle: the reference dynamics are sensitive to the
+// last bit (after one iteration its update shrinks interior triangles to
+// ~1e-4 m, where the 1e-10 branch tests of the rotation flip on a single
+// rounding). Every multiply-add of the reference path goes through these
+// helpers; the f3 operators above would let inlined multiply/add pairs fuse.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float dot_nc(f3 a, f3 b)
+{
+#pragma clang fp contract(off)
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+// F = [d0 d1] G for G = [[g00, g01], [g10, g11]].
+__device__ __forceinline__ void deformation_nc(f3 d0, f3 d1, float g00, float g10, float g01,
+                                               float g11, f3& F0, f3& F1)
+{
+#pragma clang fp contract(off)
+    F0 = f3{d0.x * g00 + d1.x * g10, d0.y * g00 + d1.y * g10, d0.z * g00 + d1.z * g10};
+    F1 = f3{d0.x * g01 + d1.x * g11, d0.y * g01 + d1.y * g11, d0.z * g01 + d1.z * g11};
+}
+
+// Corner forces wA D^T [A0 A1] of the three corners (src/pd_solver.cu:279-293).
+__device__ __forceinline__ void corner_forces_nc(float g00, float g10, float g01, float g11,
+                                                 float wA, f3 A0, f3 A1, f3 c[3])
+{
+#pragma clang fp contract(off)
+    const float s0 = g00 + g10, s1 = g01 + g11;
+    c[0] = f3{-(s0 * A0.x + s1 * A1.x) * wA, -(s0 * A0.y + s1 * A1.y) * wA,
+              -(s0 * A0.z + s1 * A1.z) * wA};
+    c[1] = f3{(g00 * A0.x + g01 * A1.x) * wA, (g00 * A0.y + g01 * A1.y) * wA,
+              (g00 * A0.z + g01 * A1.z) * wA};
+    c[2] = f3{(g10 * A0.x + g11 * A1.x) * wA, (g10 * A0.y + g11 * A1.y) * wA,
+              (g10 * A0.z + g11 * A1.z) * wA};
+}
+
+// (x + dt * u) + h2a, component-wise: the inertial prediction
+// (src/pd_solver.cu:26-29).
+__device__ __forceinline__ f3 predict_nc(f3 x, f3 u, float dt, float a0, float a1, float a2)
+{
+#pragma clang fp contract(off)
+    return f3{x.x + dt * u.x + a0, x.y + dt * u.y + a1, x.z + dt * u.z + a2};
+}
+
+// acc + w * p, component-wise.
+__device__ __forceinline__ f3 madd_nc(f3 acc, float w, f3 p)
+{
+#pragma clang fp contract(off)
+    return f3{acc.x + w * p.x, acc.y + w * p.y, acc.z + w * p.z};
+}
+
+// Hinge projection p = e * n_rest / |e| or 0 (src/pd_solver.cu:315-332).
+__device__ __forceinline__ f3 hinge_projection_nc(f3 e, float nr)
+{
+#pragma clang fp contract(off)
+    const float len = sqrtf(e.x * e.x + e.y * e.y + e.z * e.z);
+    if (len > 1e-10f && nr > 1e-10f) {
+        const float sc = nr / len;
+        return f3{e.x * sc, e.y * sc, e.z * sc};
+    }
+    return f3{0.0f, 0.0f, 0.0f};
+}
+
 // Gram-Schmidt frame (u1, u2) of the columns of F with the reference GPU
 // kernel's fallbacks (src/pd_solver.cu:79-107).
 __device__ __forceinline__ void frame_ref_cuda(f3 F0, f3 F1, f3& u1, f3& u2)
 {
-    const float l0 = sqrtf(dot(F0, F0));
+#pragma clang fp contract(off)
+    const float l0 = sqrtf(dot_nc(F0, F0));
     u1 = (l0 > 1e-10f) ? mk(F0.x / l0, F0.y / l0, F0.z / l0) : mk(1.0f, 0.0f, 0.0f);
-    const float a = dot(F1, u1);
+    const float a = dot_nc(F1, u1);
     const f3 c = mk(F1.x - a * u1.x, F1.y - a * u1.y, F1.z - a * u1.z);
-    const float lc = sqrtf(dot(c, c));
+    const float lc = sqrtf(dot_nc(c, c));
     if (lc > 1e-10f) {
         u2 = mk(c.x / lc, c.y / lc, c.z / lc);
     } else {
         u2 = (fabsf(u1.x) > 0.9f) ? mk(-u1.y, u1.x, 0.0f) : mk(0.0f, -u1.z, u1.y);
-        const float l2 = sqrtf(dot(u2, u2));
+        const float l2 = sqrtf(dot_nc(u2, u2));
         u2 = mk(u2.x / l2, u2.y / l2, u2.z / l2);
     }
 }
@@ -57,10 +124,11 @@ __device__ __forceinline__ void frame_ref_cuda(f3 F0, f3 F1, f3& u1, f3& u2)
 // Rotation (R0, R1) = columns of U * R2d as the reference GPU kernel forms it.
 __device__ __forceinline__ void ref_cuda_rotation(f3 F0, f3 F1, f3& R0, f3& R1)
 {
+#pragma clang fp contract(off)
     f3 u1, u2;
     frame_ref_cuda(F0, F1, u1, u2);
-    const float a00 = dot(u1, F0), a01 = dot(u1, F1);
-    const float a10 = dot(u2, F0), a11 = dot(u2, F1);
+    const float a00 = dot_nc(u1, F0), a01 = dot_nc(u1, F1);
+    const float a10 = dot_nc(u2, F0), a11 = dot_nc(u2, F1);
 
     // eigen-decomposition of A = F2d^T F2d
     const float A00 = a00 * a00 + a10 * a10;
@@ -119,6 +187,19 @@ __device__ __forceinline__ void ref_cuda_rotation(f3 F0, f3 F1, f3& R0, f3& R1)
     const float r11 = v1y * w1y + v2y * w2y;
     R0 = mk(u1.x * r00 + u2.x * r10, u1.y * r00 + u2.y * r10, u1.z * r00 + u2.z * r10);
     R1 = mk(u1.x * r01 + u2.x * r11, u1.y * r01 + u2.y * r11, u1.z * r01 + u2.z * r11);
+}
+
+// Corner forces wA D^T [A0 A1] (contraction allowed; PD mode).
+__device__ __forceinline__ void corner_forces(float g00, float g10, float g01, float g11,
+                                              float wA, f3 A0, f3 A1, f3 c[3])
+{
+    const float s0 = g00 + g10, s1 = g01 + g11;
+    c[0] = f3{-(s0 * A0.x + s1 * A1.x) * wA, -(s0 * A0.y + s1 * A1.y) * wA,
+              -(s0 * A0.z + s1 * A1.z) * wA};
+    c[1] = f3{(g00 * A0.x + g01 * A1.x) * wA, (g00 * A0.y + g01 * A1.y) * wA,
+              (g00 * A0.z + g01 * A1.z) * wA};
+    c[2] = f3{(g10 * A0.x + g11 * A1.x) * wA, (g10 * A0.y + g11 * A1.y) * wA,
+              (g10 * A0.z + g11 * A1.z) * wA};
 }
 
 // Closest rotation on the Stiefel manifold (the frame fallbacks follow the

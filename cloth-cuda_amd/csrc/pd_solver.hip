@@ -62,7 +62,7 @@ k_prologue(const float* __restrict__ pos, const float* __restrict__ vel,
         st3(y, v, dy);
         st3(x0, v, guess ? ld3(guess, v) - x : mk(0.0f, 0.0f, 0.0f));
     } else {
-        st3(y, v, mk(x.x + dt * u.x + h2a0, x.y + dt * u.y + h2a1, x.z + dt * u.z + h2a2));
+        st3(y, v, predict_nc(x, u, dt, h2a0, h2a1, h2a2));
         st3(x0, v, guess ? ld3(guess, v) : x);
     }
     pin_map[v] = -1;
@@ -107,30 +107,28 @@ k_triangles(const float* __restrict__ base, const float* __restrict__ x,
     }
     const float g00 = dminv[(size_t)t * 4 + 0], g10 = dminv[(size_t)t * 4 + 1];
     const float g01 = dminv[(size_t)t * 4 + 2], g11 = dminv[(size_t)t * 4 + 3];
-    const f3 F0 = mk(d0.x * g00 + d1.x * g10, d0.y * g00 + d1.y * g10, d0.z * g00 + d1.z * g10);
-    const f3 F1 = mk(d0.x * g01 + d1.x * g11, d0.y * g01 + d1.y * g11, d0.z * g01 + d1.z * g11);
     const float wA = kst[t] * area[t] * h2;
-    const float s0 = g00 + g10, s1 = g01 + g11;
-    f3 R0, R1;
-    if (kMode == CLH_MODE_REF_CUDA) ref_cuda_rotation(F0, F1, R0, R1);
-    else                            polar_rotation(F0, F1, R0, R1);
-
+    f3 F0, F1, R0, R1, c[3];
+    if (kMode == CLH_MODE_REF_CUDA) {
+        deformation_nc(d0, d1, g00, g10, g01, g11, F0, F1);
+        ref_cuda_rotation(F0, F1, R0, R1);
+        corner_forces_nc(g00, g10, g01, g11, wA, R0, R1, c);
+    } else {
+        F0 = mk(d0.x * g00 + d1.x * g10, d0.y * g00 + d1.y * g10, d0.z * g00 + d1.z * g10);
+        F1 = mk(d0.x * g01 + d1.x * g11, d0.y * g01 + d1.y * g11, d0.z * g01 + d1.z * g11);
+        polar_rotation(F0, F1, R0, R1);
+        corner_forces(g00, g10, g01, g11, wA, R0, R1, c);
+    }
     float* o = fr + (size_t)t * 9;
-    st3(o, 0, mk(-(s0 * R0.x + s1 * R1.x) * wA, -(s0 * R0.y + s1 * R1.y) * wA,
-                 -(s0 * R0.z + s1 * R1.z) * wA));
-    st3(o, 1, mk((g00 * R0.x + g01 * R1.x) * wA, (g00 * R0.y + g01 * R1.y) * wA,
-                 (g00 * R0.z + g01 * R1.z) * wA));
-    st3(o, 2, mk((g10 * R0.x + g11 * R1.x) * wA, (g10 * R0.y + g11 * R1.y) * wA,
-                 (g10 * R0.z + g11 * R1.z) * wA));
+    st3(o, 0, c[0]);
+    st3(o, 1, c[1]);
+    st3(o, 2, c[2]);
     if (kMode == CLH_MODE_PD) {
-        const f3 E0 = R0 - F0, E1 = R1 - F1;
+        corner_forces(g00, g10, g01, g11, wA, R0 - F0, R1 - F1, c);
         float* q = fd + (size_t)t * 9;
-        st3(q, 0, mk(-(s0 * E0.x + s1 * E1.x) * wA, -(s0 * E0.y + s1 * E1.y) * wA,
-                     -(s0 * E0.z + s1 * E1.z) * wA));
-        st3(q, 1, mk((g00 * E0.x + g01 * E1.x) * wA, (g00 * E0.y + g01 * E1.y) * wA,
-                     (g00 * E0.z + g01 * E1.z) * wA));
-        st3(q, 2, mk((g10 * E0.x + g11 * E1.x) * wA, (g10 * E0.y + g11 * E1.y) * wA,
-                     (g10 * E0.z + g11 * E1.z) * wA));
+        st3(q, 0, c[0]);
+        st3(q, 1, c[1]);
+        st3(q, 2, c[2]);
     }
 }
 
@@ -168,18 +166,18 @@ k_hinges(const float* __restrict__ base, const float* __restrict__ x,
         acc.z += W * p0.z;
     } else {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const float wi = w[(size_t)e * 4 + i];
-            const f3 p = ld3(x, quads[(size_t)e * 4 + i]);
-            acc.x += wi * p.x;
-            acc.y += wi * p.y;
-            acc.z += wi * p.z;
-        }
+        for (int i = 0; i < 4; ++i)
+            acc = madd_nc(acc, w[(size_t)e * 4 + i], ld3(x, quads[(size_t)e * 4 + i]));
     }
     const float nr = nrest[e];
-    const float len = sqrtf(dot(acc, acc));
-    f3 p = mk(0.0f, 0.0f, 0.0f);
-    if (len > 1e-10f && nr > 1e-10f) p = acc * (nr / len);
+    f3 p;
+    if (kMode == CLH_MODE_PD) {
+        const float len = sqrtf(dot(acc, acc));
+        p = mk(0.0f, 0.0f, 0.0f);
+        if (len > 1e-10f && nr > 1e-10f) p = acc * (nr / len);
+    } else {
+        p = hinge_projection_nc(acc, nr);
+    }
     st3(bp, e, p);
     if (kMode == CLH_MODE_PD) st3(bd, e, p - acc);
 }
@@ -232,12 +230,10 @@ __global__ void __launch_bounds__(kBlock) k_vertex_ref(VertexArgs a)
         for (int j = a.ve_ptr[v]; j < a.ve_ptr[v + 1]; ++j) {
             const int s = a.ve_idx[j];
             const float kw = (a.bk[s >> 2] * a.h2) * a.bw[s];
-            const f3 p = ld3(a.bp, s >> 2);
-            acc = acc + mk(kw * p.x, kw * p.y, kw * p.z);
+            acc = madd_nc(acc, kw, ld3(a.bp, s >> 2));
         }
     const float m = a.mass[v];
-    const f3 yv = ld3(a.y, v);
-    const f3 rhs = mk(acc.x + m * yv.x, acc.y + m * yv.y, acc.z + m * yv.z);
+    const f3 rhs = madd_nc(acc, m, ld3(a.y, v));
     const float d = a.diag[v];
     const float inv = (d > 1e-10f) ? (1.0f / d) : 0.0f;
     st3(a.out, v, rhs * inv);
