@@ -19,9 +19,9 @@
 namespace clh {
 
 struct FusedIterArgs {
-    const float* x;        // iterate k (tile order)
-    const float* xkm1;     // iterate k-1
-    float*       out;      // iterate k+1
+    const void*  x;        // iterate k (tile order): float (reference) or double (PD)
+    const void*  xkm1;     // iterate k-1
+    void*        out;      // iterate k+1
     const float* base;     // x_n (tile order; PD iterates are displacements)
     const float* y;        // y (reference mode) or y - x_n (PD), tile order
     const float* mass;     // tile order
@@ -73,12 +73,12 @@ void fused_plan_release(FusedPlan& p);
 
 // Frame prologue / epilogue in tile order (gather from and scatter to the
 // caller's original-order arrays).
-int fused_prologue(const FusedPlan& p, const clh_step_args& a, float* xprev, float* y, float* x0,
+int fused_prologue(const FusedPlan& p, const clh_step_args& a, float* xprev, float* y, void* x0,
                    float* mass, float* diag, int* pin_map, float dt, const float h2a[3],
                    int disp, hipStream_t stream);
-int fused_epilogue(const FusedPlan& p, const clh_step_args& a, const float* xf, const float* xprev,
+int fused_epilogue(const FusedPlan& p, const clh_step_args& a, const void* xf, const float* xprev,
                    const int* pin_map, float scale, int disp, hipStream_t stream);
-int fused_trace(const FusedPlan& p, const float* it, const float* xprev, float* out, int disp,
+int fused_trace(const FusedPlan& p, const void* it, const float* xprev, float* out, int disp,
                 hipStream_t stream);
 
 }  // namespace clh

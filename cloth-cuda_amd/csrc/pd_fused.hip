@@ -143,6 +143,16 @@ __device__ __forceinline__ void sts3(float* s, int i, f3 v)
     s[3 * i + 1] = v.y;
     s[3 * i + 2] = v.z;
 }
+__device__ __forceinline__ d3 lds3d(const double* s, int i)
+{
+    return d3{s[3 * i], s[3 * i + 1], s[3 * i + 2]};
+}
+__device__ __forceinline__ void sts3d(double* s, int i, d3 v)
+{
+    s[3 * i] = v.x;
+    s[3 * i + 1] = v.y;
+    s[3 * i + 2] = v.z;
+}
 
 __device__ void fused_block_sum2(double r, double b, double* out, int slot)
 {
@@ -189,23 +199,34 @@ __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
     const int vbeg = ti[0], nown = ti[1], hbeg = ti[2], nhalo = ti[3];
     const int tbeg = ti[4], ntri = ti[5], ebeg = ti[6], nhin = ti[7];
 
-    float* sb = smem;                                  // base     [lmax*3] (PD)
-    float* sx = sb + (PD ? A.lmax * 3 : 0);            // x / disp [lmax*3]
-    float* st = sx + A.lmax * 3;                       // tri out  [tmax*9]
+    double* sd = reinterpret_cast<double*>(smem);      // disp (double) [lmax*3] (PD)
+    float* sb = smem + (PD ? A.lmax * 6 : 0);          // base     [lmax*3] (PD)
+    float* sx = sb + (PD ? A.lmax * 3 : 0);            // x        [lmax*3] (reference)
+    float* st = sx + (PD ? 0 : A.lmax * 3);            // tri out  [tmax*9]
     float* st2 = st + A.tmax * 9;                      // tri R    [tmax*9] (PD, check)
     float* skw = st2 + ((PD && kCheck) ? A.tmax * 9 : 0);  // hinge k h^2 w_i [hmax*4]
     float* sq = skw + A.hmax * 4;                      // hinge p-e (PD) or p [hmax*3]
     float* sp = sq + A.hmax * 3;                       // hinge p [hmax*3] (PD, check)
 
     // 1. positions of own and halo vertices
+    const float* xf = static_cast<const float*>(a.x);
+    const double* xd = static_cast<const double*>(a.x);
     for (int i = threadIdx.x; i < nown; i += kFB) {
-        sts3(sx, i, ld3(a.x, (size_t)vbeg + i));
-        if (PD) sts3(sb, i, ld3(a.base, (size_t)vbeg + i));
+        if (PD) {
+            sts3d(sd, i, ld3d(xd, (size_t)vbeg + i));
+            sts3(sb, i, ld3(a.base, (size_t)vbeg + i));
+        } else {
+            sts3(sx, i, ld3(xf, (size_t)vbeg + i));
+        }
     }
     for (int i = threadIdx.x; i < nhalo; i += kFB) {
         const int g = A.halo[hbeg + i];
-        sts3(sx, nown + i, ld3(a.x, g));
-        if (PD) sts3(sb, nown + i, ld3(a.base, g));
+        if (PD) {
+            sts3d(sd, nown + i, ld3d(xd, g));
+            sts3(sb, nown + i, ld3(a.base, g));
+        } else {
+            sts3(sx, nown + i, ld3(xf, g));
+        }
     }
     __syncthreads();
 
@@ -214,38 +235,30 @@ __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
         for (int j = threadIdx.x; j < ntri; j += kFB) {
             const uint2 id = A.tri_idx[tbeg + j];
             const int l0 = id.x & 0xFFFF, l1 = id.x >> 16, l2 = id.y & 0xFFFF;
-            f3 d0 = lds3(sx, l1) - lds3(sx, l0);
-            f3 d1 = lds3(sx, l2) - lds3(sx, l0);
-            if (PD) {
-                const f3 b0 = lds3(sb, l0);
-                d0 = (lds3(sb, l1) - b0) + d0;
-                d1 = (lds3(sb, l2) - b0) + d1;
-            }
             const float4 G = A.tri_c0[tbeg + j];
-            const float g00 = G.x, g10 = G.y, g01 = G.z, g11 = G.w;
             const float wA = A.tri_c1[tbeg + j];
-            f3 F0, F1, R0, R1, c[3];
-            if (PD) {
-                F0 = mk(d0.x * g00 + d1.x * g10, d0.y * g00 + d1.y * g10, d0.z * g00 + d1.z * g10);
-                F1 = mk(d0.x * g01 + d1.x * g11, d0.y * g01 + d1.y * g11, d0.z * g01 + d1.z * g11);
-                polar_rotation(F0, F1, R0, R1);
-                corner_forces(g00, g10, g01, g11, wA, R0 - F0, R1 - F1, c);
-            } else {
-                deformation_nc(d0, d1, g00, g10, g01, g11, F0, F1);
-                ref_cuda_rotation(F0, F1, R0, R1);
-                corner_forces_nc(g00, g10, g01, g11, wA, R0, R1, c);
-            }
+            f3 c[3];
             float* o = st + (size_t)j * 9;
+            if (PD) {
+                f3 cr[3];
+                pd_triangle(lds3(sb, l0), lds3(sb, l1), lds3(sb, l2), lds3d(sd, l0), lds3d(sd, l1),
+                            lds3d(sd, l2), G, wA, cr, c, kCheck);
+                if (kCheck) {
+                    float* q = st2 + (size_t)j * 9;
+                    sts3(q, 0, cr[0]);
+                    sts3(q, 1, cr[1]);
+                    sts3(q, 2, cr[2]);
+                }
+            } else {
+                const f3 x0 = lds3(sx, l0);
+                f3 F0, F1, R0, R1;
+                deformation_nc(lds3(sx, l1) - x0, lds3(sx, l2) - x0, G.x, G.y, G.z, G.w, F0, F1);
+                ref_cuda_rotation(F0, F1, R0, R1);
+                corner_forces_nc(G.x, G.y, G.z, G.w, wA, R0, R1, c);
+            }
             sts3(o, 0, c[0]);
             sts3(o, 1, c[1]);
             sts3(o, 2, c[2]);
-            if (PD && kCheck) {
-                corner_forces(g00, g10, g01, g11, wA, R0, R1, c);
-                float* q = st2 + (size_t)j * 9;
-                sts3(q, 0, c[0]);
-                sts3(q, 1, c[1]);
-                sts3(q, 2, c[2]);
-            }
         }
     }
     // 2b. hinges -> bend vectors (src/pd_solver.cu:304-358)
@@ -255,42 +268,30 @@ __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
             const int q0 = id.x & 0xFFFF, q1 = id.x >> 16, q2 = id.y & 0xFFFF, q3 = id.y >> 16;
             const float4 w = A.hin_w[ebeg + j];
             const float2 c = A.hin_c[ebeg + j];  // (n_rest, k h^2)
-            f3 acc = mk(0.0f, 0.0f, 0.0f);
+            f3 p, pe;
             if (PD) {
-                const f3 b0 = lds3(sb, q0), x0 = lds3(sx, q0);
-                const int qs[3] = {q1, q2, q3};
-                const float ws[3] = {w.y, w.z, w.w};
+                const int qs[4] = {q0, q1, q2, q3};
+                f3 b[4];
+                d3 xs[4];
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    const f3 d = (lds3(sb, qs[i]) - b0) + (lds3(sx, qs[i]) - x0);
-                    acc.x += ws[i] * d.x;
-                    acc.y += ws[i] * d.y;
-                    acc.z += ws[i] * d.z;
+                for (int i = 0; i < 4; ++i) {
+                    b[i] = lds3(sb, qs[i]);
+                    xs[i] = lds3d(sd, qs[i]);
                 }
-                const float W = ((w.x + w.y) + w.z) + w.w;
-                const f3 p0 = b0 + x0;
-                acc.x += W * p0.x;
-                acc.y += W * p0.y;
-                acc.z += W * p0.z;
+                pd_hinge(b, xs, w, c.x, p, pe);
             } else {
+                f3 acc = mk(0.0f, 0.0f, 0.0f);
                 acc = madd_nc(acc, w.x, lds3(sx, q0));
                 acc = madd_nc(acc, w.y, lds3(sx, q1));
                 acc = madd_nc(acc, w.z, lds3(sx, q2));
                 acc = madd_nc(acc, w.w, lds3(sx, q3));
-            }
-            f3 p;
-            if (PD) {
-                const float len = sqrtf(dot(acc, acc));
-                p = mk(0.0f, 0.0f, 0.0f);
-                if (len > 1e-10f && c.x > 1e-10f) p = acc * (c.x / len);
-            } else {
                 p = hinge_projection_nc(acc, c.x);
             }
             skw[j * 4 + 0] = c.y * w.x;
             skw[j * 4 + 1] = c.y * w.y;
             skw[j * 4 + 2] = c.y * w.z;
             skw[j * 4 + 3] = c.y * w.w;
-            sts3(sq, j, PD ? (p - acc) : p);
+            sts3(sq, j, PD ? pe : p);
             if (PD && kCheck) sts3(sp, j, p);
         }
     }
@@ -303,7 +304,8 @@ __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
         const int slot = a.pin_map[g];
         if (slot >= 0) {
             const f3 tgt = ld3(a.pin_target, slot);
-            st3(a.out, g, PD ? tgt - lds3(sb, i) : tgt);
+            if (PD) st3d(static_cast<double*>(a.out), g, to_d(tgt) - to_d(lds3(sb, i)));
+            else    st3(static_cast<float*>(a.out), g, tgt);
             continue;
         }
         const int j0 = A.gl_ptr[g], j1 = A.gl_ptr[g + 1];
@@ -329,25 +331,28 @@ __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
             }
         const float m = a.mass[g];
         const float dg = a.diag[g];
-        const float inv = (dg > 1e-10f) ? (1.0f / dg) : 0.0f;
         const f3 yv = ld3(a.y, g);
         if (!PD) {
-            st3(a.out, g, madd_nc(ar, m, yv) * inv);
+            const float inv = (dg > 1e-10f) ? (1.0f / dg) : 0.0f;
+            st3(static_cast<float*>(a.out), g, madd_nc(ar, m, yv) * inv);
             continue;
         }
-        const f3 xv = lds3(sx, i);
-        const f3 r = mk(ar.x + m * (yv.x - xv.x), ar.y + m * (yv.y - xv.y), ar.z + m * (yv.z - xv.z));
-        f3 xn = mk(xv.x + r.x * inv, xv.y + r.y * inv, xv.z + r.z * inv);
+        const d3 xv = lds3d(sd, i);
+        const double md = m;
+        const d3 r = d3{(double)ar.x + md * ((double)yv.x - xv.x),
+                        (double)ar.y + md * ((double)yv.y - xv.y),
+                        (double)ar.z + md * ((double)yv.z - xv.z)};
+        const double inv = (dg > 1e-10f) ? 1.0 / (double)dg : 0.0;
+        d3 xn = xv + r * inv;
         if (a.blend) {
-            const f3 xp = ld3(a.xkm1, g);
-            xn = mk(a.omega * (xn.x - xp.x) + xp.x, a.omega * (xn.y - xp.y) + xp.y,
-                    a.omega * (xn.z - xp.z) + xp.z);
+            const d3 xp = ld3d(static_cast<const double*>(a.xkm1), g);
+            xn = (xn - xp) * (double)a.omega + xp;
         }
-        st3(a.out, g, xn);
+        st3d(static_cast<double*>(a.out), g, xn);
         if (kCheck) {
             const f3 ya = lds3(sb, i) + yv;
             const f3 b = mk(ab.x + m * ya.x, ab.y + m * ya.y, ab.z + m * ya.z);
-            rr += (double)r.x * r.x + (double)r.y * r.y + (double)r.z * r.z;
+            rr += r.x * r.x + r.y * r.y + r.z * r.z;
             bb += (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
         }
     }
@@ -360,7 +365,7 @@ __global__ void __launch_bounds__(256)
 k_fused_prologue(const int* __restrict__ perm, const float* __restrict__ pos,
                  const float* __restrict__ vel, const float* __restrict__ guess,
                  const float* __restrict__ mass_in, const float* __restrict__ diag_in,
-                 float* __restrict__ xprev, float* __restrict__ y, float* __restrict__ x0,
+                 float* __restrict__ xprev, float* __restrict__ y, void* __restrict__ x0,
                  float* __restrict__ mass, float* __restrict__ diag, int* __restrict__ pin_map,
                  int N, float dt, float h2a0, float h2a1, float h2a2, int disp)
 {
@@ -372,10 +377,11 @@ k_fused_prologue(const int* __restrict__ perm, const float* __restrict__ pos,
     const f3 dy = mk(dt * u.x + h2a0, dt * u.y + h2a1, dt * u.z + h2a2);
     if (disp) {
         st3(y, i, dy);
-        st3(x0, i, guess ? ld3(guess, o) - x : mk(0.0f, 0.0f, 0.0f));
+        st3d(static_cast<double*>(x0), i,
+             guess ? to_d(ld3(guess, o)) - to_d(x) : d3{0.0, 0.0, 0.0});
     } else {
         st3(y, i, predict_nc(x, u, dt, h2a0, h2a1, h2a2));
-        st3(x0, i, guess ? ld3(guess, o) : x);
+        st3(static_cast<float*>(x0), i, guess ? ld3(guess, o) : x);
     }
     mass[i] = mass_in[o];
     diag[i] = diag_in[o];
@@ -385,17 +391,18 @@ k_fused_prologue(const int* __restrict__ perm, const float* __restrict__ pos,
 __global__ void __launch_bounds__(256)
 k_fused_pins(const int* __restrict__ inv_perm, const int* __restrict__ idx,
              const float* __restrict__ target, const float* __restrict__ xprev,
-             int* __restrict__ pin_map, float* __restrict__ x0, int P, int disp)
+             int* __restrict__ pin_map, void* __restrict__ x0, int P, int disp)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P) return;
     const int i = inv_perm[idx[k]];
     pin_map[i] = k;
-    if (disp) st3(x0, i, ld3(target, k) - ld3(xprev, i));
+    if (disp)
+        st3d(static_cast<double*>(x0), i, to_d(ld3(target, k)) - to_d(ld3(xprev, i)));
 }
 
 __global__ void __launch_bounds__(256)
-k_fused_epilogue(const int* __restrict__ perm, const float* __restrict__ xf,
+k_fused_epilogue(const int* __restrict__ perm, const void* __restrict__ xf,
                  const float* __restrict__ xprev, const int* __restrict__ pin_map,
                  const float* __restrict__ target, float* __restrict__ pos,
                  float* __restrict__ vel, int N, float scale, int disp)
@@ -409,23 +416,27 @@ k_fused_epilogue(const int* __restrict__ perm, const float* __restrict__ xf,
         st3(vel, o, mk(0.0f, 0.0f, 0.0f));
         return;
     }
-    const f3 x = ld3(xf, i), xp = ld3(xprev, i);
+    const f3 xp = ld3(xprev, i);
     if (disp) {
-        st3(pos, o, xp + x);
-        st3(vel, o, x * scale);
+        const d3 d = ld3d(static_cast<const double*>(xf), i);
+        st3(pos, o, to_f(to_d(xp) + d));
+        st3(vel, o, to_f(d * (double)scale));
     } else {
+        const f3 x = ld3(static_cast<const float*>(xf), i);
         st3(pos, o, x);
         st3(vel, o, mk((x.x - xp.x) * scale, (x.y - xp.y) * scale, (x.z - xp.z) * scale));
     }
 }
 
 __global__ void __launch_bounds__(256)
-k_fused_trace(const int* __restrict__ perm, const float* __restrict__ it,
+k_fused_trace(const int* __restrict__ perm, const void* __restrict__ it,
               const float* __restrict__ xprev, float* __restrict__ out, int N, int disp)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    st3(out, perm[i], disp ? ld3(xprev, i) + ld3(it, i) : ld3(it, i));
+    st3(out, perm[i],
+        disp ? to_f(to_d(ld3(xprev, i)) + ld3d(static_cast<const double*>(it), i))
+             : ld3(static_cast<const float*>(it), i));
 }
 
 __global__ void __launch_bounds__(256)
@@ -455,7 +466,7 @@ k_pack_hinges(const int* __restrict__ gid, const float* __restrict__ w,
 size_t lds_bytes(const FusedPlan& p, int mode, bool check)
 {
     const bool pd = (mode == CLH_MODE_PD);
-    size_t f = (size_t)p.lmax * 3 * (pd ? 2 : 1) + (size_t)p.tmax * 9 * ((pd && check) ? 2 : 1) +
+    size_t f = (size_t)p.lmax * 3 * (pd ? 3 : 1) + (size_t)p.tmax * 9 * ((pd && check) ? 2 : 1) +
                (size_t)p.hmax * (4 + 3 + ((pd && check) ? 3 : 0));
     return f * sizeof(float);
 }
@@ -576,11 +587,12 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
     p.tri_factor = T > 0 ? (double)tri_gid.size() / T : 0.0;
     p.bend_factor = E > 0 ? (double)hin_gid.size() / E : 0.0;
     p.xcd_remap = (ntiles % 8 == 0) ? 1 : 0;
-    // bytes the iteration streams: per vertex x_k, x_{k-1}, base, y, out (12 B
-    // each), mass, diag, pin map, gather offset and count; halo ids and
-    // positions; packed triangle (28 B) and hinge (32 B) records; gather
-    // entries (2 B); tile descriptors
-    p.bytes_per_iter = (double)N * (5 * 12 + 4 + 4 + 4 + 4 + 1) + (double)halo.size() * (4 + 24) +
+    // bytes the PD iteration streams: per vertex the double iterates x_k,
+    // x_{k-1}, x_{k+1} (24 B each), base and dy (12 B each), mass, diag, pin
+    // map, gather offset and count; per halo vertex its id, iterate and
+    // base; packed triangle (28 B) and hinge (32 B) records; gather entries
+    // (2 B); tile descriptors
+    p.bytes_per_iter = (double)N * (3 * 24 + 2 * 12 + 4 + 4 + 4 + 4 + 1) + (double)halo.size() * (4 + 36) +
                        (double)tri_gid.size() * 28 + (double)hin_gid.size() * 32 +
                        (double)gl_idx.size() * 2 + (double)ntiles * kTileInfo * 4;
 
@@ -763,7 +775,7 @@ int fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
 
 size_t fused_lds_bytes(const FusedPlan& p, int mode, bool check) { return lds_bytes(p, mode, check); }
 
-int fused_prologue(const FusedPlan& p, const clh_step_args& a, float* xprev, float* y, float* x0,
+int fused_prologue(const FusedPlan& p, const clh_step_args& a, float* xprev, float* y, void* x0,
                    float* mass, float* diag, int* pin_map, float dt, const float h2a[3],
                    int disp, hipStream_t stream)
 {
@@ -777,7 +789,7 @@ int fused_prologue(const FusedPlan& p, const clh_step_args& a, float* xprev, flo
     return hip_status(hipGetLastError(), "fused prologue launch");
 }
 
-int fused_epilogue(const FusedPlan& p, const clh_step_args& a, const float* xf, const float* xprev,
+int fused_epilogue(const FusedPlan& p, const clh_step_args& a, const void* xf, const float* xprev,
                    const int* pin_map, float scale, int disp, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_fused_epilogue, dim3(blocks_for(p.N)), dim3(256), 0, stream, p.perm.ptr,
@@ -785,7 +797,7 @@ int fused_epilogue(const FusedPlan& p, const clh_step_args& a, const float* xf, 
     return hip_status(hipGetLastError(), "fused epilogue launch");
 }
 
-int fused_trace(const FusedPlan& p, const float* it, const float* xprev, float* out, int disp,
+int fused_trace(const FusedPlan& p, const void* it, const float* xprev, float* out, int disp,
                 hipStream_t stream)
 {
     hipLaunchKernelGGL(k_fused_trace, dim3(blocks_for(p.N)), dim3(256), 0, stream, p.perm.ptr, it,

@@ -189,6 +189,123 @@ __device__ __forceinline__ void ref_cuda_rotation(f3 F0, f3 F1, f3& R0, f3& R1)
     R1 = mk(u1.x * r01 + u2.x * r11, u1.y * r01 + u2.y * r11, u1.z * r01 + u2.z * r11);
 }
 
+// ---------------------------------------------------------------------------
+// PD-mode element arithmetic in double. Storage stays FP32; the element math
+// (edge vectors, F, the polar rotation, R - F, the hinge curvature) runs in
+// double because R - F and sum w_i x_i cancel to strain-sized values (1e-4
+// of their O(1) operands), which FP32 resolves only to ~1e-3, and that
+// rounding noise, amplified by the Chebyshev extrapolation, holds the
+// relative residual near 1e-5 at 64x64 and above it at 256x256.
+// ---------------------------------------------------------------------------
+
+struct d3 {
+    double x, y, z;
+};
+__device__ __forceinline__ d3 operator+(d3 a, d3 b) { return d3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 operator-(d3 a, d3 b) { return d3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 operator*(d3 a, double s) { return d3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3 to_d(f3 a) { return d3{(double)a.x, (double)a.y, (double)a.z}; }
+__device__ __forceinline__ f3 to_f(d3 a) { return f3{(float)a.x, (float)a.y, (float)a.z}; }
+__device__ __forceinline__ d3 ld3d(const double* __restrict__ p, size_t i)
+{
+    return d3{p[i * 3 + 0], p[i * 3 + 1], p[i * 3 + 2]};
+}
+__device__ __forceinline__ void st3d(double* __restrict__ p, size_t i, d3 v)
+{
+    p[i * 3 + 0] = v.x;
+    p[i * 3 + 1] = v.y;
+    p[i * 3 + 2] = v.z;
+}
+
+// 1/sqrt(q) for q in the normal FP32 range: FP32 hardware estimate refined
+// by one Newton step in double (relative error ~1e-14).
+__device__ __forceinline__ double rsqrt_d(double q)
+{
+    const double y = (double)__builtin_amdgcn_rsqf((float)q);
+    return y * (1.5 - 0.5 * q * y * y);
+}
+
+// Closest rotation on the Stiefel manifold, in double: the same frame and
+// 2-D rotation as polar_rotation below.
+__device__ __forceinline__ void polar_rotation_d(d3 F0, d3 F1, d3& R0, d3& R1)
+{
+    const double q0 = dot(F0, F0);
+    const d3 u1 = (q0 > 1e-20) ? F0 * rsqrt_d(q0) : d3{1.0, 0.0, 0.0};
+    const double a = dot(F1, u1);
+    const d3 c = F1 - u1 * a;
+    const double qc = dot(c, c);
+    d3 u2;
+    if (qc > 1e-20) {
+        u2 = c * rsqrt_d(qc);
+    } else {
+        u2 = d3{-u1.y, u1.x, 0.0};
+        if (dot(u2, u2) < 1e-20) u2 = d3{0.0, -u1.z, u1.y};
+        u2 = u2 * rsqrt_d(dot(u2, u2));
+    }
+    const double a00 = dot(u1, F0), a01 = dot(u1, F1);
+    const double a10 = dot(u2, F0), a11 = dot(u2, F1);
+    double cs = a00 + a11, sn = a10 - a01;
+    const double qn = cs * cs + sn * sn;
+    if (qn > 1e-30) {
+        const double r = rsqrt_d(qn);
+        cs *= r;
+        sn *= r;
+    } else {
+        cs = 1.0;
+        sn = 0.0;
+    }
+    R0 = u1 * cs + u2 * sn;
+    R1 = u2 * cs - u1 * sn;
+}
+
+// PD triangle: corner forces of R (fr) and of R - F (fd) from frame-start
+// positions b and displacements x of the three corners, all in double,
+// rounded to FP32 at the end.
+__device__ __forceinline__ void pd_triangle(f3 b0, f3 b1, f3 b2, d3 x0, d3 x1, d3 x2, float4 G,
+                                            float wAf, f3 fr[3], f3 fd[3], bool want_fr)
+{
+    const d3 db0 = to_d(b0);
+    const d3 d0 = (to_d(b1) - db0) + (x1 - x0);
+    const d3 d1 = (to_d(b2) - db0) + (x2 - x0);
+    const double g00 = G.x, g10 = G.y, g01 = G.z, g11 = G.w, wA = wAf;
+    const d3 F0 = d0 * g00 + d1 * g10;
+    const d3 F1 = d0 * g01 + d1 * g11;
+    d3 R0, R1;
+    polar_rotation_d(F0, F1, R0, R1);
+    const double s0 = g00 + g10, s1 = g01 + g11;
+    const d3 E0 = R0 - F0, E1 = R1 - F1;
+    fd[0] = to_f((E0 * s0 + E1 * s1) * (-wA));
+    fd[1] = to_f((E0 * g00 + E1 * g01) * wA);
+    fd[2] = to_f((E0 * g10 + E1 * g11) * wA);
+    if (want_fr) {
+        fr[0] = to_f((R0 * s0 + R1 * s1) * (-wA));
+        fr[1] = to_f((R0 * g00 + R1 * g01) * wA);
+        fr[2] = to_f((R0 * g10 + R1 * g11) * wA);
+    }
+}
+
+// PD hinge: e = sum_{i>0} w_i ((b_i - b_0) + (x_i - x_0)) + (sum w_i)(b_0 + x_0)
+// in double (sum w_i is zero up to the FP32 rounding of the weights, and is
+// summed in double so that no rounding of it enters e); p = e n_rest / |e|
+// (or 0); returns p and p - e in FP32.
+__device__ __forceinline__ void pd_hinge(const f3 b[4], const d3 x[4], float4 w, float nr, f3& p,
+                                         f3& pe)
+{
+    const d3 b0 = to_d(b[0]), x0 = x[0];
+    const double ws[4] = {w.x, w.y, w.z, w.w};
+    d3 e = d3{0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 1; i < 4; ++i) e = e + ((to_d(b[i]) - b0) + (x[i] - x0)) * ws[i];
+    const double W = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+    e = e + (b0 + x0) * W;
+    const double q = dot(e, e);
+    d3 pd = d3{0.0, 0.0, 0.0};
+    if (q > 1e-20 && nr > 1e-10f) pd = e * ((double)nr * rsqrt_d(q));
+    p = to_f(pd);
+    pe = to_f(pd - e);
+}
+
 // Corner forces wA D^T [A0 A1] (contraction allowed; PD mode).
 __device__ __forceinline__ void corner_forces(float g00, float g10, float g01, float g11,
                                               float wA, f3 A0, f3 A1, f3 c[3])

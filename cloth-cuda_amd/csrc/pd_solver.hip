@@ -49,7 +49,7 @@ inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 __global__ void __launch_bounds__(kBlock)
 k_prologue(const float* __restrict__ pos, const float* __restrict__ vel,
            const float* __restrict__ guess, float* __restrict__ xprev, float* __restrict__ y,
-           float* __restrict__ x0, int* __restrict__ pin_map, int N, float dt, float h2a0,
+           void* __restrict__ x0, int* __restrict__ pin_map, int N, float dt, float h2a0,
            float h2a1, float h2a2, int disp)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -60,10 +60,11 @@ k_prologue(const float* __restrict__ pos, const float* __restrict__ vel,
     const f3 dy = mk(dt * u.x + h2a0, dt * u.y + h2a1, dt * u.z + h2a2);
     if (disp) {
         st3(y, v, dy);
-        st3(x0, v, guess ? ld3(guess, v) - x : mk(0.0f, 0.0f, 0.0f));
+        st3d(static_cast<double*>(x0), v,
+             guess ? to_d(ld3(guess, v)) - to_d(x) : d3{0.0, 0.0, 0.0});
     } else {
         st3(y, v, predict_nc(x, u, dt, h2a0, h2a1, h2a2));
-        st3(x0, v, guess ? ld3(guess, v) : x);
+        st3(static_cast<float*>(x0), v, guess ? ld3(guess, v) : x);
     }
     pin_map[v] = -1;
 }
@@ -73,13 +74,13 @@ k_prologue(const float* __restrict__ pos, const float* __restrict__ vel,
 __global__ void __launch_bounds__(kBlock)
 k_pin_scatter(const int* __restrict__ idx, const float* __restrict__ target,
               const float* __restrict__ xprev, int* __restrict__ pin_map,
-              float* __restrict__ x0, int P, int disp)
+              void* __restrict__ x0, int P, int disp)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     const int v = idx[i];
     pin_map[v] = i;
-    if (disp) st3(x0, v, ld3(target, i) - ld3(xprev, v));
+    if (disp) st3d(static_cast<double*>(x0), v, to_d(ld3(target, i)) - to_d(ld3(xprev, v)));
 }
 
 // Per triangle: deformation gradient, rotation, and the three corner forces.
@@ -89,7 +90,7 @@ k_pin_scatter(const int* __restrict__ idx, const float* __restrict__ target,
 // PD mode reads positions as base + displacement x.
 template <int kMode>
 __global__ void __launch_bounds__(kBlock)
-k_triangles(const float* __restrict__ base, const float* __restrict__ x,
+k_triangles(const float* __restrict__ base, const void* __restrict__ xv,
             const int* __restrict__ tris, const float* __restrict__ dminv,
             const float* __restrict__ area, const float* __restrict__ kst, float* __restrict__ fr,
             float* __restrict__ fd, int T, float h2)
@@ -98,38 +99,30 @@ k_triangles(const float* __restrict__ base, const float* __restrict__ x,
     if (t >= T) return;
     const int i0 = tris[(size_t)t * 3 + 0], i1 = tris[(size_t)t * 3 + 1],
               i2 = tris[(size_t)t * 3 + 2];
-    const f3 p0 = ld3(x, i0), p1 = ld3(x, i1), p2 = ld3(x, i2);
-    f3 d0 = p1 - p0, d1 = p2 - p0;
-    if (kMode == CLH_MODE_PD) {
-        const f3 b0 = ld3(base, i0);
-        d0 = (ld3(base, i1) - b0) + d0;
-        d1 = (ld3(base, i2) - b0) + d1;
-    }
     const float g00 = dminv[(size_t)t * 4 + 0], g10 = dminv[(size_t)t * 4 + 1];
     const float g01 = dminv[(size_t)t * 4 + 2], g11 = dminv[(size_t)t * 4 + 3];
     const float wA = kst[t] * area[t] * h2;
-    f3 F0, F1, R0, R1, c[3];
-    if (kMode == CLH_MODE_REF_CUDA) {
-        deformation_nc(d0, d1, g00, g10, g01, g11, F0, F1);
+    f3 c[3], e[3];
+    if (kMode == CLH_MODE_PD) {
+        const double* x = static_cast<const double*>(xv);
+        pd_triangle(ld3(base, i0), ld3(base, i1), ld3(base, i2), ld3d(x, i0), ld3d(x, i1),
+                    ld3d(x, i2), make_float4(g00, g10, g01, g11), wA, c, e, true);
+        float* q = fd + (size_t)t * 9;
+        st3(q, 0, e[0]);
+        st3(q, 1, e[1]);
+        st3(q, 2, e[2]);
+    } else {
+        const float* x = static_cast<const float*>(xv);
+        const f3 p0 = ld3(x, i0), p1 = ld3(x, i1), p2 = ld3(x, i2);
+        f3 F0, F1, R0, R1;
+        deformation_nc(p1 - p0, p2 - p0, g00, g10, g01, g11, F0, F1);
         ref_cuda_rotation(F0, F1, R0, R1);
         corner_forces_nc(g00, g10, g01, g11, wA, R0, R1, c);
-    } else {
-        F0 = mk(d0.x * g00 + d1.x * g10, d0.y * g00 + d1.y * g10, d0.z * g00 + d1.z * g10);
-        F1 = mk(d0.x * g01 + d1.x * g11, d0.y * g01 + d1.y * g11, d0.z * g01 + d1.z * g11);
-        polar_rotation(F0, F1, R0, R1);
-        corner_forces(g00, g10, g01, g11, wA, R0, R1, c);
     }
     float* o = fr + (size_t)t * 9;
     st3(o, 0, c[0]);
     st3(o, 1, c[1]);
     st3(o, 2, c[2]);
-    if (kMode == CLH_MODE_PD) {
-        corner_forces(g00, g10, g01, g11, wA, R0 - F0, R1 - F1, c);
-        float* q = fd + (size_t)t * 9;
-        st3(q, 0, c[0]);
-        st3(q, 1, c[1]);
-        st3(q, 2, c[2]);
-    }
 }
 
 // Per hinge: curvature vector e = sum w_i x_i and its projection
@@ -139,54 +132,43 @@ k_triangles(const float* __restrict__ base, const float* __restrict__ x,
 // sum_{i>0} w_i ((b_i - b_0) + (x_i - x_0)) + (sum_i w_i)(b_0 + x_0).
 template <int kMode>
 __global__ void __launch_bounds__(kBlock)
-k_hinges(const float* __restrict__ base, const float* __restrict__ x,
+k_hinges(const float* __restrict__ base, const void* __restrict__ xv,
          const int* __restrict__ quads, const float* __restrict__ w,
          const float* __restrict__ nrest, float* __restrict__ bp, float* __restrict__ bd, int E)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
-    f3 acc = mk(0.0f, 0.0f, 0.0f);
     if (kMode == CLH_MODE_PD) {
-        const int q0 = quads[(size_t)e * 4];
-        const f3 b0 = ld3(base, q0), x0 = ld3(x, q0);
+        const double* x = static_cast<const double*>(xv);
+        f3 b[4];
+        d3 xs[4];
 #pragma unroll
-        for (int i = 1; i < 4; ++i) {
+        for (int i = 0; i < 4; ++i) {
             const int qi = quads[(size_t)e * 4 + i];
-            const float wi = w[(size_t)e * 4 + i];
-            const f3 d = (ld3(base, qi) - b0) + (ld3(x, qi) - x0);
-            acc.x += wi * d.x;
-            acc.y += wi * d.y;
-            acc.z += wi * d.z;
+            b[i] = ld3(base, qi);
+            xs[i] = ld3d(x, qi);
         }
-        const float W = ((w[(size_t)e * 4] + w[(size_t)e * 4 + 1]) + w[(size_t)e * 4 + 2]) +
-                        w[(size_t)e * 4 + 3];
-        const f3 p0 = b0 + x0;
-        acc.x += W * p0.x;
-        acc.y += W * p0.y;
-        acc.z += W * p0.z;
-    } else {
+        const float4 wv = make_float4(w[(size_t)e * 4], w[(size_t)e * 4 + 1], w[(size_t)e * 4 + 2],
+                                      w[(size_t)e * 4 + 3]);
+        f3 p, pe;
+        pd_hinge(b, xs, wv, nrest[e], p, pe);
+        st3(bp, e, p);
+        st3(bd, e, pe);
+        return;
+    }
+    const float* x = static_cast<const float*>(xv);
+    f3 acc = mk(0.0f, 0.0f, 0.0f);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            acc = madd_nc(acc, w[(size_t)e * 4 + i], ld3(x, quads[(size_t)e * 4 + i]));
-    }
-    const float nr = nrest[e];
-    f3 p;
-    if (kMode == CLH_MODE_PD) {
-        const float len = sqrtf(dot(acc, acc));
-        p = mk(0.0f, 0.0f, 0.0f);
-        if (len > 1e-10f && nr > 1e-10f) p = acc * (nr / len);
-    } else {
-        p = hinge_projection_nc(acc, nr);
-    }
-    st3(bp, e, p);
-    if (kMode == CLH_MODE_PD) st3(bd, e, p - acc);
+    for (int i = 0; i < 4; ++i)
+        acc = madd_nc(acc, w[(size_t)e * 4 + i], ld3(x, quads[(size_t)e * 4 + i]));
+    st3(bp, e, hinge_projection_nc(acc, nrest[e]));
 }
 
 struct VertexArgs {
-    const float* base;    // x_n (PD: iterates are displacements from it)
-    const float* x;       // x_k
-    const float* xkm1;    // x_{k-1} (Chebyshev)
-    float*       out;     // x_{k+1}
+    const float* base;    // x_n (PD: iterates are FP64 displacements from it)
+    const void*  x;       // x_k     (float in reference mode, double in PD mode)
+    const void*  xkm1;    // x_{k-1} (Chebyshev)
+    void*        out;     // x_{k+1}
     const float* y;       // y (reference mode) or y - x_n (PD)
     const float* mass;
     const float* diag;
@@ -219,8 +201,9 @@ __global__ void __launch_bounds__(kBlock) k_vertex_ref(VertexArgs a)
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= a.N) return;
     const int slot = a.pin_map[v];
+    float* out = static_cast<float*>(a.out);
     if (slot >= 0) {
-        st3(a.out, v, ld3(a.pin_target, slot));
+        st3(out, v, ld3(a.pin_target, slot));
         return;
     }
     f3 acc = mk(0.0f, 0.0f, 0.0f);
@@ -236,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_vertex_ref(VertexArgs a)
     const f3 rhs = madd_nc(acc, m, ld3(a.y, v));
     const float d = a.diag[v];
     const float inv = (d > 1e-10f) ? (1.0f / d) : 0.0f;
-    st3(a.out, v, rhs * inv);
+    st3(out, v, rhs * inv);
 }
 
 // Block-wide sum of two doubles; thread 0 writes them to out[2*blockIdx.x].
@@ -270,12 +253,14 @@ __device__ void block_sum2(double r, double b, double* out)
 __global__ void __launch_bounds__(kBlock) k_vertex_pd(VertexArgs a)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const double* x = static_cast<const double*>(a.x);
+    double* out = static_cast<double*>(a.out);
     double rr = 0.0, bb = 0.0;
     if (v < a.N) {
         const int slot = a.pin_map[v];
         const f3 bv = ld3(a.base, v);
         if (slot >= 0) {
-            st3(a.out, v, ld3(a.pin_target, slot) - bv);
+            st3d(out, v, to_d(ld3(a.pin_target, slot)) - to_d(bv));
         } else {
             f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
             if (a.use_stretch)
@@ -293,22 +278,24 @@ __global__ void __launch_bounds__(kBlock) k_vertex_pd(VertexArgs a)
                     ab = ab + mk(kw * p.x, kw * p.y, kw * p.z);
                 }
             const float m = a.mass[v];
-            const f3 xv = ld3(a.x, v), yv = ld3(a.y, v);
-            const f3 r = mk(ar.x + m * (yv.x - xv.x), ar.y + m * (yv.y - xv.y),
-                            ar.z + m * (yv.z - xv.z));
+            const f3 yv = ld3(a.y, v);
+            const d3 xv = ld3d(x, v);
+            const double md = m;
+            const d3 r = d3{(double)ar.x + md * ((double)yv.x - xv.x),
+                            (double)ar.y + md * ((double)yv.y - xv.y),
+                            (double)ar.z + md * ((double)yv.z - xv.z)};
             const float d = a.diag[v];
-            const float inv = (d > 1e-10f) ? (1.0f / d) : 0.0f;
-            f3 xn = mk(xv.x + r.x * inv, xv.y + r.y * inv, xv.z + r.z * inv);
+            const double inv = (d > 1e-10f) ? 1.0 / (double)d : 0.0;
+            d3 xn = xv + r * inv;
             if (a.blend) {
-                const f3 xp = ld3(a.xkm1, v);
-                xn = mk(a.omega * (xn.x - xp.x) + xp.x, a.omega * (xn.y - xp.y) + xp.y,
-                        a.omega * (xn.z - xp.z) + xp.z);
+                const d3 xp = ld3d(static_cast<const double*>(a.xkm1), v);
+                xn = (xn - xp) * (double)a.omega + xp;
             }
-            st3(a.out, v, xn);
+            st3d(out, v, xn);
             if (a.partials) {
                 const f3 ya = bv + yv;
                 const f3 b = mk(ab.x + m * ya.x, ab.y + m * ya.y, ab.z + m * ya.z);
-                rr = (double)r.x * r.x + (double)r.y * r.y + (double)r.z * r.z;
+                rr = r.x * r.x + r.y * r.y + r.z * r.z;
                 bb = (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
             }
         }
@@ -347,7 +334,7 @@ k_reduce_partials(const double* __restrict__ partials, int nblocks, double* __re
 // zero velocity (src/pd_solver.cu:400-412, 602-618). In PD mode xf holds the
 // displacement d, so x_{n+1} = x_n + d and v = d (1 - d)/h.
 __global__ void __launch_bounds__(kBlock)
-k_epilogue(const float* __restrict__ xf, const float* __restrict__ xprev,
+k_epilogue(const void* __restrict__ xf, const float* __restrict__ xprev,
            const int* __restrict__ pin_map, const float* __restrict__ target,
            float* __restrict__ pos, float* __restrict__ vel, int N, float scale, int disp)
 {
@@ -359,11 +346,13 @@ k_epilogue(const float* __restrict__ xf, const float* __restrict__ xprev,
         st3(vel, v, mk(0.0f, 0.0f, 0.0f));
         return;
     }
-    const f3 x = ld3(xf, v), xp = ld3(xprev, v);
+    const f3 xp = ld3(xprev, v);
     if (disp) {
-        st3(pos, v, xp + x);
-        st3(vel, v, x * scale);
+        const d3 d = ld3d(static_cast<const double*>(xf), v);
+        st3(pos, v, to_f(to_d(xp) + d));
+        st3(vel, v, to_f(d * (double)scale));
     } else {
+        const f3 x = ld3(static_cast<const float*>(xf), v);
         st3(pos, v, x);
         st3(vel, v, mk((x.x - xp.x) * scale, (x.y - xp.y) * scale, (x.z - xp.z) * scale));
     }
@@ -371,12 +360,14 @@ k_epilogue(const float* __restrict__ xf, const float* __restrict__ xprev,
 
 // Absolute positions of an iterate into a trace slot.
 __global__ void __launch_bounds__(kBlock)
-k_trace(const float* __restrict__ it, const float* __restrict__ xprev, float* __restrict__ out,
+k_trace(const void* __restrict__ it, const float* __restrict__ xprev, float* __restrict__ out,
         int N, int disp)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
-    st3(out, v, disp ? ld3(xprev, v) + ld3(it, v) : ld3(it, v));
+    st3(out, v,
+        disp ? to_f(to_d(ld3(xprev, v)) + ld3d(static_cast<const double*>(it), v))
+             : ld3(static_cast<const float*>(it), v));
 }
 
 // ---------------------------------------------------------------------------
@@ -427,7 +418,8 @@ struct clh_solver {
     DevBuf<float> mass_t, diag_t;  // tile-order copies (fused variant)
 
     // scratch
-    DevBuf<float> y, xprev, it[3], fr, fd, bp, bd;
+    DevBuf<float> y, xprev, fr, fd, bp, bd;
+    DevBuf<double> it[3];  // iterates: float triples (reference) or double (PD)
     DevBuf<int> pin_map;
     DevBuf<double> partials, hist;
     int nblocks_v = 0;
@@ -513,6 +505,7 @@ int ensure_scratch(clh_solver* s)
     const size_t N = (size_t)s->N, T = (size_t)s->T, E = (size_t)s->E;
     CLH_TRY(s->y.resize(N * 3));
     CLH_TRY(s->xprev.resize(N * 3));
+    // float iterates in reference mode need half of this
     for (auto& b : s->it) CLH_TRY(b.resize(N * 3));
     CLH_TRY(s->pin_map.resize(N));
     if (s->variant == CLH_KERNEL_FUSED) {
@@ -583,16 +576,16 @@ int enqueue_prologue(clh_solver* s, const clh_step_args& a)
 }
 
 // Enqueues iteration k (x_k in it[k%3] -> x_{k+1} in it[(k+1)%3]).
-int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega, bool check,
-                      int check_slot)
+int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega, bool blend_in,
+                      bool check, int check_slot)
 {
     const clh_config& c = s->cfg;
     const float h2 = c.dt * c.dt;
     const int pd = (c.mode == CLH_MODE_PD);
-    const float* xk = s->it[k % 3].ptr;
+    const void* xk = s->it[k % 3].ptr;
     const bool use_stretch = s->T > 0 && a.d_stretch_k != nullptr;
     const bool use_bend = s->E > 0 && a.d_bend_quads != nullptr;
-    const bool blend = pd && c.use_chebyshev && k > 0 && omega != 1.0f;
+    const bool blend = pd && c.use_chebyshev && blend_in && omega != 1.0f;
 
     if (s->variant == CLH_KERNEL_FUSED) {
         FusedIterArgs f;
@@ -715,7 +708,7 @@ int enqueue_fixed_frame(clh_solver* s, const clh_step_args& a)
     for (int k = 0; k < K; ++k) {
         omega = chebyshev_omega(s->cfg, k, omega);
         const bool check = is_check_iteration(s->cfg, k);
-        CLH_TRY(enqueue_iteration(s, a, k, omega, check, check ? checks : 0));
+        CLH_TRY(enqueue_iteration(s, a, k, omega, k > 0, check, check ? checks : 0));
         if (check) ++checks;
         if (a.d_trace) CLH_TRY(enqueue_trace(s, a, k));
     }
@@ -787,16 +780,20 @@ int step_impl(clh_solver* s, const clh_step_args& a)
     if (fixed) return enqueue_fixed_frame(s, a);
 
     // PD mode with a tolerance: residual tests every check_every iterations,
-    // read back to decide whether to continue.
+    // read back to decide whether to continue. The Chebyshev sequence
+    // restarts (one plain Jacobi step, then the weights from the start)
+    // whenever a test finds the residual above the previous test's.
     CLH_TRY(enqueue_prologue(s, a));
     float omega = 1.0f;
-    int checks = 0, k = 0;
+    int checks = 0, k = 0, ck = 0;
+    double last_r2 = -1.0;
     s->converged = 0;
     const double tol2 = (double)c.tolerance * (double)c.tolerance;
     for (; k < c.max_iterations; ++k) {
-        omega = chebyshev_omega(c, k, omega);
+        omega = chebyshev_omega(c, ck, omega);
         const bool check = is_check_iteration(c, k);
-        CLH_TRY(enqueue_iteration(s, a, k, omega, check, checks));
+        CLH_TRY(enqueue_iteration(s, a, k, omega, ck > 0, check, checks));
+        ++ck;
         if (a.d_trace) CLH_TRY(enqueue_trace(s, a, k));
         if (check) {
             CLH_HIP(hipMemcpyAsync(s->h_hist + 2 * checks, s->hist.ptr + 2 * checks,
@@ -810,6 +807,8 @@ int step_impl(clh_solver* s, const clh_step_args& a)
                 ++k;
                 break;
             }
+            if (c.use_chebyshev && last_r2 >= 0.0 && r2 > last_r2) ck = 0;
+            last_r2 = r2;
         }
     }
     s->last_iters = k;
@@ -997,10 +996,11 @@ extern "C" int clh_solver_time_iteration_kernel(clh_solver* s, const clh_step_ar
     // a fresh prologue leaves a consistent iterate in it[0]; the launches
     // below re-run iteration 0 -> 1 without touching the mesh state
     CLH_TRY(enqueue_prologue(s, *a));
-    CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, 0));  // warm-up
+    CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, false, 0));  // warm-up
     CLH_HIP(hipStreamSynchronize(s->stream));
     CLH_HIP(hipEventRecord(s->ev_t0, s->stream));
-    for (int i = 0; i < num_launches; ++i) CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, 0));
+    for (int i = 0; i < num_launches; ++i)
+        CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, false, 0));
     CLH_HIP(hipEventRecord(s->ev_t1, s->stream));
     CLH_HIP(hipEventSynchronize(s->ev_t1));
     float ms = 0.0f;

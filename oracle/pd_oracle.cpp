@@ -221,45 +221,78 @@ struct Solver {
                   (R)m.pin_target[3 * slot + 2]);
     }
 
-    // corner forces h^2 wA D^T R (fr) and, for PD, h^2 wA D^T (R - F) (fd);
-    // src/pd_solver.cu:256-298 for the layout of D. Positions are x when
-    // base is null, else base + x with edge vectors formed as
-    // (base_j - base_0) + (x_j - x_0), which keeps them accurate when x is a
-    // small displacement from the frame-start positions.
-    void triangles(const R* base, const R* x, bool pd)
+    // Reference mode: corner forces h^2 wA D^T R of every triangle from
+    // absolute positions x (src/pd_solver.cu:48-298).
+    void triangles_ref(const R* x)
     {
         if (!stretch) return;
 #pragma omp parallel for schedule(static) num_threads(nthreads(&p))
         for (int t = 0; t < T; ++t) {
             const int* tv = m.tris + (size_t)t * 3;
-            V3<R> d0 = sub(ld(x, tv[1]), ld(x, tv[0])), d1 = sub(ld(x, tv[2]), ld(x, tv[0]));
-            if (base) {
-                d0 = add(sub(ld(base, tv[1]), ld(base, tv[0])), d0);
-                d1 = add(sub(ld(base, tv[2]), ld(base, tv[0])), d1);
-            }
+            const V3<R> d0 = sub(ld(x, tv[1]), ld(x, tv[0])), d1 = sub(ld(x, tv[2]), ld(x, tv[0]));
             const R g00 = m.Dm_inv[4 * (size_t)t], g10 = m.Dm_inv[4 * (size_t)t + 1];
             const R g01 = m.Dm_inv[4 * (size_t)t + 2], g11 = m.Dm_inv[4 * (size_t)t + 3];
             const V3<R> F0 = v3(d0.x * g00 + d1.x * g10, d0.y * g00 + d1.y * g10, d0.z * g00 + d1.z * g10);
             const V3<R> F1 = v3(d0.x * g01 + d1.x * g11, d0.y * g01 + d1.y * g11, d0.z * g01 + d1.z * g11);
             V3<R> R0, R1;
-            if (pd) polar_rot(F0, F1, R0, R1);
-            else    ref_cuda_rot(F0, F1, R0, R1);
+            ref_cuda_rot(F0, F1, R0, R1);
             const R wA = (R)m.stretch_k[t] * (R)m.rest_area[t] * h2;
             const R s0 = g00 + g10, s1 = g01 + g11;
-            auto corners = [&](V3<R> A, V3<R> B, R* o) {
-                st(o, 0, v3(-(s0 * A.x + s1 * B.x) * wA, -(s0 * A.y + s1 * B.y) * wA, -(s0 * A.z + s1 * B.z) * wA));
-                st(o, 1, v3((g00 * A.x + g01 * B.x) * wA, (g00 * A.y + g01 * B.y) * wA, (g00 * A.z + g01 * B.z) * wA));
-                st(o, 2, v3((g10 * A.x + g11 * B.x) * wA, (g10 * A.y + g11 * B.y) * wA, (g10 * A.z + g11 * B.z) * wA));
-            };
-            corners(R0, R1, &fr[(size_t)t * 9]);
-            if (pd) corners(sub(R0, F0), sub(R1, F1), &fd[(size_t)t * 9]);
+            R* o = &fr[(size_t)t * 9];
+            st(o, 0, v3(-(s0 * R0.x + s1 * R1.x) * wA, -(s0 * R0.y + s1 * R1.y) * wA, -(s0 * R0.z + s1 * R1.z) * wA));
+            st(o, 1, v3((g00 * R0.x + g01 * R1.x) * wA, (g00 * R0.y + g01 * R1.y) * wA, (g00 * R0.z + g01 * R1.z) * wA));
+            st(o, 2, v3((g10 * R0.x + g11 * R1.x) * wA, (g10 * R0.y + g11 * R1.y) * wA, (g10 * R0.z + g11 * R1.z) * wA));
         }
     }
 
-    // hinge projection p = e n_rest / |e| or 0 (src/pd_solver.cu:304-335).
-    // With a base, e = sum_i w_i (base_i + x_i) is evaluated as
-    // sum_{i>0} w_i ((base_i - base_0) + (x_i - x_0)) + (sum_i w_i)(base_0 + x_0).
-    void hinges(const R* base, const R* x, bool pd)
+    // PD mode: positions are base + x with base the frame-start positions
+    // (null = 0) in the storage precision R and x the displacement, kept in
+    // double. Whatever R is, the element arithmetic (edge vectors, F, the
+    // polar rotation, R - F, the corner forces) runs in double and only the
+    // forces are stored in R: R - F cancels to the strain (1e-4..1e-3 of its
+    // O(1) operands), which single precision resolves only to ~1e-3.
+    void triangles_pd(const R* base, const double* x)
+    {
+        if (!stretch) return;
+        using D = V3<double>;
+#pragma omp parallel for schedule(static) num_threads(nthreads(&p))
+        for (int t = 0; t < T; ++t) {
+            const int* tv = m.tris + (size_t)t * 3;
+            D d0 = sub(ld(x, tv[1]), ld(x, tv[0])), d1 = sub(ld(x, tv[2]), ld(x, tv[0]));
+            if (base) {
+                d0 = add(sub(ldd(base, tv[1]), ldd(base, tv[0])), d0);
+                d1 = add(sub(ldd(base, tv[2]), ldd(base, tv[0])), d1);
+            }
+            const double g00 = m.Dm_inv[4 * (size_t)t], g10 = m.Dm_inv[4 * (size_t)t + 1];
+            const double g01 = m.Dm_inv[4 * (size_t)t + 2], g11 = m.Dm_inv[4 * (size_t)t + 3];
+            const D F0 = v3(d0.x * g00 + d1.x * g10, d0.y * g00 + d1.y * g10, d0.z * g00 + d1.z * g10);
+            const D F1 = v3(d0.x * g01 + d1.x * g11, d0.y * g01 + d1.y * g11, d0.z * g01 + d1.z * g11);
+            D R0, R1;
+            polar_rot(F0, F1, R0, R1);
+            const double wA = (double)((R)m.stretch_k[t] * (R)m.rest_area[t] * h2);
+            const double s0 = g00 + g10, s1 = g01 + g11;
+            auto corners = [&](D A, D B, R* o) {
+                st(o, 0, v3((R)(-(s0 * A.x + s1 * B.x) * wA), (R)(-(s0 * A.y + s1 * B.y) * wA), (R)(-(s0 * A.z + s1 * B.z) * wA)));
+                st(o, 1, v3((R)((g00 * A.x + g01 * B.x) * wA), (R)((g00 * A.y + g01 * B.y) * wA), (R)((g00 * A.z + g01 * B.z) * wA)));
+                st(o, 2, v3((R)((g10 * A.x + g11 * B.x) * wA), (R)((g10 * A.y + g11 * B.y) * wA), (R)((g10 * A.z + g11 * B.z) * wA)));
+            };
+            corners(R0, R1, &fr[(size_t)t * 9]);
+            corners(sub(R0, F0), sub(R1, F1), &fd[(size_t)t * 9]);
+        }
+    }
+
+    static V3<double> ldd(const R* a, int i) { return v3<double>(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+
+    static V3<R> project_hinge(V3<R> acc, R nr)
+    {
+        const R len = std::sqrt(dot3(acc, acc));
+        if (len > thr<R>() && nr > thr<R>()) return scl(acc, nr / len);
+        return v3<R>(0, 0, 0);
+    }
+
+    // Reference mode hinge projection p = e n_rest / |e| or 0
+    // (src/pd_solver.cu:304-335) from absolute positions.
+    void hinges_ref(const R* x)
     {
         if (!bend) return;
 #pragma omp parallel for schedule(static) num_threads(nthreads(&p))
@@ -267,35 +300,44 @@ struct Solver {
             const int* q = m.quads + (size_t)e * 4;
             const float* w = m.bend_w + (size_t)e * 4;
             V3<R> acc = v3<R>(0, 0, 0);
-            if (!base) {
-                for (int i = 0; i < 4; ++i) {
-                    const R wi = w[i];
-                    const V3<R> pi = ld(x, q[i]);
-                    acc.x += wi * pi.x;
-                    acc.y += wi * pi.y;
-                    acc.z += wi * pi.z;
-                }
-            } else {
-                const V3<R> b0 = ld(base, q[0]), x0 = ld(x, q[0]);
-                for (int i = 1; i < 4; ++i) {
-                    const R wi = w[i];
-                    const V3<R> di = add(sub(ld(base, q[i]), b0), sub(ld(x, q[i]), x0));
-                    acc.x += wi * di.x;
-                    acc.y += wi * di.y;
-                    acc.z += wi * di.z;
-                }
-                const R W = (((R)w[0] + (R)w[1]) + (R)w[2]) + (R)w[3];
-                const V3<R> p0 = add(b0, x0);
-                acc.x += W * p0.x;
-                acc.y += W * p0.y;
-                acc.z += W * p0.z;
+            for (int i = 0; i < 4; ++i) {
+                const R wi = w[i];
+                const V3<R> pi = ld(x, q[i]);
+                acc.x += wi * pi.x;
+                acc.y += wi * pi.y;
+                acc.z += wi * pi.z;
             }
-            const R nr = m.bend_n[e];
-            const R len = std::sqrt(dot3(acc, acc));
-            V3<R> pr = v3<R>(0, 0, 0);
-            if (len > thr<R>() && nr > thr<R>()) pr = scl(acc, nr / len);
-            st(bp.data(), e, pr);
-            if (pd) st(bd.data(), e, sub(pr, acc));
+            st(bp.data(), e, project_hinge(acc, (R)m.bend_n[e]));
+        }
+    }
+
+    // PD hinge: e = sum_i w_i (base_i + x_i) in double, evaluated as
+    // sum_{i>0} w_i ((base_i - base_0) + (x_i - x_0)) + (sum_i w_i)(base_0 + x_0)
+    // (sum_i w_i, zero up to the FP32 rounding of the weights, summed in
+    // double); p and p - e stored in R.
+    void hinges_pd(const R* base, const double* x)
+    {
+        if (!bend) return;
+        using D = V3<double>;
+#pragma omp parallel for schedule(static) num_threads(nthreads(&p))
+        for (int e = 0; e < E; ++e) {
+            const int* q = m.quads + (size_t)e * 4;
+            const float* w = m.bend_w + (size_t)e * 4;
+            const D b0 = base ? ldd(base, q[0]) : v3<double>(0, 0, 0), x0 = ld(x, q[0]);
+            D acc = v3<double>(0, 0, 0);
+            for (int i = 1; i < 4; ++i) {
+                const D bi = base ? ldd(base, q[i]) : v3<double>(0, 0, 0);
+                acc = add(acc, scl(add(sub(bi, b0), sub(ld(x, q[i]), x0)), (double)w[i]));
+            }
+            const double W = (((double)w[0] + (double)w[1]) + (double)w[2]) + (double)w[3];
+            acc = add(acc, scl(add(b0, x0), W));
+            const double nr = m.bend_n[e];
+            const double len = std::sqrt(dot3(acc, acc));
+            D pr = v3<double>(0, 0, 0);
+            if (len > 1e-10 && nr > 1e-10) pr = scl(acc, nr / len);
+            st(bp.data(), e, v3((R)pr.x, (R)pr.y, (R)pr.z));
+            const D dq = sub(pr, acc);
+            st(bd.data(), e, v3((R)dq.x, (R)dq.y, (R)dq.z));
         }
     }
 
@@ -324,22 +366,28 @@ struct Solver {
         }
     }
 
-    // PD vertex update on displacements from the frame-start positions
-    // `base` (null = zero): x and xkm1 are displacements, dy = y - base.
+    // PD vertex update on displacements x (double) from the frame-start
+    // positions base (null = 0), dy = y - base. The corner and hinge terms
+    // are summed in R; then, in double,
     // r = sum h^2 wA D^T (R - F) + sum h^2 k w (p - e) + m (dy - x),
-    // x_J = x + gamma r / diag, Chebyshev blend with x_{k-1}, pinned
-    // vertices at target - base; per-vertex |r|^2 and |b|^2 with
+    // x_J = x + gamma r / diag, the Chebyshev blend with x_{k-1}; pinned
+    // vertices at target - base. Per-vertex |r|^2 and |b|^2 with
     // b = sum h^2 wA D^T R + sum h^2 k w p + m y.
-    void vertex_pd(const R* base, const R* dy, const R* x, const R* xkm1, R* out, R omega,
-                   bool blend, std::vector<double>& rr, std::vector<double>& bb,
-                   R* r_out = nullptr, R gamma = (R)1)
+    void vertex_pd(const R* base, const R* dy, const double* x, const double* xkm1, double* out,
+                   double omega, bool blend, std::vector<double>& rr, std::vector<double>& bb,
+                   double* r_out = nullptr, double gamma = 1.0)
     {
+        using D = V3<double>;
 #pragma omp parallel for schedule(static) num_threads(nthreads(&p))
         for (int v = 0; v < N; ++v) {
             rr[v] = 0; bb[v] = 0;
-            if (r_out) st(r_out, v, v3<R>(0, 0, 0));
-            const V3<R> bv = base ? ld(base, v) : v3<R>(0, 0, 0);
-            if (pin_map[v] >= 0) { st(out, v, sub(target(pin_map[v]), bv)); continue; }
+            if (r_out) st(r_out, v, v3<double>(0, 0, 0));
+            const D bv = base ? ldd(base, v) : v3<double>(0, 0, 0);
+            if (pin_map[v] >= 0) {
+                const V3<R> t = target(pin_map[v]);
+                st(out, v, sub(v3<double>(t.x, t.y, t.z), bv));
+                continue;
+            }
             V3<R> ar = v3<R>(0, 0, 0), ab = v3<R>(0, 0, 0);
             if (stretch)
                 for (int j = vt_ptr[v]; j < vt_ptr[v + 1]; ++j) {
@@ -354,21 +402,23 @@ struct Solver {
                     ar = add(ar, v3(kw * dq.x, kw * dq.y, kw * dq.z));
                     ab = add(ab, v3(kw * pq.x, kw * pq.y, kw * pq.z));
                 }
-            const R mv = m.mass[v];
-            const V3<R> xv = ld(x, v), yv = ld(dy, v);
-            const V3<R> r = v3(ar.x + mv * (yv.x - xv.x), ar.y + mv * (yv.y - xv.y), ar.z + mv * (yv.z - xv.z));
-            const V3<R> ya = add(bv, yv);
-            const V3<R> b = v3(ab.x + mv * ya.x, ab.y + mv * ya.y, ab.z + mv * ya.z);
-            const R d = m.diag[v];
-            const R inv = (d > thr<R>()) ? gamma / d : (R)0;
-            V3<R> xn = v3(xv.x + r.x * inv, xv.y + r.y * inv, xv.z + r.z * inv);
+            const double mv = m.mass[v];
+            const D xv = ld(x, v), yv = ldd(dy, v);
+            const D r = v3((double)ar.x + mv * (yv.x - xv.x), (double)ar.y + mv * (yv.y - xv.y),
+                           (double)ar.z + mv * (yv.z - xv.z));
+            const V3<R> ya = v3((R)(bv.x + yv.x), (R)(bv.y + yv.y), (R)(bv.z + yv.z));
+            const R mr = m.mass[v];
+            const V3<R> b = v3(ab.x + mr * ya.x, ab.y + mr * ya.y, ab.z + mr * ya.z);
+            const double d = m.diag[v];
+            const double inv = (d > 1e-10) ? gamma / d : 0.0;
+            D xn = v3(xv.x + r.x * inv, xv.y + r.y * inv, xv.z + r.z * inv);
             if (blend) {
-                const V3<R> xp = ld(xkm1, v);
+                const D xp = ld(xkm1, v);
                 xn = v3(omega * (xn.x - xp.x) + xp.x, omega * (xn.y - xp.y) + xp.y, omega * (xn.z - xp.z) + xp.z);
             }
             st(out, v, xn);
             if (r_out) st(r_out, v, r);
-            rr[v] = (double)r.x * r.x + (double)r.y * r.y + (double)r.z * r.z;
+            rr[v] = r.x * r.x + r.y * r.y + r.z * r.z;
             bb[v] = (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
         }
     }
@@ -448,56 +498,71 @@ int step(const or_mesh* mp, const or_params* pp, R* pos, R* vel, const R* init, 
         for (int k = 0; k < p.iterations; ++k) {
             const R* x = buf[k % 3].data();
             R* out = buf[(k + 1) % 3].data();
-            S.triangles(nullptr, x, false);
-            S.hinges(nullptr, x, false);
+            S.triangles_ref(x);
+            S.hinges_ref(x);
             S.vertex_ref(y.data(), x, out);
             if (trace) std::memcpy(trace + (size_t)k * N * 3, out, sizeof(R) * N * 3);
             ++iters;
         }
     } else if (p.mode == OR_PD) {
-        // iterate on displacements d_k = x_k - x_n; buf holds d_k, dy = y - x_n
+        // iterate on displacements d_k = x_k - x_n, kept in double; dy = y - x_n
         std::vector<R> dy((size_t)N * 3);
         for (size_t i = 0; i < (size_t)N; ++i)
             for (int k = 0; k < 3; ++k) dy[3 * i + k] = dt * vel[3 * i + k] + h2 * acc[k];
-        for (size_t i = 0; i < (size_t)N * 3; ++i) buf[0][i] = init ? init[i] - xprev[i] : (R)0;
-        for (int i = 0; i < m.P; ++i)
-            st(buf[0].data(), m.pin_idx[i], sub(S.target(i), ld(xprev.data(), m.pin_idx[i])));
+        std::vector<double> db[3];
+        for (auto& b : db) b.assign((size_t)N * 3, 0.0);
+        if (init)
+            for (size_t i = 0; i < (size_t)N * 3; ++i) db[0][i] = (double)init[i] - (double)xprev[i];
+        for (int i = 0; i < m.P; ++i) {
+            const int v = m.pin_idx[i];
+            for (int k = 0; k < 3; ++k)
+                db[0][3 * (size_t)v + k] = (double)m.pin_target[3 * i + k] - (double)xprev[3 * (size_t)v + k];
+        }
         std::vector<double> rr((size_t)N), bb((size_t)N);
-        std::vector<R> absx(trace ? (size_t)N * 3 : 0);
-        R omega = 1;
+        float omf = 1.0f;         // the weights are computed in FP32, as the solver does
         const double tol2 = (double)p.tolerance * (double)p.tolerance;
         const int every = std::max(1, p.check_every);
+        int ck = 0;               // iterations since the last Chebyshev (re)start
+        double last_r2 = -1.0;    // residual at the previous check
         for (int k = 0; k < p.iterations; ++k) {
-            omega = cheb_omega<R>(p, k, omega);
-            const bool blend = p.use_chebyshev && k > 0 && omega != (R)1;
-            const R* x = buf[k % 3].data();
-            S.triangles(xprev.data(), x, true);
-            S.hinges(xprev.data(), x, true);
-            S.vertex_pd(xprev.data(), dy.data(), x, buf[(k + 2) % 3].data(), buf[(k + 1) % 3].data(),
-                        omega, blend, rr, bb, nullptr, p.relax > 0 ? (R)p.relax : (R)1);
+            omf = cheb_omega<float>(p, ck, omf);
+            const double omega = omf;
+            const bool blend = p.use_chebyshev && ck > 0 && omf != 1.0f;
+            const double* x = db[k % 3].data();
+            S.triangles_pd(xprev.data(), x);
+            S.hinges_pd(xprev.data(), x);
+            S.vertex_pd(xprev.data(), dy.data(), x, db[(k + 2) % 3].data(), db[(k + 1) % 3].data(),
+                        omega, blend, rr, bb, nullptr, p.relax > 0 ? (double)p.relax : 1.0);
+            ++ck;
             double r2 = 0, b2 = 0;
             for (int v = 0; v < N; ++v) { r2 += rr[v]; b2 += bb[v]; }
             if (resid) { resid[2 * k] = r2; resid[2 * k + 1] = b2; }
             if (trace) {
-                const R* d = buf[(k + 1) % 3].data();
+                const double* d = db[(k + 1) % 3].data();
                 R* o = trace + (size_t)k * N * 3;
-                for (size_t i = 0; i < (size_t)N * 3; ++i) o[i] = xprev[i] + d[i];
+                for (size_t i = 0; i < (size_t)N * 3; ++i) o[i] = (R)((double)xprev[i] + d[i]);
             }
             ++iters;
-            if (p.tolerance > 0 && ((k + 1) % every == 0) && r2 <= tol2 * b2) break;
+            if (p.tolerance > 0 && ((k + 1) % every == 0)) {
+                if (r2 <= tol2 * b2) break;
+                // Chebyshev restart when the residual grew since the last test
+                if (p.use_chebyshev && last_r2 >= 0.0 && r2 > last_r2) ck = 0;
+                last_r2 = r2;
+            }
         }
         // x_{n+1} = x_n + d, v = d (1 - damping)/h, pins at target with v = 0
-        const R* d = buf[iters % 3].data();
-        const R scale = ((R)1 - (R)p.damping) / dt;
+        const double* d = db[iters % 3].data();
+        const double scale = (1.0 - (double)p.damping) / (double)dt;
         for (int v = 0; v < N; ++v) {
             if (S.pin_map[v] >= 0) {
                 st(pos, v, S.target(S.pin_map[v]));
                 st(vel, v, v3<R>(0, 0, 0));
                 continue;
             }
-            const V3<R> dv = ld(d, v);
-            st(pos, v, add(ld(xprev.data(), v), dv));
-            st(vel, v, scl(dv, scale));
+            for (int k = 0; k < 3; ++k) {
+                pos[3 * (size_t)v + k] = (R)((double)xprev[3 * (size_t)v + k] + d[3 * (size_t)v + k]);
+                vel[3 * (size_t)v + k] = (R)(d[3 * (size_t)v + k] * scale);
+            }
         }
         if (iters_out) *iters_out = iters;
         return 0;
@@ -568,7 +633,7 @@ int step(const or_mesh* mp, const or_params* pp, R* pos, R* vel, const R* init, 
                         }
                 }
             if (ebend) {
-                S.hinges(nullptr, x.data(), false);
+                S.hinges_ref(x.data());
                 for (int e = 0; e < S.E; ++e)
                     for (int a = 0; a < 4; ++a) {
                         const R kw = h2 * (R)m.bend_k[e] * (R)m.bend_w[4 * e + a];
@@ -766,11 +831,32 @@ extern "C" void or_residual_f64(const or_mesh* m, float dt, const double* x, con
     or_params_default(&p);
     p.dt = dt;
     Solver<double> S(*m, p);
-    S.triangles(nullptr, x, true);
-    S.hinges(nullptr, x, true);
+    S.triangles_pd(nullptr, x);
+    S.hinges_pd(nullptr, x);
     std::vector<double> rr((size_t)m->N), bb((size_t)m->N);
     std::vector<double> out((size_t)m->N * 3);
     S.vertex_pd(nullptr, y, x, x, out.data(), 1.0, false, rr, bb, r_out);
+    double sr = 0, sb = 0;
+    for (int v = 0; v < m->N; ++v) { sr += rr[v]; sb += bb[v]; }
+    *r2 = sr;
+    *b2 = sb;
+}
+
+// FP32 residual of base + disp in the displacement form the PD iteration
+// uses (dy = y - base); r_out [N*3] in float. For precision studies.
+extern "C" void or_residual_disp_f32(const or_mesh* m, float dt, const float* base,
+                                     const double* disp, const float* dy, double* r_out,
+                                     double* r2, double* b2)
+{
+    or_params p;
+    or_params_default(&p);
+    p.dt = dt;
+    Solver<float> S(*m, p);
+    S.triangles_pd(base, disp);
+    S.hinges_pd(base, disp);
+    std::vector<double> rr((size_t)m->N), bb((size_t)m->N);
+    std::vector<double> out((size_t)m->N * 3);
+    S.vertex_pd(base, dy, disp, disp, out.data(), 1.0, false, rr, bb, r_out);
     double sr = 0, sb = 0;
     for (int v = 0; v < m->N; ++v) { sr += rr[v]; sb += bb[v]; }
     *r2 = sr;

@@ -66,6 +66,8 @@ typedef struct or_params {
     int   exact_pins;      /* OR_EXACT: eliminate pins correctly (reference: 0)   */
     int   threads;         /* OpenMP threads for OR_PD / OR_REF_CUDA (<= 0: 1)    */
     float relax;           /* OR_PD: under-relaxation gamma of the Jacobi step    */
+    /* OR_PD with a tolerance restarts the Chebyshev sequence whenever the
+     * residual at a test exceeds the one at the previous test. */
 } or_params;
 
 void or_params_default(or_params* p);
@@ -104,6 +106,12 @@ void or_jacobi_diag_f64(const or_mesh* m, float dt, double* diag);
  * r_out is non-NULL, r [N*3]. y is the inertial prediction. */
 void or_residual_f64(const or_mesh* m, float dt, const double* x, const double* y,
                      double* r_out, double* r2, double* b2);
+
+/* Residual of base + disp in the form the PD iteration evaluates it with
+ * FP32 storage (FP32 base, dy = y - base and element outputs; double
+ * displacement and element arithmetic); r_out [N*3]. */
+void or_residual_disp_f32(const or_mesh* m, float dt, const float* base, const double* disp,
+                          const float* dy, double* r_out, double* r2, double* b2);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,9 @@ def lib():
         L.or_jacobi_diag_f64.restype = None
         L.or_residual_f64.argtypes = [C.POINTER(OrMesh), C.c_float, _dp, _dp, _dp, _dp, _dp]
         L.or_residual_f64.restype = None
+        L.or_residual_disp_f32.argtypes = [C.POINTER(OrMesh), C.c_float, _fp, _dp, _fp, _dp, _dp,
+                                           _dp]
+        L.or_residual_disp_f32.restype = None
         _lib = L
     return _lib
 
@@ -211,3 +214,16 @@ def prediction(pos, vel, dt, gravity, ext=(0.0, 0.0, 0.0)):
     vel = np.asarray(vel, np.float64).reshape(-1, 3)
     a = np.array([ext[0], gravity + ext[1], ext[2]], np.float64)
     return pos + dt * vel + dt * dt * a
+
+
+def residual_disp_f32(mesh: Mesh, dt: float, base, disp, dy):
+    """FP32 (r, ||r||^2, ||b||^2) of base + disp in the iteration's form."""
+    base = np.ascontiguousarray(base, np.float32).reshape(-1, 3)
+    disp = np.ascontiguousarray(disp, np.float64).reshape(-1, 3)
+    dy = np.ascontiguousarray(dy, np.float32).reshape(-1, 3)
+    r = np.zeros(base.shape, np.float64)
+    r2, b2 = C.c_double(0), C.c_double(0)
+    m = mesh.struct()
+    lib().or_residual_disp_f32(C.byref(m), dt, _f(base), _d(disp), _f(dy), _d(r), C.byref(r2),
+                               C.byref(b2))
+    return r, r2.value, b2.value
