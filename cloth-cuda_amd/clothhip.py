@@ -40,7 +40,7 @@ class Config(C.Structure):
                 ("rho", C.c_float), ("gravity", C.c_float), ("dt", C.c_float),
                 ("damping", C.c_float), ("tolerance", C.c_float), ("check_every", C.c_int),
                 ("cheb_delay", C.c_int), ("ext_accel", C.c_float * 3),
-                ("kernel_variant", C.c_int), ("use_graph", C.c_int)]
+                ("kernel_variant", C.c_int), ("use_graph", C.c_int), ("tile_verts", C.c_int)]
 
 
 class MeshDesc(C.Structure):

@@ -39,7 +39,7 @@ struct FusedPlan {
     int N = 0;
     int num_tiles = 0;
     int tile_verts = 0;
-    int lmax = 0, tmax = 0, hmax = 0, vmax = 0;  // per-tile maxima (LDS sizing)
+    int lmax = 0, tmax = 0, hmax = 0, gmax = 0, vmax = 0;  // per-tile maxima (LDS sizing)
     double tri_factor = 0.0, bend_factor = 0.0, bytes_per_iter = 0.0;
     int xcd_remap = 1;
 
@@ -66,6 +66,9 @@ int  fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
 int  fused_pack_constants(FusedPlan& p, const clh_step_args& a, float h2, hipStream_t stream);
 int  fused_max_blocks(const FusedPlan& p);
 size_t fused_lds_bytes(const FusedPlan& p, int mode, bool check);
+// Lets the fused kernels use up to `bytes` of dynamic LDS per workgroup;
+// lowers `bytes` to 64 KB when the runtime does not allow more.
+int  fused_enable_lds(int& bytes);
 int  fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
                             hipStream_t stream);
 void fused_plan_info(const FusedPlan& p, clh_plan_info* out);

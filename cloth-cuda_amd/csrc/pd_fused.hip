@@ -14,7 +14,8 @@ namespace clh {
 namespace {
 
 constexpr int kFB = 256;  // threads per fused workgroup
-constexpr int kTileInfo = 8;  // vbeg, nown, halo_beg, nhalo, tri_beg, ntri, hin_beg, nhin
+constexpr int kTileInfo = 10;  // vbeg, nown, halo_beg, nhalo, tri_beg, ntri, hin_beg, nhin,
+                                // gather_beg, gather_end
 
 // ---------------------------------------------------------------------------
 // host: plan
@@ -179,14 +180,24 @@ __device__ void fused_block_sum2(double r, double b, double* out, int slot)
 }
 
 // One local-global iteration of one tile. kMode selects the reference
-// update (absolute positions, rhs / diag) or PD (displacements, Jacobi +
-// Chebyshev); kCheck adds the right-hand-side norm for the FP64 residual.
+// update (absolute FP32 positions, rhs / diag) or PD (FP64 displacements,
+// Jacobi + Chebyshev); kCheck adds the right-hand-side norm for the FP64
+// residual.
+//
+// Every load that does not depend on another is issued before the first
+// barrier: the own vertices' positions and update inputs, the first element
+// records (into registers), the tile's gather list (into LDS) and the halo
+// ids, whose positions follow. A tile costs three dependent trips to memory
+// (descriptor, bulk data, halo positions) instead of one per phase.
 template <int kMode, bool kCheck>
 __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
 {
     constexpr bool PD = (kMode == CLH_MODE_PD);
+    constexpr int kPT = 2;  // triangle records prefetched per thread
+    constexpr int kPH = 3;  // hinge records prefetched per thread
     extern __shared__ float smem[];
     const FusedIterArgs& a = A.it;
+    const int tid = threadIdx.x;
 
     int tile = blockIdx.x;
     if (A.xcd_remap) {
@@ -198,6 +209,7 @@ __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
     const int* ti = A.tile_info + (size_t)tile * kTileInfo;
     const int vbeg = ti[0], nown = ti[1], hbeg = ti[2], nhalo = ti[3];
     const int tbeg = ti[4], ntri = ti[5], ebeg = ti[6], nhin = ti[7];
+    const int gbeg = ti[8], gend = ti[9];
 
     double* sd = reinterpret_cast<double*>(smem);      // disp (double) [lmax*3] (PD)
     float* sb = smem + (PD ? A.lmax * 6 : 0);          // base     [lmax*3] (PD)
@@ -207,153 +219,201 @@ __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
     float* skw = st2 + ((PD && kCheck) ? A.tmax * 9 : 0);  // hinge k h^2 w_i [hmax*4]
     float* sq = skw + A.hmax * 4;                      // hinge p-e (PD) or p [hmax*3]
     float* sp = sq + A.hmax * 3;                       // hinge p [hmax*3] (PD, check)
+    unsigned short* sg =
+        reinterpret_cast<unsigned short*>(sp + ((PD && kCheck) ? A.hmax * 3 : 0));  // gather list
 
-    // 1. positions of own and halo vertices
     const float* xf = static_cast<const float*>(a.x);
     const double* xd = static_cast<const double*>(a.x);
-    for (int i = threadIdx.x; i < nown; i += kFB) {
-        if (PD) {
-            sts3d(sd, i, ld3d(xd, (size_t)vbeg + i));
-            sts3(sb, i, ld3(a.base, (size_t)vbeg + i));
-        } else {
-            sts3(sx, i, ld3(xf, (size_t)vbeg + i));
-        }
-    }
-    for (int i = threadIdx.x; i < nhalo; i += kFB) {
-        const int g = A.halo[hbeg + i];
-        if (PD) {
-            sts3d(sd, nown + i, ld3d(xd, g));
-            sts3(sb, nown + i, ld3(a.base, g));
-        } else {
-            sts3(sx, nown + i, ld3(xf, g));
-        }
-    }
-    __syncthreads();
 
-    // 2a. triangles -> corner forces (src/pd_solver.cu:48-298)
-    if (a.use_stretch) {
-        for (int j = threadIdx.x; j < ntri; j += kFB) {
-            const uint2 id = A.tri_idx[tbeg + j];
-            const int l0 = id.x & 0xFFFF, l1 = id.x >> 16, l2 = id.y & 0xFFFF;
-            const float4 G = A.tri_c0[tbeg + j];
-            const float wA = A.tri_c1[tbeg + j];
-            f3 c[3];
-            float* o = st + (size_t)j * 9;
-            if (PD) {
-                f3 cr[3];
-                pd_triangle(lds3(sb, l0), lds3(sb, l1), lds3(sb, l2), lds3d(sd, l0), lds3d(sd, l1),
-                            lds3d(sd, l2), G, wA, cr, c, kCheck);
-                if (kCheck) {
-                    float* q = st2 + (size_t)j * 9;
-                    sts3(q, 0, cr[0]);
-                    sts3(q, 1, cr[1]);
-                    sts3(q, 2, cr[2]);
-                }
-            } else {
-                const f3 x0 = lds3(sx, l0);
-                f3 F0, F1, R0, R1;
-                deformation_nc(lds3(sx, l1) - x0, lds3(sx, l2) - x0, G.x, G.y, G.z, G.w, F0, F1);
-                ref_cuda_rotation(F0, F1, R0, R1);
-                corner_forces_nc(G.x, G.y, G.z, G.w, wA, R0, R1, c);
-            }
-            sts3(o, 0, c[0]);
-            sts3(o, 1, c[1]);
-            sts3(o, 2, c[2]);
+    // ---- issue the independent loads ------------------------------------------
+    // own vertex (one per thread: tiles have at most kFB vertices)
+    const bool own = tid < nown;
+    const size_t g = (size_t)vbeg + (own ? tid : 0);
+    d3 ox_d = d3{0.0, 0.0, 0.0}, oxp = d3{0.0, 0.0, 0.0};
+    f3 ox_f = mk(0.0f, 0.0f, 0.0f), ob = mk(0.0f, 0.0f, 0.0f), oy = mk(0.0f, 0.0f, 0.0f);
+    int opin = -1, oj0 = 0, oj1 = 0, ont = 0;
+    float om = 0.0f, odg = 0.0f;
+    if (own) {
+        if (PD) {
+            ox_d = ld3d(xd, g);
+            ob = ld3(a.base, g);
+            if (a.blend) oxp = ld3d(static_cast<const double*>(a.xkm1), g);
+        } else {
+            ox_f = ld3(xf, g);
         }
+        opin = a.pin_map[g];
+        om = a.mass[g];
+        odg = a.diag[g];
+        oy = ld3(a.y, g);
+        oj0 = A.gl_ptr[g] - gbeg;
+        oj1 = A.gl_ptr[g + 1] - gbeg;
+        ont = A.gl_ntri[g];
     }
-    // 2b. hinges -> bend vectors (src/pd_solver.cu:304-358)
-    if (a.use_bend) {
-        for (int j = threadIdx.x; j < nhin; j += kFB) {
-            const uint2 id = A.hin_idx[ebeg + j];
-            const int q0 = id.x & 0xFFFF, q1 = id.x >> 16, q2 = id.y & 0xFFFF, q3 = id.y >> 16;
-            const float4 w = A.hin_w[ebeg + j];
-            const float2 c = A.hin_c[ebeg + j];  // (n_rest, k h^2)
-            f3 p, pe;
-            if (PD) {
-                const int qs[4] = {q0, q1, q2, q3};
-                f3 b[4];
-                d3 xs[4];
+    // element records
+    uint2 rt[kPT];
+    float4 rG[kPT];
+    float rwA[kPT];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    b[i] = lds3(sb, qs[i]);
-                    xs[i] = lds3d(sd, qs[i]);
-                }
-                pd_hinge(b, xs, w, c.x, p, pe);
-            } else {
-                f3 acc = mk(0.0f, 0.0f, 0.0f);
-                acc = madd_nc(acc, w.x, lds3(sx, q0));
-                acc = madd_nc(acc, w.y, lds3(sx, q1));
-                acc = madd_nc(acc, w.z, lds3(sx, q2));
-                acc = madd_nc(acc, w.w, lds3(sx, q3));
-                p = hinge_projection_nc(acc, c.x);
-            }
-            skw[j * 4 + 0] = c.y * w.x;
-            skw[j * 4 + 1] = c.y * w.y;
-            skw[j * 4 + 2] = c.y * w.z;
-            skw[j * 4 + 3] = c.y * w.w;
-            sts3(sq, j, PD ? pe : p);
-            if (PD && kCheck) sts3(sp, j, p);
+    for (int r = 0; r < kPT; ++r) {
+        const int j = tid + r * kFB;
+        if (a.use_stretch && j < ntri) {
+            rt[r] = A.tri_idx[tbeg + j];
+            rG[r] = A.tri_c0[tbeg + j];
+            rwA[r] = A.tri_c1[tbeg + j];
+        }
+    }
+    uint2 rh[kPH];
+    float4 rw[kPH];
+    float2 rc[kPH];
+#pragma unroll
+    for (int r = 0; r < kPH; ++r) {
+        const int j = tid + r * kFB;
+        if (a.use_bend && j < nhin) {
+            rh[r] = A.hin_idx[ebeg + j];
+            rw[r] = A.hin_w[ebeg + j];
+            rc[r] = A.hin_c[ebeg + j];
+        }
+    }
+    // the tile's gather list
+    for (int j = tid; j < gend - gbeg; j += kFB) sg[j] = A.gl_idx[gbeg + j];
+    // halo positions (ids first)
+    for (int i = tid; i < nhalo; i += kFB) {
+        const int hg = A.halo[hbeg + i];
+        if (PD) {
+            sts3d(sd, nown + i, ld3d(xd, hg));
+            sts3(sb, nown + i, ld3(a.base, hg));
+        } else {
+            sts3(sx, nown + i, ld3(xf, hg));
+        }
+    }
+    if (own) {
+        if (PD) {
+            sts3d(sd, tid, ox_d);
+            sts3(sb, tid, ob);
+        } else {
+            sts3(sx, tid, ox_f);
         }
     }
     __syncthreads();
 
-    // 3. own vertices: gather, update, pins, residual
-    double rr = 0.0, bb = 0.0;
-    for (int i = threadIdx.x; i < nown; i += kFB) {
-        const size_t g = (size_t)vbeg + i;
-        const int slot = a.pin_map[g];
-        if (slot >= 0) {
-            const f3 tgt = ld3(a.pin_target, slot);
-            if (PD) st3d(static_cast<double*>(a.out), g, to_d(tgt) - to_d(lds3(sb, i)));
-            else    st3(static_cast<float*>(a.out), g, tgt);
-            continue;
-        }
-        const int j0 = A.gl_ptr[g], j1 = A.gl_ptr[g + 1];
-        const int jt = j0 + A.gl_ntri[g];
-        f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
-        if (a.use_stretch)
-            for (int j = j0; j < jt; ++j) {
-                const int s = A.gl_idx[j];
-                ar = ar + lds3(st, s);
-                if (PD && kCheck) ab = ab + lds3(st2, s);
+    // ---- triangles -> corner forces (src/pd_solver.cu:48-298) -----------------
+    auto triangle = [&](int j, uint2 id, float4 G, float wA) {
+        const int l0 = id.x & 0xFFFF, l1 = id.x >> 16, l2 = id.y & 0xFFFF;
+        f3 c[3];
+        float* o = st + (size_t)j * 9;
+        if (PD) {
+            f3 cr[3];
+            pd_triangle(lds3(sb, l0), lds3(sb, l1), lds3(sb, l2), lds3d(sd, l0), lds3d(sd, l1),
+                        lds3d(sd, l2), G, wA, cr, c, kCheck);
+            if (kCheck) {
+                float* q = st2 + (size_t)j * 9;
+                sts3(q, 0, cr[0]);
+                sts3(q, 1, cr[1]);
+                sts3(q, 2, cr[2]);
             }
-        if (a.use_bend)
-            for (int j = jt; j < j1; ++j) {
-                const int s = A.gl_idx[j];
-                const float kw = skw[s];
-                const f3 d = lds3(sq, s >> 2);
-                if (PD) ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
-                else    ar = madd_nc(ar, kw, d);
-                if (PD && kCheck) {
-                    const f3 p = lds3(sp, s >> 2);
-                    ab = ab + mk(kw * p.x, kw * p.y, kw * p.z);
+        } else {
+            const f3 x0 = lds3(sx, l0);
+            f3 F0, F1, R0, R1;
+            deformation_nc(lds3(sx, l1) - x0, lds3(sx, l2) - x0, G.x, G.y, G.z, G.w, F0, F1);
+            ref_cuda_rotation(F0, F1, R0, R1);
+            corner_forces_nc(G.x, G.y, G.z, G.w, wA, R0, R1, c);
+        }
+        sts3(o, 0, c[0]);
+        sts3(o, 1, c[1]);
+        sts3(o, 2, c[2]);
+    };
+    if (a.use_stretch) {
+#pragma unroll
+        for (int r = 0; r < kPT; ++r)
+            if (tid + r * kFB < ntri) triangle(tid + r * kFB, rt[r], rG[r], rwA[r]);
+        for (int j = tid + kPT * kFB; j < ntri; j += kFB)
+            triangle(j, A.tri_idx[tbeg + j], A.tri_c0[tbeg + j], A.tri_c1[tbeg + j]);
+    }
+    // ---- hinges -> bend vectors (src/pd_solver.cu:304-358) --------------------
+    auto hinge = [&](int j, uint2 id, float4 w, float2 c) {  // c = (n_rest, k h^2)
+        const int q0 = id.x & 0xFFFF, q1 = id.x >> 16, q2 = id.y & 0xFFFF, q3 = id.y >> 16;
+        f3 p, pe;
+        if (PD) {
+            const int qs[4] = {q0, q1, q2, q3};
+            f3 b[4];
+            d3 xs[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                b[i] = lds3(sb, qs[i]);
+                xs[i] = lds3d(sd, qs[i]);
+            }
+            pd_hinge(b, xs, w, c.x, p, pe);
+        } else {
+            f3 acc = mk(0.0f, 0.0f, 0.0f);
+            acc = madd_nc(acc, w.x, lds3(sx, q0));
+            acc = madd_nc(acc, w.y, lds3(sx, q1));
+            acc = madd_nc(acc, w.z, lds3(sx, q2));
+            acc = madd_nc(acc, w.w, lds3(sx, q3));
+            p = hinge_projection_nc(acc, c.x);
+        }
+        skw[j * 4 + 0] = c.y * w.x;
+        skw[j * 4 + 1] = c.y * w.y;
+        skw[j * 4 + 2] = c.y * w.z;
+        skw[j * 4 + 3] = c.y * w.w;
+        sts3(sq, j, PD ? pe : p);
+        if (PD && kCheck) sts3(sp, j, p);
+    };
+    if (a.use_bend) {
+#pragma unroll
+        for (int r = 0; r < kPH; ++r)
+            if (tid + r * kFB < nhin) hinge(tid + r * kFB, rh[r], rw[r], rc[r]);
+        for (int j = tid + kPH * kFB; j < nhin; j += kFB)
+            hinge(j, A.hin_idx[ebeg + j], A.hin_w[ebeg + j], A.hin_c[ebeg + j]);
+    }
+    __syncthreads();
+
+    // ---- own vertices: gather, update, pins, residual ---------------------------
+    double rr = 0.0, bb = 0.0;
+    if (own) {
+        if (opin >= 0) {
+            const f3 tgt = ld3(a.pin_target, opin);
+            if (PD) st3d(static_cast<double*>(a.out), g, to_d(tgt) - to_d(ob));
+            else    st3(static_cast<float*>(a.out), g, tgt);
+        } else {
+            const int jt = oj0 + ont;
+            f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
+            if (a.use_stretch)
+                for (int j = oj0; j < jt; ++j) {
+                    const int s = sg[j];
+                    ar = ar + lds3(st, s);
+                    if (PD && kCheck) ab = ab + lds3(st2, s);
+                }
+            if (a.use_bend)
+                for (int j = jt; j < oj1; ++j) {
+                    const int s = sg[j];
+                    const float kw = skw[s];
+                    const f3 d = lds3(sq, s >> 2);
+                    if (PD) ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+                    else    ar = madd_nc(ar, kw, d);
+                    if (PD && kCheck) {
+                        const f3 p = lds3(sp, s >> 2);
+                        ab = ab + mk(kw * p.x, kw * p.y, kw * p.z);
+                    }
+                }
+            if (!PD) {
+                const float inv = (odg > 1e-10f) ? (1.0f / odg) : 0.0f;
+                st3(static_cast<float*>(a.out), g, madd_nc(ar, om, oy) * inv);
+            } else {
+                const double md = om;
+                const d3 r = d3{(double)ar.x + md * ((double)oy.x - ox_d.x),
+                                (double)ar.y + md * ((double)oy.y - ox_d.y),
+                                (double)ar.z + md * ((double)oy.z - ox_d.z)};
+                const double inv = (odg > 1e-10f) ? 1.0 / (double)odg : 0.0;
+                d3 xn = ox_d + r * inv;
+                if (a.blend) xn = (xn - oxp) * (double)a.omega + oxp;
+                st3d(static_cast<double*>(a.out), g, xn);
+                if (kCheck) {
+                    const f3 ya = ob + oy;
+                    const f3 b = mk(ab.x + om * ya.x, ab.y + om * ya.y, ab.z + om * ya.z);
+                    rr = r.x * r.x + r.y * r.y + r.z * r.z;
+                    bb = (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
                 }
             }
-        const float m = a.mass[g];
-        const float dg = a.diag[g];
-        const f3 yv = ld3(a.y, g);
-        if (!PD) {
-            const float inv = (dg > 1e-10f) ? (1.0f / dg) : 0.0f;
-            st3(static_cast<float*>(a.out), g, madd_nc(ar, m, yv) * inv);
-            continue;
-        }
-        const d3 xv = lds3d(sd, i);
-        const double md = m;
-        const d3 r = d3{(double)ar.x + md * ((double)yv.x - xv.x),
-                        (double)ar.y + md * ((double)yv.y - xv.y),
-                        (double)ar.z + md * ((double)yv.z - xv.z)};
-        const double inv = (dg > 1e-10f) ? 1.0 / (double)dg : 0.0;
-        d3 xn = xv + r * inv;
-        if (a.blend) {
-            const d3 xp = ld3d(static_cast<const double*>(a.xkm1), g);
-            xn = (xn - xp) * (double)a.omega + xp;
-        }
-        st3d(static_cast<double*>(a.out), g, xn);
-        if (kCheck) {
-            const f3 ya = lds3(sb, i) + yv;
-            const f3 b = mk(ab.x + m * ya.x, ab.y + m * ya.y, ab.z + m * ya.z);
-            rr += r.x * r.x + r.y * r.y + r.z * r.z;
-            bb += (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
         }
     }
     if (kCheck) fused_block_sum2(rr, bb, a.partials, tile);
@@ -468,7 +528,7 @@ size_t lds_bytes(const FusedPlan& p, int mode, bool check)
     const bool pd = (mode == CLH_MODE_PD);
     size_t f = (size_t)p.lmax * 3 * (pd ? 3 : 1) + (size_t)p.tmax * 9 * ((pd && check) ? 2 : 1) +
                (size_t)p.hmax * (4 + 3 + ((pd && check) ? 3 : 0));
-    return f * sizeof(float);
+    return f * sizeof(float) + (size_t)(p.gmax + 1) / 2 * 4;
 }
 
 unsigned blocks_for(int n) { return (unsigned)((n + 255) / 256); }
@@ -480,7 +540,7 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
 {
     if (N <= 0) return fail(CLH_ERR_UNSUPPORTED, "fused plan: empty mesh");
     const int T = (int)(tris.size() / 3), E = (int)(quads.size() / 4);
-    const int V = std::max(32, std::min(tile_verts, 1024));
+    const int V = std::max(32, std::min(tile_verts, kFB));  // one own vertex per thread
 
     std::vector<int> perm = morton_order(N, pos);
     std::vector<int> inv((size_t)N);
@@ -497,7 +557,7 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
     std::vector<unsigned short> gl_idx;
     std::vector<unsigned char> gl_ntri((size_t)N, 0);
     std::vector<int> loc((size_t)N, -1);  // tile-order id -> local id in the current tile
-    int lmax = 0, tmax = 0, hmax = 0;
+    int lmax = 0, tmax = 0, hmax = 0, gmax = 0;
 
     for (int t = 0; t < ntiles; ++t) {
         const int vbeg = t * V, nown = std::min(N, vbeg + V) - vbeg;
@@ -570,6 +630,8 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
         int* ti = &info[(size_t)t * kTileInfo];
         ti[0] = vbeg; ti[1] = nown; ti[2] = hbeg; ti[3] = nl - nown;
         ti[4] = tbeg; ti[5] = (int)tl.size(); ti[6] = ebeg; ti[7] = (int)hl.size();
+        ti[8] = gl_ptr[(size_t)vbeg]; ti[9] = gl_ptr[(size_t)vbeg + nown];
+        gmax = std::max(gmax, ti[9] - ti[8]);
         lmax = std::max(lmax, nl);
         tmax = std::max(tmax, (int)tl.size());
         hmax = std::max(hmax, (int)hl.size());
@@ -583,6 +645,7 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
     p.lmax = lmax;
     p.tmax = std::max(tmax, 1);
     p.hmax = std::max(hmax, 1);
+    p.gmax = std::max(gmax, 2);
     p.vmax = V;
     p.tri_factor = T > 0 ? (double)tri_gid.size() / T : 0.0;
     p.bend_factor = E > 0 ? (double)hin_gid.size() / E : 0.0;
@@ -640,6 +703,8 @@ int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::
         if (vbeg != covered || nown <= 0 || nown > p.tile_verts) return fail(CLH_ERR_INVALID, "plan: own range");
         covered += nown;
         if (nl > p.lmax || ntri > p.tmax || nhin > p.hmax) return fail(CLH_ERR_INVALID, "plan: tile maxima");
+        if (ti[8] != h.gl_ptr[(size_t)vbeg] || ti[9] != h.gl_ptr[(size_t)vbeg + nown] || ti[9] - ti[8] > p.gmax)
+            return fail(CLH_ERR_INVALID, "plan: gather range");
         auto tile_id = [&](int l) { return l < nown ? vbeg + l : h.halo[(size_t)hbeg + l - nown]; };
         for (int l = nown; l < nl; ++l) {
             const int g = tile_id(l);
@@ -695,7 +760,7 @@ int fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
     CLH_TRY(host_plan_build(h, N, tris, quads, pos, tile_verts));
     const FusedPlan& m = h.meta;
     p.N = m.N; p.num_tiles = m.num_tiles; p.tile_verts = m.tile_verts;
-    p.lmax = m.lmax; p.tmax = m.tmax; p.hmax = m.hmax; p.vmax = m.vmax;
+    p.lmax = m.lmax; p.tmax = m.tmax; p.hmax = m.hmax; p.gmax = m.gmax; p.vmax = m.vmax;
     p.tri_factor = m.tri_factor; p.bend_factor = m.bend_factor; p.xcd_remap = m.xcd_remap;
     p.bytes_per_iter = m.bytes_per_iter;
     CLH_TRY(p.perm.upload(h.perm));
@@ -774,6 +839,24 @@ int fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
 }
 
 size_t fused_lds_bytes(const FusedPlan& p, int mode, bool check) { return lds_bytes(p, mode, check); }
+
+int fused_enable_lds(int& bytes)
+{
+    // above 64 KB the kernels must opt in; when the runtime refuses, stay at
+    // 64 KB
+    if (bytes <= 64 * 1024) return CLH_OK;
+    const void* fns[] = {reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false>)};
+    for (const void* f : fns) {
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            bytes = 64 * 1024;
+            return CLH_OK;
+        }
+    }
+    return CLH_OK;
+}
 
 int fused_prologue(const FusedPlan& p, const clh_step_args& a, float* xprev, float* y, void* x0,
                    float* mass, float* diag, int* pin_map, float dt, const float h2a[3],

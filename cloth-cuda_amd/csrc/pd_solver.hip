@@ -481,7 +481,7 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
         int dev = 0, lds_limit = 0;
         CLH_HIP(hipGetDevice(&dev));
         CLH_HIP(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        lds_limit = std::min(lds_limit, 64 * 1024);
+        CLH_TRY(fused_enable_lds(lds_limit));
         int st = CLH_ERR_UNSUPPORTED;
         for (int tv = s->tile_verts; tv >= 32; tv /= 2) {
             st = fused_plan_build(s->fused, N, tris, quads, pos, tv);
@@ -861,6 +861,7 @@ extern "C" int clh_solver_create(const clh_config* cfg, const clh_mesh_desc* mes
     s->N = mesh->num_verts;
     s->T = mesh->num_tris;
     s->E = mesh->num_bend;
+    if (cfg->tile_verts > 0) s->tile_verts = std::max(32, std::min(cfg->tile_verts, 1024));
     int st = hip_status(hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking),
                         "hipStreamCreate");
     if (st == CLH_OK) st = hip_status(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming), "hipEventCreate");

@@ -134,14 +134,13 @@ extern "C" int clh_apply_constraints(float* d_pos, float* d_vel, const int* d_in
     return hip_status(hipGetLastError(), "k_apply_pins launch");
 }
 
-// Grid-stride 16-byte copy; the tail (bytes % 16) is copied bytewise.
+// One 16-byte element per thread; the tail (bytes % 16) is copied bytewise.
 __global__ void __launch_bounds__(256)
 k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16,
          unsigned char* __restrict__ dtail, const unsigned char* __restrict__ stail, int ntail)
 {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
-        dst[i] = src[i];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n16) dst[i] = src[i];
     if (blockIdx.x == 0 && (int)threadIdx.x < ntail) dtail[threadIdx.x] = stail[threadIdx.x];
 }
 
@@ -152,12 +151,9 @@ extern "C" int clh_measure_copy(void* d_dst, const void* d_src, size_t bytes, in
         return fail(CLH_ERR_INVALID, "clh_measure_copy: bad arguments");
     if (((uintptr_t)d_dst | (uintptr_t)d_src) & 15)
         return fail(CLH_ERR_INVALID, "clh_measure_copy: buffers must be 16-byte aligned");
-    int dev = 0, cus = 0;
-    CLH_HIP(hipGetDevice(&dev));
-    CLH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const size_t n16 = bytes / 16;
     const int ntail = (int)(bytes % 16);
-    const int blocks = cus * 8;
+    const unsigned blocks = (unsigned)((n16 + 255) / 256);
     auto launch = [&]() {
         hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, 0, (uint4*)d_dst,
                            (const uint4*)d_src, n16, (unsigned char*)d_dst + n16 * 16,

@@ -110,6 +110,7 @@ typedef struct clh_config {
     float ext_accel[3];     /* uniform external acceleration (wind), m/s^2      */
     int   kernel_variant;   /* CLH_KERNEL_*                                     */
     int   use_graph;        /* capture fixed-length frames in a hipGraph        */
+    int   tile_verts;       /* fused variant: vertices per tile (0 = default)   */
 } clh_config;
 
 /* Fills *cfg with the defaults of PDSolverConfig (include/pd_solver.h:5-14)
