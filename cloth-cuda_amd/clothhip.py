@@ -40,7 +40,8 @@ class Config(C.Structure):
                 ("rho", C.c_float), ("gravity", C.c_float), ("dt", C.c_float),
                 ("damping", C.c_float), ("tolerance", C.c_float), ("check_every", C.c_int),
                 ("cheb_delay", C.c_int), ("ext_accel", C.c_float * 3),
-                ("kernel_variant", C.c_int), ("use_graph", C.c_int), ("tile_verts", C.c_int)]
+                ("kernel_variant", C.c_int), ("use_graph", C.c_int), ("tile_verts", C.c_int),
+                ("block_threads", C.c_int)]
 
 
 class MeshDesc(C.Structure):
@@ -66,7 +67,8 @@ class StepStats(C.Structure):
 class PlanInfo(C.Structure):
     _fields_ = [("variant", C.c_int), ("num_tiles", C.c_int), ("tile_verts", C.c_int),
                 ("halo_tri_factor", C.c_double), ("halo_bend_factor", C.c_double),
-                ("bytes_per_iter", C.c_double), ("lds_bytes", C.c_double)]
+                ("bytes_per_iter", C.c_double), ("lds_bytes", C.c_double),
+                ("block_threads", C.c_int)]
 
 
 class SceneSpec(C.Structure):

@@ -8,10 +8,11 @@
 // (16-bit) vertex indices into its LDS position table (own vertices first,
 // then the halo). One workgroup per tile per iteration:
 //   1. stage own + halo positions in LDS,
-//   2. triangles -> corner forces in LDS, hinges -> bend vectors in LDS,
-//   3. each own vertex gathers its corners and hinges (ascending element
-//      id, the order of the unfused pipeline), then Jacobi / Chebyshev /
-//      pins and FP64 residual partials; x_{k+1} goes to HBM.
+//   2. triangles -> corner forces in LDS; each own vertex sums its corners,
+//   3. hinges -> bend vectors in the same LDS region; each own vertex sums
+//      its hinge slots (ascending element id after the triangles, the order
+//      of the unfused pipeline), then Jacobi / Chebyshev / pins and FP64
+//      residual partials; x_{k+1} goes to HBM.
 #include "clh_internal.h"
 
 #include <vector>
@@ -24,9 +25,7 @@ struct FusedIterArgs {
     void*        out;      // iterate k+1
     const float* base;     // x_n (tile order; PD iterates are displacements)
     const float* y;        // y (reference mode) or y - x_n (PD), tile order
-    const float* mass;     // tile order
-    const float* diag;     // tile order
-    const int*   pin_map;  // tile order -> pin slot or -1
+    const int4*  vinfo;    // tile order: (mass, diag as float bits, pin slot or -1, gather meta)
     const float* pin_target;
     double*      partials; // per-tile |r|^2, |b|^2 when non-null
     float omega;
@@ -39,21 +38,22 @@ struct FusedPlan {
     int N = 0;
     int num_tiles = 0;
     int tile_verts = 0;
+    int threads = 256;                                     // workgroup size (256 or 512)
+    int hstride = 0;                                       // halo ids per tile (padded)
     int lmax = 0, tmax = 0, hmax = 0, gmax = 0, vmax = 0;  // per-tile maxima (LDS sizing)
     double tri_factor = 0.0, bend_factor = 0.0, bytes_per_iter = 0.0;
     int xcd_remap = 1;
 
     DevBuf<int>   perm, inv_perm;             // tile order <-> original
-    DevBuf<int>   tile_info;                  // [num_tiles * 8]
-    DevBuf<int>   halo;                       // tile-order ids of halo vertices
+    DevBuf<int>   tile_info;                  // [num_tiles * kTileInfo]
+    DevBuf<int>   halo;                       // [num_tiles * hstride] tile-order halo ids
     DevBuf<int>   tri_gid, hin_gid;           // original element ids
     DevBuf<uint2> tri_idx, hin_idx;           // packed u16 local vertex ids
     DevBuf<float4> tri_c0, hin_w;             // Dm_inv | bend weights
     DevBuf<float> tri_c1;                     // h^2 k A
     DevBuf<float2> hin_c;                     // (n_rest, h^2 k)
-    DevBuf<int>   gl_ptr;                     // [N+1] gather list offsets (tile order)
-    DevBuf<unsigned short> gl_idx;            // local slots: tri*3+c, then hinge*4+i
-    DevBuf<unsigned char>  gl_ntri;           // [N] leading triangle entries
+    DevBuf<unsigned> gl_words;                // gather entries, two u16 per word
+    DevBuf<unsigned> gl_meta;                 // [N] start in tile | n_tri << 16 | n_all << 24
 
     // packed-constant provenance (repack when any pointer changes)
     const void* packed_src[8] = {};
@@ -77,10 +77,9 @@ void fused_plan_release(FusedPlan& p);
 // Frame prologue / epilogue in tile order (gather from and scatter to the
 // caller's original-order arrays).
 int fused_prologue(const FusedPlan& p, const clh_step_args& a, float* xprev, float* y, void* x0,
-                   float* mass, float* diag, int* pin_map, float dt, const float h2a[3],
-                   int disp, hipStream_t stream);
+                   int4* vinfo, float dt, const float h2a[3], int disp, hipStream_t stream);
 int fused_epilogue(const FusedPlan& p, const clh_step_args& a, const void* xf, const float* xprev,
-                   const int* pin_map, float scale, int disp, hipStream_t stream);
+                   const int4* vinfo, float scale, int disp, hipStream_t stream);
 int fused_trace(const FusedPlan& p, const void* it, const float* xprev, float* out, int disp,
                 hipStream_t stream);
 

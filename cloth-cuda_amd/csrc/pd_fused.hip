@@ -13,9 +13,9 @@
 namespace clh {
 namespace {
 
-constexpr int kFB = 256;  // threads per fused workgroup
-constexpr int kTileInfo = 10;  // vbeg, nown, halo_beg, nhalo, tri_beg, ntri, hin_beg, nhin,
-                                // gather_beg, gather_end
+constexpr int kFB = 256;  // largest tile (one own vertex per thread)
+constexpr int kTileInfo = 8;  // nhalo, tri_beg, ntri, hin_beg, nhin, gather word begin,
+                               // gather entries, vbeg
 
 // ---------------------------------------------------------------------------
 // host: plan
@@ -106,10 +106,10 @@ void build_slots(const std::vector<int>& elems, int arity, int N, std::vector<in
 
 struct HostPlan {
     FusedPlan meta;  // sizes only; device buffers unused
-    std::vector<int> perm, inv, info, halo, tri_gid, hin_gid, gl_ptr;
+    std::vector<int> perm, inv, info, halo, tri_gid, hin_gid;
     std::vector<uint2> tri_idx, hin_idx;
     std::vector<unsigned short> gl_idx;
-    std::vector<unsigned char> gl_ntri;
+    std::vector<unsigned> gl_meta;
 };
 
 // ---------------------------------------------------------------------------
@@ -118,18 +118,18 @@ struct HostPlan {
 
 struct FusedKernelArgs {
     FusedIterArgs it;
-    const int*            tile_info;
-    const int*            halo;
-    const uint2*          tri_idx;
-    const float4*         tri_c0;
-    const float*          tri_c1;
-    const uint2*          hin_idx;
-    const float4*         hin_w;
-    const float2*         hin_c;
-    const int*            gl_ptr;
-    const unsigned short* gl_idx;
-    const unsigned char*  gl_ntri;
+    const int*      tile_info;
+    const int*      halo;
+    const uint2*    tri_idx;
+    const float4*   tri_c0;
+    const float*    tri_c1;
+    const uint2*    hin_idx;
+    const float4*   hin_w;
+    const float2*   hin_c;
+    const unsigned* gl_words;
+    int N, tile_verts, hstride;
     int lmax, tmax, hmax;
+    int off_g, off_r;  // LDS byte offsets: gather list, element region
     int num_tiles;
     int xcd_remap;
 };
@@ -144,20 +144,23 @@ __device__ __forceinline__ void sts3(float* s, int i, f3 v)
     s[3 * i + 1] = v.y;
     s[3 * i + 2] = v.z;
 }
-__device__ __forceinline__ d3 lds3d(const double* s, int i)
+// PD position table: three FP64 component arrays of stride n (a vertex's
+// components sit in different banks from its neighbours')
+__device__ __forceinline__ d3 ldsX(const double* s, int n, int i)
 {
-    return d3{s[3 * i], s[3 * i + 1], s[3 * i + 2]};
+    return d3{s[i], s[n + i], s[2 * n + i]};
 }
-__device__ __forceinline__ void sts3d(double* s, int i, d3 v)
+__device__ __forceinline__ void stsX(double* s, int n, int i, d3 v)
 {
-    s[3 * i] = v.x;
-    s[3 * i + 1] = v.y;
-    s[3 * i + 2] = v.z;
+    s[i] = v.x;
+    s[n + i] = v.y;
+    s[2 * n + i] = v.z;
 }
 
+template <int kT>
 __device__ void fused_block_sum2(double r, double b, double* out, int slot)
 {
-    __shared__ double sr[kFB / 64], sb[kFB / 64];
+    __shared__ double sr[kT / 64], sb[kT / 64];
     for (int off = 32; off > 0; off >>= 1) {
         r += __shfl_down(r, off, 64);
         b += __shfl_down(b, off, 64);
@@ -170,7 +173,7 @@ __device__ void fused_block_sum2(double r, double b, double* out, int slot)
     __syncthreads();
     if (threadIdx.x == 0) {
         double tr = 0.0, tb = 0.0;
-        for (int i = 0; i < kFB / 64; ++i) {
+        for (int i = 0; i < kT / 64; ++i) {
             tr += sr[i];
             tb += sb[i];
         }
@@ -184,21 +187,25 @@ __device__ void fused_block_sum2(double r, double b, double* out, int slot)
 // Jacobi + Chebyshev); kCheck adds the right-hand-side norm for the FP64
 // residual.
 //
-// Every load that does not depend on another is issued before the first
-// barrier: the own vertices' positions and update inputs, the first element
-// records (into registers), the tile's gather list (into LDS) and the halo
-// ids, whose positions follow. A tile costs three dependent trips to memory
-// (descriptor, bulk data, halo positions) instead of one per phase.
-template <int kMode, bool kCheck>
-__global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
+// Memory trips: (1) the tile descriptor, the own vertices' data (their range
+// is tile * tile_verts) and the halo ids (fixed stride per tile) are all
+// independent; (2) halo positions, the first element records and the gather
+// list follow the descriptor. Further element records are fetched one pass
+// ahead of their use.
+//
+// LDS: position table (PD: FP64 x_n + d as three component arrays;
+// reference: FP32 triples), the tile's gather list, and one element region
+// that holds the triangle corner forces and, once they are summed, the hinge
+// weights (four component arrays) and vectors.
+template <int kMode, bool kCheck, int kT>
+__global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
 {
     constexpr bool PD = (kMode == CLH_MODE_PD);
-    constexpr int kPT = 2;  // triangle records prefetched per thread
-    constexpr int kPH = 3;  // hinge records prefetched per thread
-    extern __shared__ float smem[];
+    extern __shared__ double smem[];
     const FusedIterArgs& a = A.it;
     const int tid = threadIdx.x;
 
+    const int L = A.lmax;
     int tile = blockIdx.x;
     if (A.xcd_remap) {
         // consecutive tiles (a spatial region) on one XCD: workgroup b runs
@@ -206,217 +213,209 @@ __global__ void __launch_bounds__(kFB) k_fused_iter(FusedKernelArgs A)
         const int per = A.num_tiles >> 3;
         tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     }
-    const int* ti = A.tile_info + (size_t)tile * kTileInfo;
-    const int vbeg = ti[0], nown = ti[1], hbeg = ti[2], nhalo = ti[3];
-    const int tbeg = ti[4], ntri = ti[5], ebeg = ti[6], nhin = ti[7];
-    const int gbeg = ti[8], gend = ti[9];
+    const int vbeg = tile * A.tile_verts;
+    const int nown = min(A.tile_verts, A.N - vbeg);
 
-    double* sd = reinterpret_cast<double*>(smem);      // disp (double) [lmax*3] (PD)
-    float* sb = smem + (PD ? A.lmax * 6 : 0);          // base     [lmax*3] (PD)
-    float* sx = sb + (PD ? A.lmax * 3 : 0);            // x        [lmax*3] (reference)
-    float* st = sx + (PD ? 0 : A.lmax * 3);            // tri out  [tmax*9]
-    float* st2 = st + A.tmax * 9;                      // tri R    [tmax*9] (PD, check)
-    float* skw = st2 + ((PD && kCheck) ? A.tmax * 9 : 0);  // hinge k h^2 w_i [hmax*4]
-    float* sq = skw + A.hmax * 4;                      // hinge p-e (PD) or p [hmax*3]
-    float* sp = sq + A.hmax * 3;                       // hinge p [hmax*3] (PD, check)
-    unsigned short* sg =
-        reinterpret_cast<unsigned short*>(sp + ((PD && kCheck) ? A.hmax * 3 : 0));  // gather list
+    char* lds = reinterpret_cast<char*>(smem);
+    double* sX = smem;                                   // PD positions
+    float* sx = reinterpret_cast<float*>(smem);          // reference positions
+    unsigned* sgw = reinterpret_cast<unsigned*>(lds + A.off_g);
+    const unsigned short* sg = reinterpret_cast<const unsigned short*>(sgw);
+    float* st = reinterpret_cast<float*>(lds + A.off_r);     // tri corner forces [tmax*9]
+    float* st2 = st + A.tmax * 9;                            // tri R part [tmax*9] (PD, check)
+    float* skw = st;                                         // hinge k h^2 w_i [4][hmax]
+    float* sq = skw + A.hmax * 4;                            // hinge p-e (PD) or p [hmax*3]
+    float* sp = sq + A.hmax * 3;                             // hinge p [hmax*3] (PD, check)
 
     const float* xf = static_cast<const float*>(a.x);
     const double* xd = static_cast<const double*>(a.x);
 
-    // ---- issue the independent loads ------------------------------------------
-    // own vertex (one per thread: tiles have at most kFB vertices)
+    // ---- trip 1: descriptor, own vertices, halo ids --------------------------
+    const int* ti = A.tile_info + (size_t)tile * kTileInfo;
     const bool own = tid < nown;
     const size_t g = (size_t)vbeg + (own ? tid : 0);
-    d3 ox_d = d3{0.0, 0.0, 0.0}, oxp = d3{0.0, 0.0, 0.0};
-    f3 ox_f = mk(0.0f, 0.0f, 0.0f), ob = mk(0.0f, 0.0f, 0.0f), oy = mk(0.0f, 0.0f, 0.0f);
-    int opin = -1, oj0 = 0, oj1 = 0, ont = 0;
-    float om = 0.0f, odg = 0.0f;
+    int4 vi = make_int4(0, 0, -1, 0);
+    d3 od = d3{0.0, 0.0, 0.0}, oxp = d3{0.0, 0.0, 0.0};
+    f3 ob = mk(0.0f, 0.0f, 0.0f), oy = mk(0.0f, 0.0f, 0.0f);
     if (own) {
+        vi = a.vinfo[g];
+        oy = ld3(a.y, g);
         if (PD) {
-            ox_d = ld3d(xd, g);
+            od = ld3d(xd, g);
             ob = ld3(a.base, g);
             if (a.blend) oxp = ld3d(static_cast<const double*>(a.xkm1), g);
         } else {
-            ox_f = ld3(xf, g);
-        }
-        opin = a.pin_map[g];
-        om = a.mass[g];
-        odg = a.diag[g];
-        oy = ld3(a.y, g);
-        oj0 = A.gl_ptr[g] - gbeg;
-        oj1 = A.gl_ptr[g + 1] - gbeg;
-        ont = A.gl_ntri[g];
-    }
-    // element records
-    uint2 rt[kPT];
-    float4 rG[kPT];
-    float rwA[kPT];
-#pragma unroll
-    for (int r = 0; r < kPT; ++r) {
-        const int j = tid + r * kFB;
-        if (a.use_stretch && j < ntri) {
-            rt[r] = A.tri_idx[tbeg + j];
-            rG[r] = A.tri_c0[tbeg + j];
-            rwA[r] = A.tri_c1[tbeg + j];
+            ob = ld3(xf, g);
         }
     }
-    uint2 rh[kPH];
-    float4 rw[kPH];
-    float2 rc[kPH];
-#pragma unroll
-    for (int r = 0; r < kPH; ++r) {
-        const int j = tid + r * kFB;
-        if (a.use_bend && j < nhin) {
-            rh[r] = A.hin_idx[ebeg + j];
-            rw[r] = A.hin_w[ebeg + j];
-            rc[r] = A.hin_c[ebeg + j];
-        }
+    const int* hids = A.halo + (size_t)tile * A.hstride;
+    const int hid0 = tid < A.hstride ? hids[tid] : 0;
+
+    const int nhalo = ti[0], tbeg = ti[1], ntri = ti[2], ebeg = ti[3], nhin = ti[4];
+    const int gbeg = ti[5], glen = ti[6];
+
+    // ---- trip 2: halo positions, first records, gather list -------------------
+    if (tid < nhalo) {
+        if (PD) stsX(sX, L, nown + tid, to_d(ld3(a.base, hid0)) + ld3d(xd, hid0));
+        else    sts3(sx, nown + tid, ld3(xf, hid0));
     }
-    // the tile's gather list
-    for (int j = tid; j < gend - gbeg; j += kFB) sg[j] = A.gl_idx[gbeg + j];
-    // halo positions (ids first)
-    for (int i = tid; i < nhalo; i += kFB) {
-        const int hg = A.halo[hbeg + i];
-        if (PD) {
-            sts3d(sd, nown + i, ld3d(xd, hg));
-            sts3(sb, nown + i, ld3(a.base, hg));
-        } else {
-            sts3(sx, nown + i, ld3(xf, hg));
-        }
+    for (int i = tid + kT; i < nhalo; i += kT) {
+        const int hg = hids[i];
+        if (PD) stsX(sX, L, nown + i, to_d(ld3(a.base, hg)) + ld3d(xd, hg));
+        else    sts3(sx, nown + i, ld3(xf, hg));
     }
+    uint2 rt = make_uint2(0, 0);
+    float4 rG = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float rwA = 0.0f;
+    if (a.use_stretch && tid < ntri) {
+        rt = A.tri_idx[tbeg + tid];
+        rG = A.tri_c0[tbeg + tid];
+        rwA = A.tri_c1[tbeg + tid];
+    }
+    for (int j = tid; j < (glen + 1) >> 1; j += kT) sgw[j] = A.gl_words[gbeg + j];
     if (own) {
-        if (PD) {
-            sts3d(sd, tid, ox_d);
-            sts3(sb, tid, ob);
-        } else {
-            sts3(sx, tid, ox_f);
-        }
+        if (PD) stsX(sX, L, tid, to_d(ob) + od);
+        else    sts3(sx, tid, ob);
     }
     __syncthreads();
 
     // ---- triangles -> corner forces (src/pd_solver.cu:48-298) -----------------
-    auto triangle = [&](int j, uint2 id, float4 G, float wA) {
-        const int l0 = id.x & 0xFFFF, l1 = id.x >> 16, l2 = id.y & 0xFFFF;
-        f3 c[3];
-        float* o = st + (size_t)j * 9;
-        if (PD) {
-            f3 cr[3];
-            pd_triangle(lds3(sb, l0), lds3(sb, l1), lds3(sb, l2), lds3d(sd, l0), lds3d(sd, l1),
-                        lds3d(sd, l2), G, wA, cr, c, kCheck);
-            if (kCheck) {
-                float* q = st2 + (size_t)j * 9;
-                sts3(q, 0, cr[0]);
-                sts3(q, 1, cr[1]);
-                sts3(q, 2, cr[2]);
-            }
-        } else {
-            const f3 x0 = lds3(sx, l0);
-            f3 F0, F1, R0, R1;
-            deformation_nc(lds3(sx, l1) - x0, lds3(sx, l2) - x0, G.x, G.y, G.z, G.w, F0, F1);
-            ref_cuda_rotation(F0, F1, R0, R1);
-            corner_forces_nc(G.x, G.y, G.z, G.w, wA, R0, R1, c);
-        }
-        sts3(o, 0, c[0]);
-        sts3(o, 1, c[1]);
-        sts3(o, 2, c[2]);
-    };
     if (a.use_stretch) {
-#pragma unroll
-        for (int r = 0; r < kPT; ++r)
-            if (tid + r * kFB < ntri) triangle(tid + r * kFB, rt[r], rG[r], rwA[r]);
-        for (int j = tid + kPT * kFB; j < ntri; j += kFB)
-            triangle(j, A.tri_idx[tbeg + j], A.tri_c0[tbeg + j], A.tri_c1[tbeg + j]);
-    }
-    // ---- hinges -> bend vectors (src/pd_solver.cu:304-358) --------------------
-    auto hinge = [&](int j, uint2 id, float4 w, float2 c) {  // c = (n_rest, k h^2)
-        const int q0 = id.x & 0xFFFF, q1 = id.x >> 16, q2 = id.y & 0xFFFF, q3 = id.y >> 16;
-        f3 p, pe;
-        if (PD) {
-            const int qs[4] = {q0, q1, q2, q3};
-            f3 b[4];
-            d3 xs[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                b[i] = lds3(sb, qs[i]);
-                xs[i] = lds3d(sd, qs[i]);
+        for (int j = tid; j < ntri; j += kT) {
+            const uint2 id = rt;
+            const float4 G = rG;
+            const float wA = rwA;
+            if (j + kT < ntri) {  // next pass's record
+                rt = A.tri_idx[tbeg + j + kT];
+                rG = A.tri_c0[tbeg + j + kT];
+                rwA = A.tri_c1[tbeg + j + kT];
             }
-            pd_hinge(b, xs, w, c.x, p, pe);
-        } else {
-            f3 acc = mk(0.0f, 0.0f, 0.0f);
-            acc = madd_nc(acc, w.x, lds3(sx, q0));
-            acc = madd_nc(acc, w.y, lds3(sx, q1));
-            acc = madd_nc(acc, w.z, lds3(sx, q2));
-            acc = madd_nc(acc, w.w, lds3(sx, q3));
-            p = hinge_projection_nc(acc, c.x);
+            const int l0 = id.x & 0xFFFF, l1 = id.x >> 16, l2 = id.y & 0xFFFF;
+            f3 c[3];
+            if (PD) {
+                f3 cr[3];
+                pd_triangle(ldsX(sX, L, l0), ldsX(sX, L, l1), ldsX(sX, L, l2), G, wA, cr, c, kCheck);
+                if (kCheck) {
+                    float* q = st2 + (size_t)j * 9;
+                    sts3(q, 0, cr[0]);
+                    sts3(q, 1, cr[1]);
+                    sts3(q, 2, cr[2]);
+                }
+            } else {
+                const f3 x0 = lds3(sx, l0);
+                f3 F0, F1, R0, R1;
+                deformation_nc(lds3(sx, l1) - x0, lds3(sx, l2) - x0, G.x, G.y, G.z, G.w, F0, F1);
+                ref_cuda_rotation(F0, F1, R0, R1);
+                corner_forces_nc(G.x, G.y, G.z, G.w, wA, R0, R1, c);
+            }
+            float* o = st + (size_t)j * 9;
+            sts3(o, 0, c[0]);
+            sts3(o, 1, c[1]);
+            sts3(o, 2, c[2]);
         }
-        skw[j * 4 + 0] = c.y * w.x;
-        skw[j * 4 + 1] = c.y * w.y;
-        skw[j * 4 + 2] = c.y * w.z;
-        skw[j * 4 + 3] = c.y * w.w;
-        sts3(sq, j, PD ? pe : p);
-        if (PD && kCheck) sts3(sp, j, p);
-    };
-    if (a.use_bend) {
-#pragma unroll
-        for (int r = 0; r < kPH; ++r)
-            if (tid + r * kFB < nhin) hinge(tid + r * kFB, rh[r], rw[r], rc[r]);
-        for (int j = tid + kPH * kFB; j < nhin; j += kFB)
-            hinge(j, A.hin_idx[ebeg + j], A.hin_w[ebeg + j], A.hin_c[ebeg + j]);
+    }
+    // first hinge record (its latency overlaps the triangle sums)
+    uint2 rh = make_uint2(0, 0);
+    float4 rw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float2 rc = make_float2(0.0f, 0.0f);
+    if (a.use_bend && tid < nhin) {
+        rh = A.hin_idx[ebeg + tid];
+        rw = A.hin_w[ebeg + tid];
+        rc = A.hin_c[ebeg + tid];
     }
     __syncthreads();
 
-    // ---- own vertices: gather, update, pins, residual ---------------------------
+    // ---- own vertices: triangle corners (ascending element id) ---------------
+    const int opin = vi.z;
+    const unsigned meta = (unsigned)vi.w;
+    const int j0 = (int)(meta & 0xFFFFu), jt = j0 + (int)((meta >> 16) & 0xFFu),
+              j1 = j0 + (int)(meta >> 24);
+    const bool gather = own && opin < 0;
+    f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
+    if (gather && a.use_stretch) {
+        for (int j = j0; j < jt; ++j) {
+            const int s = sg[j];
+            ar = ar + lds3(st, s);
+            if (PD && kCheck) ab = ab + lds3(st2, s);
+        }
+    }
+    if (a.use_bend) {
+        __syncthreads();  // the hinge vectors reuse the triangle region
+        // ---- hinges -> bend vectors (src/pd_solver.cu:304-358) ----------------
+        for (int j = tid; j < nhin; j += kT) {
+            const uint2 id = rh;
+            const float4 w = rw;
+            const float2 c = rc;  // (n_rest, k h^2)
+            if (j + kT < nhin) {
+                rh = A.hin_idx[ebeg + j + kT];
+                rw = A.hin_w[ebeg + j + kT];
+                rc = A.hin_c[ebeg + j + kT];
+            }
+            const int q0 = id.x & 0xFFFF, q1 = id.x >> 16, q2 = id.y & 0xFFFF, q3 = id.y >> 16;
+            f3 p, pe;
+            if (PD) {
+                const d3 X[4] = {ldsX(sX, L, q0), ldsX(sX, L, q1), ldsX(sX, L, q2), ldsX(sX, L, q3)};
+                pd_hinge(X, w, c.x, p, pe);
+            } else {
+                f3 acc = mk(0.0f, 0.0f, 0.0f);
+                acc = madd_nc(acc, w.x, lds3(sx, q0));
+                acc = madd_nc(acc, w.y, lds3(sx, q1));
+                acc = madd_nc(acc, w.z, lds3(sx, q2));
+                acc = madd_nc(acc, w.w, lds3(sx, q3));
+                p = hinge_projection_nc(acc, c.x);
+            }
+            skw[j] = c.y * w.x;
+            skw[A.hmax + j] = c.y * w.y;
+            skw[2 * A.hmax + j] = c.y * w.z;
+            skw[3 * A.hmax + j] = c.y * w.w;
+            sts3(sq, j, PD ? pe : p);
+            if (PD && kCheck) sts3(sp, j, p);
+        }
+        __syncthreads();
+        if (gather) {
+            for (int j = jt; j < j1; ++j) {
+                const int s = sg[j];
+                const float kw = skw[(s & 3) * A.hmax + (s >> 2)];
+                const f3 d = lds3(sq, s >> 2);
+                if (PD) ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+                else    ar = madd_nc(ar, kw, d);
+                if (PD && kCheck) {
+                    const f3 q = lds3(sp, s >> 2);
+                    ab = ab + mk(kw * q.x, kw * q.y, kw * q.z);
+                }
+            }
+        }
+    }
+
+    // ---- own vertices: update, pins, residual ----------------------------------
     double rr = 0.0, bb = 0.0;
     if (own) {
+        const float om = __int_as_float(vi.x), odg = __int_as_float(vi.y);
         if (opin >= 0) {
             const f3 tgt = ld3(a.pin_target, opin);
             if (PD) st3d(static_cast<double*>(a.out), g, to_d(tgt) - to_d(ob));
             else    st3(static_cast<float*>(a.out), g, tgt);
+        } else if (!PD) {
+            const float inv = (odg > 1e-10f) ? (1.0f / odg) : 0.0f;
+            st3(static_cast<float*>(a.out), g, madd_nc(ar, om, oy) * inv);
         } else {
-            const int jt = oj0 + ont;
-            f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
-            if (a.use_stretch)
-                for (int j = oj0; j < jt; ++j) {
-                    const int s = sg[j];
-                    ar = ar + lds3(st, s);
-                    if (PD && kCheck) ab = ab + lds3(st2, s);
-                }
-            if (a.use_bend)
-                for (int j = jt; j < oj1; ++j) {
-                    const int s = sg[j];
-                    const float kw = skw[s];
-                    const f3 d = lds3(sq, s >> 2);
-                    if (PD) ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
-                    else    ar = madd_nc(ar, kw, d);
-                    if (PD && kCheck) {
-                        const f3 p = lds3(sp, s >> 2);
-                        ab = ab + mk(kw * p.x, kw * p.y, kw * p.z);
-                    }
-                }
-            if (!PD) {
-                const float inv = (odg > 1e-10f) ? (1.0f / odg) : 0.0f;
-                st3(static_cast<float*>(a.out), g, madd_nc(ar, om, oy) * inv);
-            } else {
-                const double md = om;
-                const d3 r = d3{(double)ar.x + md * ((double)oy.x - ox_d.x),
-                                (double)ar.y + md * ((double)oy.y - ox_d.y),
-                                (double)ar.z + md * ((double)oy.z - ox_d.z)};
-                const double inv = (odg > 1e-10f) ? 1.0 / (double)odg : 0.0;
-                d3 xn = ox_d + r * inv;
-                if (a.blend) xn = (xn - oxp) * (double)a.omega + oxp;
-                st3d(static_cast<double*>(a.out), g, xn);
-                if (kCheck) {
-                    const f3 ya = ob + oy;
-                    const f3 b = mk(ab.x + om * ya.x, ab.y + om * ya.y, ab.z + om * ya.z);
-                    rr = r.x * r.x + r.y * r.y + r.z * r.z;
-                    bb = (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
-                }
+            const double md = om;
+            const d3 r = d3{(double)ar.x + md * ((double)oy.x - od.x),
+                            (double)ar.y + md * ((double)oy.y - od.y),
+                            (double)ar.z + md * ((double)oy.z - od.z)};
+            const double inv = (odg > 1e-10f) ? 1.0 / (double)odg : 0.0;
+            d3 xn = od + r * inv;
+            if (a.blend) xn = (xn - oxp) * (double)a.omega + oxp;
+            st3d(static_cast<double*>(a.out), g, xn);
+            if (kCheck) {
+                const f3 ya = ob + oy;
+                const f3 b = mk(ab.x + om * ya.x, ab.y + om * ya.y, ab.z + om * ya.z);
+                rr = r.x * r.x + r.y * r.y + r.z * r.z;
+                bb = (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
             }
         }
     }
-    if (kCheck) fused_block_sum2(rr, bb, a.partials, tile);
+    if (kCheck) fused_block_sum2<kT>(rr, bb, a.partials, tile);
 }
 
 // ---- tile-order frame kernels ------------------------------------------------
@@ -425,9 +424,9 @@ __global__ void __launch_bounds__(256)
 k_fused_prologue(const int* __restrict__ perm, const float* __restrict__ pos,
                  const float* __restrict__ vel, const float* __restrict__ guess,
                  const float* __restrict__ mass_in, const float* __restrict__ diag_in,
-                 float* __restrict__ xprev, float* __restrict__ y, void* __restrict__ x0,
-                 float* __restrict__ mass, float* __restrict__ diag, int* __restrict__ pin_map,
-                 int N, float dt, float h2a0, float h2a1, float h2a2, int disp)
+                 const unsigned* __restrict__ gl_meta, float* __restrict__ xprev,
+                 float* __restrict__ y, void* __restrict__ x0, int4* __restrict__ vinfo, int N,
+                 float dt, float h2a0, float h2a1, float h2a2, int disp)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -443,34 +442,33 @@ k_fused_prologue(const int* __restrict__ perm, const float* __restrict__ pos,
         st3(y, i, predict_nc(x, u, dt, h2a0, h2a1, h2a2));
         st3(static_cast<float*>(x0), i, guess ? ld3(guess, o) : x);
     }
-    mass[i] = mass_in[o];
-    diag[i] = diag_in[o];
-    pin_map[i] = -1;
+    vinfo[i] = make_int4(__float_as_int(mass_in[o]), __float_as_int(diag_in[o]), -1,
+                         (int)gl_meta[i]);
 }
 
 __global__ void __launch_bounds__(256)
 k_fused_pins(const int* __restrict__ inv_perm, const int* __restrict__ idx,
              const float* __restrict__ target, const float* __restrict__ xprev,
-             int* __restrict__ pin_map, void* __restrict__ x0, int P, int disp)
+             int4* __restrict__ vinfo, void* __restrict__ x0, int P, int disp)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P) return;
     const int i = inv_perm[idx[k]];
-    pin_map[i] = k;
+    vinfo[i].z = k;
     if (disp)
         st3d(static_cast<double*>(x0), i, to_d(ld3(target, k)) - to_d(ld3(xprev, i)));
 }
 
 __global__ void __launch_bounds__(256)
 k_fused_epilogue(const int* __restrict__ perm, const void* __restrict__ xf,
-                 const float* __restrict__ xprev, const int* __restrict__ pin_map,
+                 const float* __restrict__ xprev, const int4* __restrict__ vinfo,
                  const float* __restrict__ target, float* __restrict__ pos,
                  float* __restrict__ vel, int N, float scale, int disp)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int o = perm[i];
-    const int slot = pin_map[i];
+    const int slot = vinfo[i].z;
     if (slot >= 0) {
         st3(pos, o, ld3(target, slot));
         st3(vel, o, mk(0.0f, 0.0f, 0.0f));
@@ -523,13 +521,27 @@ k_pack_hinges(const int* __restrict__ gid, const float* __restrict__ w,
     hc[j] = make_float2(nrest[e], bk[e] * h2);
 }
 
-size_t lds_bytes(const FusedPlan& p, int mode, bool check)
+size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+// LDS byte offsets of the fused kernel: position table, gather list, element
+// region (triangle corner forces, then hinge vectors).
+struct LdsLayout {
+    size_t off_g, off_r, total;
+};
+
+LdsLayout lds_layout(const FusedPlan& p, int mode, bool check)
 {
-    const bool pd = (mode == CLH_MODE_PD);
-    size_t f = (size_t)p.lmax * 3 * (pd ? 3 : 1) + (size_t)p.tmax * 9 * ((pd && check) ? 2 : 1) +
-               (size_t)p.hmax * (4 + 3 + ((pd && check) ? 3 : 0));
-    return f * sizeof(float) + (size_t)(p.gmax + 1) / 2 * 4;
+    const bool pd = (mode == CLH_MODE_PD), two = pd && check;
+    LdsLayout L;
+    L.off_g = align16((size_t)p.lmax * (pd ? 24 : 12));
+    L.off_r = L.off_g + align16((size_t)(p.gmax + 1) / 2 * 4);
+    const size_t ra = (size_t)p.tmax * 9 * (two ? 2 : 1) * sizeof(float);
+    const size_t rb = (size_t)p.hmax * (4 + 3 + (two ? 3 : 0)) * sizeof(float);
+    L.total = L.off_r + align16(std::max(ra, rb));
+    return L;
 }
+
+size_t lds_bytes(const FusedPlan& p, int mode, bool check) { return lds_layout(p, mode, check).total; }
 
 unsigned blocks_for(int n) { return (unsigned)((n + 255) / 256); }
 
@@ -551,13 +563,13 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
     incidence(quads, 4, N, ve_ptr, ve_ids);
 
     const int ntiles = (N + V - 1) / V;
-    std::vector<int> info((size_t)ntiles * kTileInfo), halo, tri_gid, hin_gid;
+    std::vector<int> info((size_t)ntiles * kTileInfo), halo_list, halo_beg((size_t)ntiles + 1, 0);
+    std::vector<int> tri_gid, hin_gid;
     std::vector<uint2> tri_idx, hin_idx;
-    std::vector<int> gl_ptr((size_t)N + 1, 0);
     std::vector<unsigned short> gl_idx;
-    std::vector<unsigned char> gl_ntri((size_t)N, 0);
+    std::vector<unsigned> gl_meta((size_t)N, 0);
     std::vector<int> loc((size_t)N, -1);  // tile-order id -> local id in the current tile
-    int lmax = 0, tmax = 0, hmax = 0, gmax = 0;
+    int lmax = 0, tmax = 0, hmax = 0, gmax = 0, hstride = 1;
 
     for (int t = 0; t < ntiles; ++t) {
         const int vbeg = t * V, nown = std::min(N, vbeg + V) - vbeg;
@@ -577,12 +589,11 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
             touched.push_back(vbeg + i);
         }
         int nl = nown;
-        const int hbeg = (int)halo.size();
         auto local_of = [&](int orig) {
             const int g = inv[(size_t)orig];
             if (loc[(size_t)g] < 0) {
                 loc[(size_t)g] = nl++;
-                halo.push_back(g);
+                halo_list.push_back(g);
                 touched.push_back(g);
             }
             return loc[(size_t)g];
@@ -603,10 +614,15 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
         }
         if (nl > 65535 || (int)tl.size() * 3 > 65535 || (int)hl.size() * 4 > 65535)
             return fail(CLH_ERR_UNSUPPORTED, "fused plan: tile exceeds 16-bit local indexing");
+        halo_beg[(size_t)t + 1] = (int)halo_list.size();
 
-        // gather lists: triangle corners then hinge slots, ascending element id
+        // gather lists: triangle corners then hinge slots, ascending element
+        // id; each tile's list starts on a 32-bit word
+        if (gl_idx.size() & 1) gl_idx.push_back(0);
+        const size_t gbeg = gl_idx.size();
         for (int i = 0; i < nown; ++i) {
             const int g = vbeg + i, o = perm[(size_t)g];
+            const size_t start = gl_idx.size() - gbeg;
             int nt = 0;
             for (int j = vt_ptr[(size_t)o]; j < vt_ptr[(size_t)o + 1]; ++j) {
                 const int e = vt_ids[(size_t)j];
@@ -617,31 +633,46 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
                         ++nt;
                     }
             }
-            if (nt > 255) return fail(CLH_ERR_UNSUPPORTED, "fused plan: vertex valence above 255");
-            gl_ntri[(size_t)g] = (unsigned char)nt;
             for (int j = ve_ptr[(size_t)o]; j < ve_ptr[(size_t)o + 1]; ++j) {
                 const int e = ve_ids[(size_t)j];
                 const int slot = (int)(std::lower_bound(hl.begin(), hl.end(), e) - hl.begin());
                 for (int c = 0; c < 4; ++c)
                     if (quads[(size_t)e * 4 + c] == o) gl_idx.push_back((unsigned short)(slot * 4 + c));
             }
-            gl_ptr[(size_t)g + 1] = (int)gl_idx.size();
+            const size_t nall = gl_idx.size() - gbeg - start;
+            if (nall > 255) return fail(CLH_ERR_UNSUPPORTED, "fused plan: vertex valence above 255");
+            if (start > 0xFFFF) return fail(CLH_ERR_UNSUPPORTED, "fused plan: tile gather list too long");
+            gl_meta[(size_t)g] = (unsigned)start | ((unsigned)nt << 16) | ((unsigned)nall << 24);
         }
+        const size_t glen = gl_idx.size() - gbeg;
+        if (glen > 0xFFFF + 255) return fail(CLH_ERR_UNSUPPORTED, "fused plan: tile gather list too long");
         int* ti = &info[(size_t)t * kTileInfo];
-        ti[0] = vbeg; ti[1] = nown; ti[2] = hbeg; ti[3] = nl - nown;
-        ti[4] = tbeg; ti[5] = (int)tl.size(); ti[6] = ebeg; ti[7] = (int)hl.size();
-        ti[8] = gl_ptr[(size_t)vbeg]; ti[9] = gl_ptr[(size_t)vbeg + nown];
-        gmax = std::max(gmax, ti[9] - ti[8]);
+        ti[0] = nl - nown; ti[1] = tbeg; ti[2] = (int)tl.size(); ti[3] = ebeg; ti[4] = (int)hl.size();
+        ti[5] = (int)(gbeg / 2); ti[6] = (int)glen; ti[7] = vbeg;
+        gmax = std::max(gmax, (int)glen);
         lmax = std::max(lmax, nl);
         tmax = std::max(tmax, (int)tl.size());
         hmax = std::max(hmax, (int)hl.size());
+        hstride = std::max(hstride, nl - nown);
         for (int g : touched) loc[(size_t)g] = -1;
+    }
+    if (gl_idx.size() & 1) gl_idx.push_back(0);
+    if (gl_idx.empty()) gl_idx.assign(2, 0);
+
+    // halo ids at a fixed stride per tile, padded with the tile's first own
+    // vertex (an in-range id that is never read as a halo vertex)
+    std::vector<int> halo((size_t)ntiles * hstride);
+    for (int t = 0; t < ntiles; ++t) {
+        const int n = halo_beg[(size_t)t + 1] - halo_beg[(size_t)t];
+        for (int i = 0; i < hstride; ++i)
+            halo[(size_t)t * hstride + i] = i < n ? halo_list[(size_t)halo_beg[(size_t)t] + i] : t * V;
     }
 
     FusedPlan& p = h.meta;
     p.N = N;
     p.num_tiles = ntiles;
     p.tile_verts = V;
+    p.hstride = hstride;
     p.lmax = lmax;
     p.tmax = std::max(tmax, 1);
     p.hmax = std::max(hmax, 1);
@@ -651,18 +682,17 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
     p.bend_factor = E > 0 ? (double)hin_gid.size() / E : 0.0;
     p.xcd_remap = (ntiles % 8 == 0) ? 1 : 0;
     // bytes the PD iteration streams: per vertex the double iterates x_k,
-    // x_{k-1}, x_{k+1} (24 B each), base and dy (12 B each), mass, diag, pin
-    // map, gather offset and count; per halo vertex its id, iterate and
-    // base; packed triangle (28 B) and hinge (32 B) records; gather entries
-    // (2 B); tile descriptors
-    p.bytes_per_iter = (double)N * (3 * 24 + 2 * 12 + 4 + 4 + 4 + 4 + 1) + (double)halo.size() * (4 + 36) +
+    // x_{k-1}, x_{k+1} (24 B each), base and dy (12 B each) and the 16-byte
+    // vertex record; per tile its descriptor and padded halo id block; per
+    // halo vertex its iterate and base; packed triangle (28 B) and hinge
+    // (32 B) records; gather entries (2 B)
+    p.bytes_per_iter = (double)N * (3 * 24 + 2 * 12 + 16) + (double)halo_list.size() * 36 +
+                       (double)ntiles * (kTileInfo + hstride) * 4 +
                        (double)tri_gid.size() * 28 + (double)hin_gid.size() * 32 +
-                       (double)gl_idx.size() * 2 + (double)ntiles * kTileInfo * 4;
+                       (double)gl_idx.size() * 2;
 
-    if (halo.empty()) halo.push_back(0);
     if (tri_gid.empty()) { tri_gid.push_back(0); tri_idx.push_back(make_uint2(0, 0)); }
     if (hin_gid.empty()) { hin_gid.push_back(0); hin_idx.push_back(make_uint2(0, 0)); }
-    if (gl_idx.empty()) gl_idx.push_back(0);
     h.perm.swap(perm);
     h.inv.swap(inv);
     h.info.swap(info);
@@ -671,44 +701,52 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
     h.hin_gid.swap(hin_gid);
     h.tri_idx.swap(tri_idx);
     h.hin_idx.swap(hin_idx);
-    h.gl_ptr.swap(gl_ptr);
     h.gl_idx.swap(gl_idx);
-    h.gl_ntri.swap(gl_ntri);
+    h.gl_meta.swap(gl_meta);
     return CLH_OK;
 }
 
-// Checks every index of a host plan against the mesh: local ids inside the
-// tile's table, halo ids outside the own range, each local slot naming the
-// element's vertex, and each own vertex's gather list equal to its
-// incident (element, corner) slots in ascending order (the order of the
-// unfused pipeline and the oracle).
+// Checks every index of a host plan against the mesh: own ranges, local ids
+// inside the tile's table, halo ids outside the own range, each local slot
+// naming the element's vertex, and each own vertex's gather list equal to
+// its incident (element, corner) slots in ascending order (the order of the
+// unfused pipeline and the oracle) and inside the tile's gather segment.
 int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::vector<int>& quads)
 {
     const FusedPlan& p = h.meta;
-    const int N = p.N;
+    const int N = p.N, V = p.tile_verts;
     if ((int)h.perm.size() != N || (int)h.inv.size() != N) return fail(CLH_ERR_INVALID, "plan: perm size");
     for (int i = 0; i < N; ++i)
         if (h.perm[(size_t)i] < 0 || h.perm[(size_t)i] >= N || h.inv[(size_t)h.perm[(size_t)i]] != i)
             return fail(CLH_ERR_INVALID, "plan: perm is not a permutation");
+    if ((long long)p.num_tiles * V < N || (long long)(p.num_tiles - 1) * V >= N)
+        return fail(CLH_ERR_INVALID, "plan: tile count");
+    if (h.halo.size() != (size_t)p.num_tiles * p.hstride) return fail(CLH_ERR_INVALID, "plan: halo size");
+    for (int v : h.halo)
+        if (v < 0 || v >= N) return fail(CLH_ERR_INVALID, "plan: halo id range");
     std::vector<int> vt_ptr, vt_ids, ve_ptr, ve_ids;
     build_slots(tris, 3, N, vt_ptr, vt_ids);
     build_slots(quads, 4, N, ve_ptr, ve_ids);
     const int T = (int)(tris.size() / 3), E = (int)(quads.size() / 4);
-    int covered = 0;
     for (int t = 0; t < p.num_tiles; ++t) {
         const int* ti = &h.info[(size_t)t * kTileInfo];
-        const int vbeg = ti[0], nown = ti[1], hbeg = ti[2], nhalo = ti[3];
-        const int tbeg = ti[4], ntri = ti[5], ebeg = ti[6], nhin = ti[7];
+        const int vbeg = t * V, nown = std::min(V, N - vbeg);
+        const int nhalo = ti[0], tbeg = ti[1], ntri = ti[2], ebeg = ti[3], nhin = ti[4];
+        const size_t gbeg = (size_t)ti[5] * 2;
+        const int glen = ti[6];
         const int nl = nown + nhalo;
-        if (vbeg != covered || nown <= 0 || nown > p.tile_verts) return fail(CLH_ERR_INVALID, "plan: own range");
-        covered += nown;
-        if (nl > p.lmax || ntri > p.tmax || nhin > p.hmax) return fail(CLH_ERR_INVALID, "plan: tile maxima");
-        if (ti[8] != h.gl_ptr[(size_t)vbeg] || ti[9] != h.gl_ptr[(size_t)vbeg + nown] || ti[9] - ti[8] > p.gmax)
+        if (ti[7] != vbeg) return fail(CLH_ERR_INVALID, "plan: own range");
+        if (nhalo < 0 || nhalo > p.hstride || nl > p.lmax || ntri > p.tmax || nhin > p.hmax)
+            return fail(CLH_ERR_INVALID, "plan: tile maxima");
+        if (tbeg < 0 || (size_t)tbeg + ntri > h.tri_idx.size() || ebeg < 0 ||
+            (size_t)ebeg + nhin > h.hin_idx.size())
+            return fail(CLH_ERR_INVALID, "plan: element range");
+        if (glen < 0 || glen > p.gmax || gbeg + (size_t)((glen + 1) & ~1) > h.gl_idx.size())
             return fail(CLH_ERR_INVALID, "plan: gather range");
-        auto tile_id = [&](int l) { return l < nown ? vbeg + l : h.halo[(size_t)hbeg + l - nown]; };
+        auto tile_id = [&](int l) { return l < nown ? vbeg + l : h.halo[(size_t)t * p.hstride + l - nown]; };
         for (int l = nown; l < nl; ++l) {
             const int g = tile_id(l);
-            if (g < 0 || g >= N || (g >= vbeg && g < vbeg + nown)) return fail(CLH_ERR_INVALID, "plan: halo id");
+            if (g >= vbeg && g < vbeg + nown) return fail(CLH_ERR_INVALID, "plan: halo id");
         }
         for (int j = 0; j < ntri; ++j) {
             const uint2 id = h.tri_idx[(size_t)tbeg + j];
@@ -730,10 +768,12 @@ int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::
         }
         for (int i = 0; i < nown; ++i) {
             const int g = vbeg + i, o = h.perm[(size_t)g];
-            const int j0 = h.gl_ptr[(size_t)g], j1 = h.gl_ptr[(size_t)g + 1], nt = h.gl_ntri[(size_t)g];
+            const unsigned m = h.gl_meta[(size_t)g];
+            const int j0 = (int)(m & 0xFFFFu), nt = (int)((m >> 16) & 0xFFu), na = (int)(m >> 24);
+            if (nt > na || j0 + na > glen) return fail(CLH_ERR_INVALID, "plan: gather meta");
             std::vector<int> got;
-            for (int j = j0; j < j1; ++j) {
-                const int sl = h.gl_idx[(size_t)j];
+            for (int j = j0; j < j0 + na; ++j) {
+                const int sl = h.gl_idx[gbeg + (size_t)j];
                 if (j < j0 + nt) {
                     if (sl / 3 >= ntri) return fail(CLH_ERR_INVALID, "plan: gather triangle slot");
                     got.push_back(h.tri_gid[(size_t)tbeg + sl / 3] * 3 + sl % 3);
@@ -748,7 +788,6 @@ int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::
             if (got != want) return fail(CLH_ERR_INVALID, "plan: gather list differs from element order");
         }
     }
-    if (covered != N) return fail(CLH_ERR_INVALID, "plan: tiles do not cover the vertices");
     return CLH_OK;
 }
 
@@ -759,7 +798,7 @@ int fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
     HostPlan h;
     CLH_TRY(host_plan_build(h, N, tris, quads, pos, tile_verts));
     const FusedPlan& m = h.meta;
-    p.N = m.N; p.num_tiles = m.num_tiles; p.tile_verts = m.tile_verts;
+    p.N = m.N; p.num_tiles = m.num_tiles; p.tile_verts = m.tile_verts; p.hstride = m.hstride;
     p.lmax = m.lmax; p.tmax = m.tmax; p.hmax = m.hmax; p.gmax = m.gmax; p.vmax = m.vmax;
     p.tri_factor = m.tri_factor; p.bend_factor = m.bend_factor; p.xcd_remap = m.xcd_remap;
     p.bytes_per_iter = m.bytes_per_iter;
@@ -775,9 +814,10 @@ int fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
     CLH_TRY(p.tri_c1.resize(h.tri_gid.size()));
     CLH_TRY(p.hin_w.resize(h.hin_gid.size()));
     CLH_TRY(p.hin_c.resize(h.hin_gid.size()));
-    CLH_TRY(p.gl_ptr.upload(h.gl_ptr));
-    CLH_TRY(p.gl_idx.upload(h.gl_idx));
-    CLH_TRY(p.gl_ntri.upload(h.gl_ntri));
+    std::vector<unsigned> words(h.gl_idx.size() / 2);
+    std::memcpy(words.data(), h.gl_idx.data(), words.size() * sizeof(unsigned));
+    CLH_TRY(p.gl_words.upload(words));
+    CLH_TRY(p.gl_meta.upload(h.gl_meta));
     std::memset(p.packed_src, 0, sizeof(p.packed_src));
     p.packed_h2 = -1.0f;
     return CLH_OK;
@@ -818,22 +858,37 @@ int fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
     A.hin_idx = p.hin_idx.ptr;
     A.hin_w = p.hin_w.ptr;
     A.hin_c = p.hin_c.ptr;
-    A.gl_ptr = p.gl_ptr.ptr;
-    A.gl_idx = p.gl_idx.ptr;
-    A.gl_ntri = p.gl_ntri.ptr;
+    A.gl_words = p.gl_words.ptr;
+    A.N = p.N;
+    A.tile_verts = p.tile_verts;
+    A.hstride = p.hstride;
     A.lmax = p.lmax;
     A.tmax = p.tmax;
     A.hmax = p.hmax;
     A.num_tiles = p.num_tiles;
     A.xcd_remap = p.xcd_remap;
     const bool check = a.partials != nullptr;
-    const size_t lds = lds_bytes(p, mode, check);
-    const dim3 grid(p.num_tiles), block(kFB);
-    if (mode == CLH_MODE_PD) {
-        if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true>), grid, block, lds, stream, A);
-        else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false>), grid, block, lds, stream, A);
+    const LdsLayout L = lds_layout(p, mode, check);
+    A.off_g = (int)L.off_g;
+    A.off_r = (int)L.off_r;
+    const size_t lds = L.total;
+    const dim3 grid(p.num_tiles);
+    if (p.threads == 512) {
+        const dim3 block(512);
+        if (mode == CLH_MODE_PD) {
+            if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true, 512>), grid, block, lds, stream, A);
+            else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false, 512>), grid, block, lds, stream, A);
+        } else {
+            hipLaunchKernelGGL((k_fused_iter<CLH_MODE_REF_CUDA, false, 512>), grid, block, lds, stream, A);
+        }
     } else {
-        hipLaunchKernelGGL((k_fused_iter<CLH_MODE_REF_CUDA, false>), grid, block, lds, stream, A);
+        const dim3 block(256);
+        if (mode == CLH_MODE_PD) {
+            if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true, 256>), grid, block, lds, stream, A);
+            else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false, 256>), grid, block, lds, stream, A);
+        } else {
+            hipLaunchKernelGGL((k_fused_iter<CLH_MODE_REF_CUDA, false, 256>), grid, block, lds, stream, A);
+        }
     }
     return hip_status(hipGetLastError(), "fused iteration launch");
 }
@@ -845,9 +900,12 @@ int fused_enable_lds(int& bytes)
     // above 64 KB the kernels must opt in; when the runtime refuses, stay at
     // 64 KB
     if (bytes <= 64 * 1024) return CLH_OK;
-    const void* fns[] = {reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false>)};
+    const void* fns[] = {reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 256>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 256>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false, 256>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 512>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 512>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false, 512>)};
     for (const void* f : fns) {
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -859,24 +917,23 @@ int fused_enable_lds(int& bytes)
 }
 
 int fused_prologue(const FusedPlan& p, const clh_step_args& a, float* xprev, float* y, void* x0,
-                   float* mass, float* diag, int* pin_map, float dt, const float h2a[3],
-                   int disp, hipStream_t stream)
+                   int4* vinfo, float dt, const float h2a[3], int disp, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_fused_prologue, dim3(blocks_for(p.N)), dim3(256), 0, stream, p.perm.ptr,
-                       a.d_pos, a.d_vel, a.d_init_guess, a.d_mass, a.d_jacobi_diag, xprev, y, x0,
-                       mass, diag, pin_map, p.N, dt, h2a[0], h2a[1], h2a[2], disp);
+                       a.d_pos, a.d_vel, a.d_init_guess, a.d_mass, a.d_jacobi_diag, p.gl_meta.ptr,
+                       xprev, y, x0, vinfo, p.N, dt, h2a[0], h2a[1], h2a[2], disp);
     if (a.num_pinned > 0)
         hipLaunchKernelGGL(k_fused_pins, dim3(blocks_for(a.num_pinned)), dim3(256), 0, stream,
-                           p.inv_perm.ptr, a.d_pin_idx, a.d_pin_target, xprev, pin_map, x0,
+                           p.inv_perm.ptr, a.d_pin_idx, a.d_pin_target, xprev, vinfo, x0,
                            a.num_pinned, disp);
     return hip_status(hipGetLastError(), "fused prologue launch");
 }
 
 int fused_epilogue(const FusedPlan& p, const clh_step_args& a, const void* xf, const float* xprev,
-                   const int* pin_map, float scale, int disp, hipStream_t stream)
+                   const int4* vinfo, float scale, int disp, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_fused_epilogue, dim3(blocks_for(p.N)), dim3(256), 0, stream, p.perm.ptr,
-                       xf, xprev, pin_map, a.d_pin_target, a.d_pos, a.d_vel, p.N, scale, disp);
+                       xf, xprev, vinfo, a.d_pin_target, a.d_pos, a.d_vel, p.N, scale, disp);
     return hip_status(hipGetLastError(), "fused epilogue launch");
 }
 
@@ -923,6 +980,7 @@ void fused_plan_info(const FusedPlan& p, clh_plan_info* out)
     out->halo_tri_factor = p.tri_factor;
     out->halo_bend_factor = p.bend_factor;
     out->bytes_per_iter = p.bytes_per_iter;
+    out->block_threads = p.threads;
 }
 
 void fused_plan_release(FusedPlan& p)
@@ -939,9 +997,8 @@ void fused_plan_release(FusedPlan& p)
     p.hin_w.release();
     p.tri_c1.release();
     p.hin_c.release();
-    p.gl_ptr.release();
-    p.gl_idx.release();
-    p.gl_ntri.release();
+    p.gl_words.release();
+    p.gl_meta.release();
     p.num_tiles = 0;
 }
 

@@ -259,15 +259,16 @@ __device__ __forceinline__ void polar_rotation_d(d3 F0, d3 F1, d3& R0, d3& R1)
     R1 = u2 * cs - u1 * sn;
 }
 
-// PD triangle: corner forces of R (fr) and of R - F (fd) from frame-start
-// positions b and displacements x of the three corners, all in double,
-// rounded to FP32 at the end.
-__device__ __forceinline__ void pd_triangle(f3 b0, f3 b1, f3 b2, d3 x0, d3 x1, d3 x2, float4 G,
-                                            float wAf, f3 fr[3], f3 fd[3], bool want_fr)
+// PD triangle: corner forces of R (fr) and of R - F (fd) from the corner
+// positions X = x_n + (x - x_n), all in double, rounded to FP32 at the end.
+// The edges X_j - X_0 are formed from the FP64 sums, so their rounding is
+// 1e-16 of the cloth's coordinates, far below the FP32 rounding of the
+// stored results.
+__device__ __forceinline__ void pd_triangle(d3 X0, d3 X1, d3 X2, float4 G, float wAf, f3 fr[3],
+                                            f3 fd[3], bool want_fr)
 {
-    const d3 db0 = to_d(b0);
-    const d3 d0 = (to_d(b1) - db0) + (x1 - x0);
-    const d3 d1 = (to_d(b2) - db0) + (x2 - x0);
+    const d3 d0 = X1 - X0;
+    const d3 d1 = X2 - X0;
     const double g00 = G.x, g10 = G.y, g01 = G.z, g11 = G.w, wA = wAf;
     const d3 F0 = d0 * g00 + d1 * g10;
     const d3 F1 = d0 * g01 + d1 * g11;
@@ -285,20 +286,18 @@ __device__ __forceinline__ void pd_triangle(f3 b0, f3 b1, f3 b2, d3 x0, d3 x1, d
     }
 }
 
-// PD hinge: e = sum_{i>0} w_i ((b_i - b_0) + (x_i - x_0)) + (sum w_i)(b_0 + x_0)
-// in double (sum w_i is zero up to the FP32 rounding of the weights, and is
-// summed in double so that no rounding of it enters e); p = e n_rest / |e|
-// (or 0); returns p and p - e in FP32.
-__device__ __forceinline__ void pd_hinge(const f3 b[4], const d3 x[4], float4 w, float nr, f3& p,
-                                         f3& pe)
+// PD hinge: e = sum_{i>0} w_i (X_i - X_0) + (sum w_i) X_0 in double (sum w_i
+// is zero up to the FP32 rounding of the weights, and is summed in double so
+// that no rounding of it enters e); p = e n_rest / |e| (or 0); returns p and
+// p - e in FP32.
+__device__ __forceinline__ void pd_hinge(const d3 X[4], float4 w, float nr, f3& p, f3& pe)
 {
-    const d3 b0 = to_d(b[0]), x0 = x[0];
     const double ws[4] = {w.x, w.y, w.z, w.w};
     d3 e = d3{0.0, 0.0, 0.0};
 #pragma unroll
-    for (int i = 1; i < 4; ++i) e = e + ((to_d(b[i]) - b0) + (x[i] - x0)) * ws[i];
+    for (int i = 1; i < 4; ++i) e = e + (X[i] - X[0]) * ws[i];
     const double W = ((ws[0] + ws[1]) + ws[2]) + ws[3];
-    e = e + (b0 + x0) * W;
+    e = e + X[0] * W;
     const double q = dot(e, e);
     d3 pd = d3{0.0, 0.0, 0.0};
     if (q > 1e-20 && nr > 1e-10f) pd = e * ((double)nr * rsqrt_d(q));

@@ -105,8 +105,9 @@ k_triangles(const float* __restrict__ base, const void* __restrict__ xv,
     f3 c[3], e[3];
     if (kMode == CLH_MODE_PD) {
         const double* x = static_cast<const double*>(xv);
-        pd_triangle(ld3(base, i0), ld3(base, i1), ld3(base, i2), ld3d(x, i0), ld3d(x, i1),
-                    ld3d(x, i2), make_float4(g00, g10, g01, g11), wA, c, e, true);
+        pd_triangle(to_d(ld3(base, i0)) + ld3d(x, i0), to_d(ld3(base, i1)) + ld3d(x, i1),
+                    to_d(ld3(base, i2)) + ld3d(x, i2), make_float4(g00, g10, g01, g11), wA, c, e,
+                    true);
         float* q = fd + (size_t)t * 9;
         st3(q, 0, e[0]);
         st3(q, 1, e[1]);
@@ -140,18 +141,16 @@ k_hinges(const float* __restrict__ base, const void* __restrict__ xv,
     if (e >= E) return;
     if (kMode == CLH_MODE_PD) {
         const double* x = static_cast<const double*>(xv);
-        f3 b[4];
-        d3 xs[4];
+        d3 X[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int qi = quads[(size_t)e * 4 + i];
-            b[i] = ld3(base, qi);
-            xs[i] = ld3d(x, qi);
+            X[i] = to_d(ld3(base, qi)) + ld3d(x, qi);
         }
         const float4 wv = make_float4(w[(size_t)e * 4], w[(size_t)e * 4 + 1], w[(size_t)e * 4 + 2],
                                       w[(size_t)e * 4 + 3]);
         f3 p, pe;
-        pd_hinge(b, xs, wv, nrest[e], p, pe);
+        pd_hinge(X, wv, nrest[e], p, pe);
         st3(bp, e, p);
         st3(bd, e, pe);
         return;
@@ -415,7 +414,8 @@ struct clh_solver {
     FusedPlan fused;
     int variant = CLH_KERNEL_UNFUSED;
     int tile_verts = 128;
-    DevBuf<float> mass_t, diag_t;  // tile-order copies (fused variant)
+    int block_threads = 256;
+    DevBuf<int4> vinfo;  // tile-order vertex records (fused variant)
 
     // scratch
     DevBuf<float> y, xprev, fr, fd, bp, bd;
@@ -486,6 +486,7 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
         for (int tv = s->tile_verts; tv >= 32; tv /= 2) {
             st = fused_plan_build(s->fused, N, tris, quads, pos, tv);
             if (st != CLH_OK) break;
+            s->fused.threads = s->block_threads;
             if (fused_lds_bytes(s->fused, s->cfg.mode, true) <= (size_t)lds_limit) break;
             st = fail(CLH_ERR_UNSUPPORTED, "fused plan: tile does not fit in LDS");
         }
@@ -509,8 +510,7 @@ int ensure_scratch(clh_solver* s)
     for (auto& b : s->it) CLH_TRY(b.resize(N * 3));
     CLH_TRY(s->pin_map.resize(N));
     if (s->variant == CLH_KERNEL_FUSED) {
-        CLH_TRY(s->mass_t.resize(N));
-        CLH_TRY(s->diag_t.resize(N));
+        CLH_TRY(s->vinfo.resize(N));
     }
     CLH_TRY(s->fr.resize(T * 9));
     CLH_TRY(s->bp.resize(E * 3));
@@ -560,8 +560,8 @@ int enqueue_prologue(clh_solver* s, const clh_step_args& a)
     if (s->variant == CLH_KERNEL_FUSED) {
         const float h2a[3] = {h2a0, h2a1, h2a2};
         CLH_TRY(fused_pack_constants(s->fused, a, h2, s->stream));
-        return fused_prologue(s->fused, a, s->xprev.ptr, s->y.ptr, s->it[0].ptr, s->mass_t.ptr,
-                              s->diag_t.ptr, s->pin_map.ptr, c.dt, h2a, disp, s->stream);
+        return fused_prologue(s->fused, a, s->xprev.ptr, s->y.ptr, s->it[0].ptr, s->vinfo.ptr, c.dt,
+                              h2a, disp, s->stream);
     }
     if (s->N > 0)
         hipLaunchKernelGGL(k_prologue, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream, a.d_pos,
@@ -594,9 +594,7 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
         f.out = s->it[(k + 1) % 3].ptr;
         f.base = s->xprev.ptr;
         f.y = s->y.ptr;
-        f.mass = s->mass_t.ptr;
-        f.diag = s->diag_t.ptr;
-        f.pin_map = s->pin_map.ptr;
+        f.vinfo = s->vinfo.ptr;
         f.pin_target = a.d_pin_target;
         f.partials = check ? s->partials.ptr : nullptr;
         f.omega = omega;
@@ -674,7 +672,7 @@ int enqueue_epilogue(clh_solver* s, const clh_step_args& a, int iters)
     const clh_config& c = s->cfg;
     const float scale = (1.0f - c.damping) / c.dt;
     if (s->variant == CLH_KERNEL_FUSED)
-        return fused_epilogue(s->fused, a, s->it[iters % 3].ptr, s->xprev.ptr, s->pin_map.ptr,
+        return fused_epilogue(s->fused, a, s->it[iters % 3].ptr, s->xprev.ptr, s->vinfo.ptr,
                               scale, c.mode == CLH_MODE_PD ? 1 : 0, s->stream);
     if (s->N > 0)
         hipLaunchKernelGGL(k_epilogue, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream,
@@ -862,6 +860,11 @@ extern "C" int clh_solver_create(const clh_config* cfg, const clh_mesh_desc* mes
     s->T = mesh->num_tris;
     s->E = mesh->num_bend;
     if (cfg->tile_verts > 0) s->tile_verts = std::max(32, std::min(cfg->tile_verts, 1024));
+    if (cfg->block_threads != 0 && cfg->block_threads != 256 && cfg->block_threads != 512) {
+        clh_solver_destroy(s);
+        return fail(CLH_ERR_INVALID, "clh_solver_create: block_threads must be 0, 256 or 512");
+    }
+    if (cfg->block_threads > 0) s->block_threads = cfg->block_threads;
     int st = hip_status(hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking),
                         "hipStreamCreate");
     if (st == CLH_OK) st = hip_status(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming), "hipEventCreate");

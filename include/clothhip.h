@@ -111,6 +111,7 @@ typedef struct clh_config {
     int   kernel_variant;   /* CLH_KERNEL_*                                     */
     int   use_graph;        /* capture fixed-length frames in a hipGraph        */
     int   tile_verts;       /* fused variant: vertices per tile (0 = default)   */
+    int   block_threads;    /* fused variant: workgroup size, 256 or 512 (0 = default) */
 } clh_config;
 
 /* Fills *cfg with the defaults of PDSolverConfig (include/pd_solver.h:5-14)
@@ -210,6 +211,7 @@ typedef struct clh_plan_info {
     double halo_bend_factor; /* hinges processed / E    */
     double bytes_per_iter;   /* bytes the kernel streams per iteration (model) */
     double lds_bytes;        /* LDS per workgroup of the residual-check variant */
+    int    block_threads;    /* workgroup size of the fused kernel */
 } clh_plan_info;
 int clh_solver_plan_info(clh_solver* s, clh_plan_info* out);
 
