@@ -26,6 +26,35 @@ int hip_status(hipError_t e, const char* what);
         if (_s != CLH_OK) return _s;                                   \
     } while (0)
 
+// Device-side control of a PD frame run to a residual tolerance. Each
+// iteration kernel reads the Chebyshev state of its iteration from
+// slot[k & 1] and (block 0) writes the next iteration's into the other slot;
+// the residual reduction after a check iteration records the history,
+// restarts the Chebyshev sequence when the residual grew since the previous
+// check, and raises `stop` on convergence, after which every later kernel of
+// the frame returns at once. The host therefore needs no round trip per check.
+struct CtlSlot {
+    float omega;  // Chebyshev weight of this iteration
+    int   blend;  // apply the blend
+    int   ck;     // iterations since the (re)start of the sequence
+    int   k;      // iteration index in the frame
+};
+struct FrameCtl {
+    CtlSlot slot[2];
+    float  rho;             // spectral radius estimate in use this frame
+    int    use_cheb;
+    int    delay;
+    int    stop;            // converged or non-finite: later kernels are no-ops
+    int    iters;           // iterations run when stop was raised
+    int    checks;          // residual evaluations recorded
+    int    restarts;        // Chebyshev restarts this frame
+    int    converged;
+    int    nonfinite;
+    int    pad;
+    double last_r2;         // residual^2 at the previous check (-1: none)
+    double tol2;            // tolerance^2
+};
+
 // Device buffer owned by the solver.
 template <typename T>
 struct DevBuf {

@@ -32,6 +32,8 @@ struct FusedIterArgs {
     int   blend;
     int   use_stretch;
     int   use_bend;
+    FrameCtl* ctl;         // when non-null, omega and blend come from ctl->slot[par]
+    int   par;
 };
 
 struct FusedPlan {

@@ -202,8 +202,16 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
 {
     constexpr bool PD = (kMode == CLH_MODE_PD);
     extern __shared__ double smem[];
-    const FusedIterArgs& a = A.it;
+    FusedIterArgs& a = A.it;
     const int tid = threadIdx.x;
+    // control state of a frame run to a tolerance: loaded with the first
+    // trip, acted on before anything is written
+    int stopped = 0;
+    CtlSlot cs = {a.omega, a.blend, 0, 0};
+    if (a.ctl) {
+        stopped = a.ctl->stop;
+        cs = a.ctl->slot[a.par];
+    }
 
     const int L = A.lmax;
     int tile = blockIdx.x;
@@ -243,7 +251,8 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
         if (PD) {
             od = ld3d(xd, g);
             ob = ld3(a.base, g);
-            if (a.blend) oxp = ld3d(static_cast<const double*>(a.xkm1), g);
+            // under device control the blend flag arrives with this trip
+            if (a.blend || a.ctl) oxp = ld3d(static_cast<const double*>(a.xkm1), g);
         } else {
             ob = ld3(xf, g);
         }
@@ -253,6 +262,12 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
 
     const int nhalo = ti[0], tbeg = ti[1], ntri = ti[2], ebeg = ti[3], nhin = ti[4];
     const int gbeg = ti[5], glen = ti[6];
+    if (a.ctl) {
+        if (stopped) return;
+        a.omega = cs.omega;
+        a.blend = cs.blend;
+        ctl_write_next(a.ctl, a.par, cs);
+    }
 
     // ---- trip 2: halo positions, first records, gather list -------------------
     if (tid < nhalo) {

@@ -15,6 +15,8 @@
 //    it is the sign-corrected factor of the same code (lines 61-65).
 #include <hip/hip_runtime.h>
 
+#include "clh_internal.h"
+
 namespace clh {
 
 struct f3 {
@@ -350,6 +352,80 @@ __device__ __forceinline__ void polar_rotation(f3 F0, f3 F1, f3& R0, f3& R1)
     // R2d = [[cs, -sn], [sn, cs]]; R = [u1 u2] R2d
     R0 = mk(u1.x * cs + u2.x * sn, u1.y * cs + u2.y * sn, u1.z * cs + u2.z * sn);
     R1 = mk(u2.x * cs - u1.x * sn, u2.y * cs - u1.y * sn, u2.z * cs - u1.z * sn);
+}
+
+// ---------------------------------------------------------------------------
+// Device-side Chebyshev control (FrameCtl): weights of Wang 2015 with delay
+// S = max(1, delay): 1 before S, 2 / (2 - rho^2) at S, 4 / (4 - rho^2 w) after,
+// in FP32 as the host computes them.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float ctl_omega(const FrameCtl& c, int ck, float prev)
+{
+    const int S = c.delay > 1 ? c.delay : 1;
+    const float r2 = c.rho * c.rho;
+    if (ck < S) return 1.0f;
+    if (ck == S) return 2.0f / (2.0f - r2);
+    return 4.0f / (4.0f - r2 * prev);
+}
+
+// Writes the state of the iteration after the one in `s` into slot par ^ 1
+// (block 0 only: the one writer of that slot during a launch; readers use
+// `par`).
+__device__ __forceinline__ void ctl_write_next(FrameCtl* c, int par, const CtlSlot& s)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    CtlSlot n;
+    n.ck = s.ck + 1;
+    n.omega = ctl_omega(*c, n.ck, s.omega);
+    n.blend = (c->use_cheb && n.ck > 0 && n.omega != 1.0f) ? 1 : 0;
+    n.k = s.k + 1;
+    c->slot[par ^ 1] = n;
+}
+
+// Reads the state of iteration slot `par`; false when the frame has stopped.
+__device__ __forceinline__ bool ctl_begin(FrameCtl* c, int par, float& omega, int& blend)
+{
+    if (c->stop) return false;
+    const CtlSlot s = c->slot[par];
+    omega = s.omega;
+    blend = s.blend;
+    ctl_write_next(c, par, s);
+    return true;
+}
+
+// After the residual of a check iteration (slot `par`) has been summed:
+// history, stop on convergence or a non-finite residual, and a Chebyshev
+// restart when the residual grew since the previous check. Every third
+// restart in a frame halves 1 - rho. One thread.
+__device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, double b2, double* hist)
+{
+    const int slot = c->checks;
+    hist[2 * slot + 0] = r2;
+    hist[2 * slot + 1] = b2;
+    c->checks = slot + 1;
+    const int k = c->slot[par].k;
+    if (!(r2 == r2) || r2 > 1.7e308) {
+        c->nonfinite = 1;
+        c->stop = 1;
+        c->iters = k + 1;
+    } else if (r2 <= c->tol2 * b2) {
+        c->converged = 1;
+        c->stop = 1;
+        c->iters = k + 1;
+    } else {
+        if (c->use_cheb && c->last_r2 >= 0.0 && r2 > c->last_r2) {
+            CtlSlot n;
+            n.ck = 0;
+            n.omega = 1.0f;
+            n.blend = 0;
+            n.k = k + 1;
+            c->slot[par ^ 1] = n;
+            c->restarts += 1;
+            if (c->restarts % 3 == 0) c->rho = 1.0f - 2.0f * (1.0f - c->rho);
+        }
+        c->last_r2 = r2;
+    }
 }
 
 }  // namespace clh

@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(kBlock)
 k_triangles(const float* __restrict__ base, const void* __restrict__ xv,
             const int* __restrict__ tris, const float* __restrict__ dminv,
             const float* __restrict__ area, const float* __restrict__ kst, float* __restrict__ fr,
-            float* __restrict__ fd, int T, float h2)
+            float* __restrict__ fd, int T, float h2, const FrameCtl* __restrict__ ctl)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
@@ -120,6 +120,7 @@ k_triangles(const float* __restrict__ base, const void* __restrict__ xv,
         ref_cuda_rotation(F0, F1, R0, R1);
         corner_forces_nc(g00, g10, g01, g11, wA, R0, R1, c);
     }
+    if (ctl && ctl->stop) return;
     float* o = fr + (size_t)t * 9;
     st3(o, 0, c[0]);
     st3(o, 1, c[1]);
@@ -135,7 +136,8 @@ template <int kMode>
 __global__ void __launch_bounds__(kBlock)
 k_hinges(const float* __restrict__ base, const void* __restrict__ xv,
          const int* __restrict__ quads, const float* __restrict__ w,
-         const float* __restrict__ nrest, float* __restrict__ bp, float* __restrict__ bd, int E)
+         const float* __restrict__ nrest, float* __restrict__ bp, float* __restrict__ bd, int E,
+         const FrameCtl* __restrict__ ctl)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
@@ -151,6 +153,7 @@ k_hinges(const float* __restrict__ base, const void* __restrict__ xv,
                                       w[(size_t)e * 4 + 3]);
         f3 p, pe;
         pd_hinge(X, wv, nrest[e], p, pe);
+        if (ctl && ctl->stop) return;
         st3(bp, e, p);
         st3(bd, e, pe);
         return;
@@ -190,6 +193,8 @@ struct VertexArgs {
     float h2;
     float omega;          // Chebyshev weight of this update (PD)
     int   blend;          // apply the Chebyshev blend (PD)
+    FrameCtl* ctl;        // when non-null, omega and blend come from ctl->slot[par]
+    int   par;
 };
 
 // Reference mode: rhs = sum corner forces + sum k h^2 w p + m y,
@@ -251,6 +256,7 @@ __device__ void block_sum2(double r, double b, double* out)
 // x_{k-1} when blending, pinned vertices held at target - base.
 __global__ void __launch_bounds__(kBlock) k_vertex_pd(VertexArgs a)
 {
+    if (a.ctl && !ctl_begin(a.ctl, a.par, a.omega, a.blend)) return;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     const double* x = static_cast<const double*>(a.x);
     double* out = static_cast<double*>(a.out);
@@ -305,8 +311,9 @@ __global__ void __launch_bounds__(kBlock) k_vertex_pd(VertexArgs a)
 // Sums the per-block partials in a fixed order into hist[2*slot .. +1].
 __global__ void __launch_bounds__(kBlock)
 k_reduce_partials(const double* __restrict__ partials, int nblocks, double* __restrict__ hist,
-                  int slot)
+                  int slot, FrameCtl* __restrict__ ctl, int par)
 {
+    if (ctl && ctl->stop) return;
     double r = 0.0, b = 0.0;
     for (int i = threadIdx.x; i < nblocks; i += kBlock) {
         r += partials[2 * i + 0];
@@ -324,9 +331,39 @@ k_reduce_partials(const double* __restrict__ partials, int nblocks, double* __re
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        hist[2 * slot + 0] = sr[0];
-        hist[2 * slot + 1] = sb[0];
+        if (ctl) {
+            ctl_check(ctl, par, sr[0], sb[0], hist);
+        } else {
+            hist[2 * slot + 0] = sr[0];
+            hist[2 * slot + 1] = sb[0];
+        }
     }
+}
+
+// Starts the control state of a frame run to a tolerance.
+__global__ void k_ctl_frame_init(FrameCtl* __restrict__ c, float rho, int use_cheb, int delay,
+                                 double tol2)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    FrameCtl f;
+    f.slot[0].omega = 1.0f;
+    f.slot[0].blend = 0;
+    f.slot[0].ck = 0;
+    f.slot[0].k = 0;
+    f.slot[1] = f.slot[0];
+    f.rho = rho;
+    f.use_cheb = use_cheb;
+    f.delay = delay;
+    f.stop = 0;
+    f.iters = 0;
+    f.checks = 0;
+    f.restarts = 0;
+    f.converged = 0;
+    f.nonfinite = 0;
+    f.pad = 0;
+    f.last_r2 = -1.0;
+    f.tol2 = tol2;
+    *c = f;
 }
 
 // x_{n+1} and v = (x_{n+1} - x_n)(1 - d)/h; pinned vertices at target with
@@ -434,6 +471,17 @@ struct clh_solver {
     clh_step_args graph_args = {};
     bool graph_valid = false;
 
+    // frames run to a tolerance: device control block, its pinned host
+    // snapshots (one per chunk in flight), and graphs of one chunk of
+    // check_every iterations, by the chunk's first iteration modulo 6 (the
+    // iterate buffers rotate by 3, the control slots by 2)
+    DevBuf<FrameCtl> ctl;
+    FrameCtl* h_ctl = nullptr;
+    hipEvent_t ev_chunk[2] = {nullptr, nullptr};
+    hipGraphExec_t chunk_exec[6] = {};
+    clh_step_args chunk_args = {};
+    int chunk_len = 0;
+
     // last pin list whose indices were range-checked
     const int* pins_checked_ptr = nullptr;
     int pins_checked_n = -1;
@@ -446,6 +494,14 @@ struct clh_solver {
 };
 
 namespace {
+
+void drop_chunk_graphs(clh_solver* s)
+{
+    for (auto& g : s->chunk_exec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
 
 int ensure_plan(clh_solver* s, const clh_step_args& a)
 {
@@ -498,6 +554,7 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
     s->plan_quads = a.d_bend_quads;
     s->plan_ready = true;
     s->graph_valid = false;
+    drop_chunk_graphs(s);
     return CLH_OK;
 }
 
@@ -520,6 +577,10 @@ int ensure_scratch(clh_solver* s)
     }
     s->nblocks_v = (int)grid_for(N);
     CLH_TRY(s->partials.resize((size_t)std::max(s->nblocks_v, fused_max_blocks(s->fused)) * 2));
+    CLH_TRY(s->ctl.resize(1));
+    if (!s->h_ctl) CLH_HIP(hipHostMalloc((void**)&s->h_ctl, 2 * sizeof(FrameCtl)));
+    for (auto& e : s->ev_chunk)
+        if (!e) CLH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const int max_checks = s->cfg.max_iterations + 1;
     CLH_TRY(s->hist.resize((size_t)max_checks * 2));
     if (s->h_hist_cap < max_checks) {
@@ -577,9 +638,10 @@ int enqueue_prologue(clh_solver* s, const clh_step_args& a)
 
 // Enqueues iteration k (x_k in it[k%3] -> x_{k+1} in it[(k+1)%3]).
 int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega, bool blend_in,
-                      bool check, int check_slot)
+                      bool check, int check_slot, bool device_ctl = false)
 {
     const clh_config& c = s->cfg;
+    FrameCtl* ctl = device_ctl ? s->ctl.ptr : nullptr;
     const float h2 = c.dt * c.dt;
     const int pd = (c.mode == CLH_MODE_PD);
     const void* xk = s->it[k % 3].ptr;
@@ -601,10 +663,13 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
         f.blend = blend ? 1 : 0;
         f.use_stretch = use_stretch ? 1 : 0;
         f.use_bend = use_bend ? 1 : 0;
+        f.ctl = ctl;
+        f.par = k & 1;
         CLH_TRY(fused_launch_iteration(s->fused, f, c.mode, s->stream));
         if (check)
             hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->stream,
-                               s->partials.ptr, s->fused.num_tiles, s->hist.ptr, check_slot);
+                               s->partials.ptr, s->fused.num_tiles, s->hist.ptr, check_slot, ctl,
+                               k & 1);
         return hip_status(hipGetLastError(), "fused iteration launch");
     }
 
@@ -612,21 +677,21 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
         if (pd)
             hipLaunchKernelGGL(k_triangles<CLH_MODE_PD>, dim3(grid_for(s->T)), dim3(kBlock), 0,
                                s->stream, s->xprev.ptr, xk, a.d_tris, a.d_Dm_inv, a.d_rest_area, a.d_stretch_k,
-                               s->fr.ptr, s->fd.ptr, s->T, h2);
+                               s->fr.ptr, s->fd.ptr, s->T, h2, ctl);
         else
             hipLaunchKernelGGL(k_triangles<CLH_MODE_REF_CUDA>, dim3(grid_for(s->T)), dim3(kBlock),
                                0, s->stream, s->xprev.ptr, xk, a.d_tris, a.d_Dm_inv, a.d_rest_area,
-                               a.d_stretch_k, s->fr.ptr, s->fd.ptr, s->T, h2);
+                               a.d_stretch_k, s->fr.ptr, s->fd.ptr, s->T, h2, nullptr);
     }
     if (use_bend) {
         if (pd)
             hipLaunchKernelGGL(k_hinges<CLH_MODE_PD>, dim3(grid_for(s->E)), dim3(kBlock), 0,
                                s->stream, s->xprev.ptr, xk, a.d_bend_quads, a.d_bend_w, a.d_bend_n, s->bp.ptr,
-                               s->bd.ptr, s->E);
+                               s->bd.ptr, s->E, ctl);
         else
             hipLaunchKernelGGL(k_hinges<CLH_MODE_REF_CUDA>, dim3(grid_for(s->E)), dim3(kBlock), 0,
                                s->stream, s->xprev.ptr, xk, a.d_bend_quads, a.d_bend_w, a.d_bend_n, s->bp.ptr,
-                               s->bd.ptr, s->E);
+                               s->bd.ptr, s->E, nullptr);
     }
     VertexArgs va;
     va.base = s->xprev.ptr;
@@ -655,6 +720,8 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
     va.h2 = h2;
     va.omega = omega;
     va.blend = blend ? 1 : 0;
+    va.ctl = ctl;
+    va.par = k & 1;
     if (s->N > 0) {
         if (pd)
             hipLaunchKernelGGL(k_vertex_pd, dim3(s->nblocks_v), dim3(kBlock), 0, s->stream, va);
@@ -663,7 +730,7 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
     }
     if (check && s->N > 0)
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->stream, s->partials.ptr,
-                           s->nblocks_v, s->hist.ptr, check_slot);
+                           s->nblocks_v, s->hist.ptr, check_slot, ctl, k & 1);
     return hip_status(hipGetLastError(), "iteration launch");
 }
 
@@ -777,41 +844,79 @@ int step_impl(clh_solver* s, const clh_step_args& a)
     }
     if (fixed) return enqueue_fixed_frame(s, a);
 
-    // PD mode with a tolerance: residual tests every check_every iterations,
-    // read back to decide whether to continue. The Chebyshev sequence
-    // restarts (one plain Jacobi step, then the weights from the start)
-    // whenever a test finds the residual above the previous test's.
-    CLH_TRY(enqueue_prologue(s, a));
-    float omega = 1.0f;
-    int checks = 0, k = 0, ck = 0;
-    double last_r2 = -1.0;
-    s->converged = 0;
-    const double tol2 = (double)c.tolerance * (double)c.tolerance;
-    for (; k < c.max_iterations; ++k) {
-        omega = chebyshev_omega(c, ck, omega);
-        const bool check = is_check_iteration(c, k);
-        CLH_TRY(enqueue_iteration(s, a, k, omega, ck > 0, check, checks));
-        ++ck;
-        if (a.d_trace) CLH_TRY(enqueue_trace(s, a, k));
-        if (check) {
-            CLH_HIP(hipMemcpyAsync(s->h_hist + 2 * checks, s->hist.ptr + 2 * checks,
-                                   2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-            CLH_HIP(hipStreamSynchronize(s->stream));
-            const double r2 = s->h_hist[2 * checks], b2 = s->h_hist[2 * checks + 1];
-            ++checks;
-            if (!std::isfinite(r2)) return fail(CLH_ERR_NUMERIC, "non-finite residual");
-            if (r2 <= tol2 * b2) {
-                s->converged = 1;
-                ++k;
-                break;
-            }
-            if (c.use_chebyshev && last_r2 >= 0.0 && r2 > last_r2) ck = 0;
-            last_r2 = r2;
-        }
+    // PD mode with a tolerance: a residual test every check_every iterations
+    // (and at the last), evaluated and acted on by the device (FrameCtl). The
+    // host enqueues the frame in chunks of check_every iterations, each a
+    // replayed hipGraph, one chunk ahead of the control snapshot it reads;
+    // once the device has stopped, the chunk in flight returns at once.
+    const int K = c.max_iterations, P = std::max(1, c.check_every);
+    const bool graphs = c.use_graph && !a.d_trace;
+    if (!same_args(s->chunk_args, a) || s->chunk_len != P) {
+        drop_chunk_graphs(s);
+        s->chunk_args = a;
+        s->chunk_len = P;
     }
-    s->last_iters = k;
+    CLH_TRY(enqueue_prologue(s, a));
+    hipLaunchKernelGGL(k_ctl_frame_init, dim3(1), dim3(64), 0, s->stream, s->ctl.ptr, c.rho,
+                       c.use_chebyshev ? 1 : 0, c.cheb_delay,
+                       (double)c.tolerance * (double)c.tolerance);
+    CLH_HIP(hipGetLastError());
+    auto enqueue_chunk = [&](int k0, int k1) -> int {
+        for (int k = k0; k < k1; ++k) {
+            CLH_TRY(enqueue_iteration(s, a, k, 1.0f, false, is_check_iteration(c, k), 0, true));
+            if (a.d_trace) CLH_TRY(enqueue_trace(s, a, k));
+        }
+        return CLH_OK;
+    };
+    auto issue = [&](int ci) -> int {
+        const int k0 = ci * P, k1 = std::min(K, k0 + P);
+        if (graphs && k1 - k0 == P) {
+            hipGraphExec_t& ex = s->chunk_exec[k0 % 6];
+            if (!ex) {
+                CLH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+                const int st = enqueue_chunk(k0, k1);
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(s->stream, &g);
+                if (st != CLH_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return st;
+                }
+                CLH_HIP(e);
+                const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                CLH_HIP(ei);
+            }
+            CLH_HIP(hipGraphLaunch(ex, s->stream));
+        } else {
+            CLH_TRY(enqueue_chunk(k0, k1));
+        }
+        CLH_HIP(hipMemcpyAsync(&s->h_ctl[ci & 1], s->ctl.ptr, sizeof(FrameCtl),
+                               hipMemcpyDeviceToHost, s->stream));
+        CLH_HIP(hipEventRecord(s->ev_chunk[ci & 1], s->stream));
+        return CLH_OK;
+    };
+    const int nchunks = (K + P - 1) / P;
+    int iters = K, checks = 0;
+    s->converged = 0;
+    bool nonfinite = false;
+    if (nchunks > 0) CLH_TRY(issue(0));
+    if (nchunks > 1) CLH_TRY(issue(1));
+    for (int ci = 0; ci < nchunks; ++ci) {
+        CLH_HIP(hipEventSynchronize(s->ev_chunk[ci & 1]));
+        const FrameCtl h = s->h_ctl[ci & 1];
+        checks = h.checks;
+        if (h.stop) {
+            iters = h.iters;
+            s->converged = h.converged;
+            nonfinite = h.nonfinite != 0;
+            break;
+        }
+        if (ci + 2 < nchunks) CLH_TRY(issue(ci + 2));
+    }
+    s->last_iters = iters;
     s->last_checks = checks;
-    return enqueue_epilogue(s, a, k);
+    if (nonfinite) return fail(CLH_ERR_NUMERIC, "non-finite residual");
+    return enqueue_epilogue(s, a, iters);
 }
 
 }  // namespace
@@ -888,8 +993,10 @@ extern "C" int clh_solver_destroy(clh_solver* s)
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
-    for (hipEvent_t e : {s->ev_in, s->ev_out, s->ev_t0, s->ev_t1})
+    drop_chunk_graphs(s);
+    for (hipEvent_t e : {s->ev_in, s->ev_out, s->ev_t0, s->ev_t1, s->ev_chunk[0], s->ev_chunk[1]})
         if (e) (void)hipEventDestroy(e);
+    if (s->h_ctl) (void)hipHostFree(s->h_ctl);
     if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
     if (s->h_hist) (void)hipHostFree(s->h_hist);
     fused_plan_release(s->fused);
@@ -903,6 +1010,7 @@ extern "C" int clh_solver_reset(clh_solver* s)
     CLH_HIP(hipStreamSynchronize(s->stream));
     s->plan_ready = false;
     s->graph_valid = false;
+    drop_chunk_graphs(s);
     s->pins_checked_ptr = nullptr;
     s->pins_checked_n = -1;
     s->frames = 0;

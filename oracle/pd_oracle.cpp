@@ -523,9 +523,11 @@ int step(const or_mesh* mp, const or_params* pp, R* pos, R* vel, const R* init, 
         const double tol2 = (double)p.tolerance * (double)p.tolerance;
         const int every = std::max(1, p.check_every);
         int ck = 0;               // iterations since the last Chebyshev (re)start
+        int restarts = 0;
         double last_r2 = -1.0;    // residual at the previous check
+        or_params q = p;          // rho is lowered after every third restart
         for (int k = 0; k < p.iterations; ++k) {
-            omf = cheb_omega<float>(p, ck, omf);
+            omf = cheb_omega<float>(q, ck, omf);
             const double omega = omf;
             const bool blend = p.use_chebyshev && ck > 0 && omf != 1.0f;
             const double* x = db[k % 3].data();
@@ -545,8 +547,12 @@ int step(const or_mesh* mp, const or_params* pp, R* pos, R* vel, const R* init, 
             ++iters;
             if (p.tolerance > 0 && ((k + 1) % every == 0)) {
                 if (r2 <= tol2 * b2) break;
-                // Chebyshev restart when the residual grew since the last test
-                if (p.use_chebyshev && last_r2 >= 0.0 && r2 > last_r2) ck = 0;
+                // Chebyshev restart when the residual grew since the last
+                // test; every third restart halves 1 - rho
+                if (p.use_chebyshev && last_r2 >= 0.0 && r2 > last_r2) {
+                    ck = 0;
+                    if (++restarts % 3 == 0) q.rho = 1.0f - 2.0f * (1.0f - q.rho);
+                }
                 last_r2 = r2;
             }
         }

@@ -261,6 +261,47 @@ def test_tolerance_mode_reports_fp64_residual():
     np.testing.assert_allclose(hist[:5], np.sqrt(rs[:5, 0] / rs[:5, 1]), rtol=1e-3)
 
 
+@pytest.mark.parametrize("variant", [ch.KERNEL_UNFUSED, ch.KERNEL_FUSED])
+def test_tolerance_frames_graph_chunks_match_direct(variant):
+    """Frames run to a tolerance are enqueued in chunks of check_every
+    iterations (replayed hipGraphs, or direct launches) under device-side
+    control; both give the same iterations, residual history and state, bit
+    for bit, and match the oracle's stopping iteration (restart rule
+    included)."""
+    s, arr = make(CFG1)
+    pos, vel = s.get_state()
+    out = []
+    for use_graph in (0, 1):
+        s.set_state(pos, vel)
+        sol = solver_for(s, **pd_cfg(600, CFG1["dt"], tolerance=1e-6, check_every=7,
+                                     use_graph=use_graph, kernel_variant=variant))
+        res = []
+        for _ in range(3):
+            sol.step(s.step_args())
+            st = sol.stats()
+            res.append((st.iterations, st.converged, tuple(sol.residual_history())))
+        out.append((res, s.get_state()))
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1][0], out[1][1][0])
+    np.testing.assert_array_equal(out[0][1][1], out[1][1][1])
+    # the FP32 oracle with the same check period, restart rule and stopping
+    # test: same stopping iteration and positions, frame after frame
+    m = oracle_mesh(arr)
+    p = po.params(mode=po.PD, iterations=600, gravity=-9.81, dt=CFG1["dt"], rho=0.99,
+                  cheb_delay=1, tolerance=1e-6, check_every=7)
+    x, v = pos.copy(), vel.copy()
+    L = extent(pos)
+    s.set_state(pos, vel)
+    sol = solver_for(s, **pd_cfg(600, CFG1["dt"], tolerance=1e-6, check_every=7,
+                                 kernel_variant=variant))
+    for f in range(3):
+        x, v, _, _, it = po.step(m, p, x, v, "f32", resid=True)
+        sol.step(s.step_args())
+        st = sol.stats()
+        assert abs(it - st.iterations) <= 7, (f, it, st.iterations)
+        assert rel_err(s.get_state()[0], x, L) < 1e-5, f
+
+
 def test_invalid_indices_rejected():
     s, arr = make(CFG1)
     bad = arr["tris"].copy()
