@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-1024", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -178,6 +179,44 @@ def main():
             f"({100 * bw3 / HBM_PEAK_GBS:.1f}% of peak); {1e3 / ms3:.1f} frames/s at 30 iterations")
         del sol3, s3
 
+    # ---- 512 batched cloths of uneven size, one launch per iteration -----------
+    batch = None
+    if not args.no_batch:
+        from helpers import cfg4_specs
+        t0 = time.time()
+        specs = cfg4_specs()
+        sb = build_scene(dict(specs=specs))
+        ab = sb.step_args()
+        log(f"batch: {sb.C} cloths, N={sb.N} T={sb.T} E={sb.E} built in {time.time() - t0:.1f} s")
+        solb = ch.Solver(ch.config(max_iterations=15, mode=ch.MODE_PD, gravity=-9.81,
+                                   dt=specs[0]["dt"], rho=0.99, cheb_delay=1), sb.N, sb.T, sb.E,
+                         cloth_offsets=sb.vert_off)
+        t0 = time.time()
+        rep = solb.autotune(ab, 10)
+        cands = [{"tile_verts": rep.tile_verts[i], "block_threads": rep.block_threads[i],
+                  "us_per_launch": rep.us_per_launch[i]} for i in range(rep.num_candidates)]
+        log(f"autotune ({time.time() - t0:.1f} s): " + ", ".join(
+            f"{c['tile_verts']}/{c['block_threads']}: {c['us_per_launch']:.0f} us" for c in cands))
+        for _ in range(2):
+            solb.step(ab)
+        msb = solb.run_frames(ab, 10) / 10
+        tb = solb.time_iteration_kernel(ab, 20)
+        bb = algorithmic_bytes(sb.N, sb.T, sb.E)
+        bwb = bb / (tb * 1e-3) / 1e9
+        pib = solb.plan_info()
+        batch = {"workload": "512 cloths, sides round(16*16^U) in [16, 256], stretch log-U "
+                             "[1e2, 1e4], bend log-U [1e-3, 1e-1], 1-4 random boundary pins, "
+                             "15 iterations/frame, graph-captured frame",
+                 "cloths": sb.C, "N": sb.N, "T": sb.T, "E": sb.E,
+                 "frames_per_s": 1e3 / msb, "cloth_frames_per_s": sb.C * 1e3 / msb,
+                 "ms_per_frame": msb, "us_per_iteration_launch": tb * 1e3,
+                 "bytes_per_launch": bb, "achieved": bwb, "frac": bwb / HBM_PEAK_GBS,
+                 "autotune": {"best": cands[rep.best], "candidates": cands},
+                 "padded_verts": int(pib.padded_verts)}
+        log(f"batch: {1e3 / msb:.1f} frames/s ({sb.C * 1e3 / msb:.0f} cloth-frames/s), iteration "
+            f"{tb * 1e3:.0f} us = {bwb:.0f} GB/s ({100 * bwb / HBM_PEAK_GBS:.1f}% of peak)")
+        del solb, sb
+
     # ---- CPU baseline: the oracle's FP32 PD on the host cores --------------------
     cpu = None
     if not args.no_cpu:
@@ -231,7 +270,8 @@ def main():
                      "frac": bw / HBM_PEAK_GBS, "traffic": profiled_traffic(TRAFFIC_FILE),
                      "kernel": "k_fused_iter (one PD iteration, 256x256)",
                      "us_per_launch": t_iter * 1e3, "bytes_per_launch": b_iter,
-                     "copy_probe_GBs": copy_gbs, "mesh_1024": roof_1024},
+                     "copy_probe_GBs": copy_gbs, "mesh_1024": roof_1024,
+                     "batched_512": batch},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

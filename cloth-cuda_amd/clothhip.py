@@ -56,7 +56,8 @@ STEP_FIELDS = ["d_pos", "d_vel", "d_mass", "d_tris", "d_Dm_inv", "d_rest_area", 
 
 class StepArgs(C.Structure):
     _fields_ = ([(f, _vp) for f in STEP_FIELDS] +
-                [("num_pinned", C.c_int), ("d_init_guess", _vp), ("d_trace", _vp)])
+                [("num_pinned", C.c_int), ("d_init_guess", _vp), ("d_trace", _vp),
+                 ("d_cloth_accel", _vp)])
 
 
 class StepStats(C.Structure):
@@ -68,7 +69,13 @@ class PlanInfo(C.Structure):
     _fields_ = [("variant", C.c_int), ("num_tiles", C.c_int), ("tile_verts", C.c_int),
                 ("halo_tri_factor", C.c_double), ("halo_bend_factor", C.c_double),
                 ("bytes_per_iter", C.c_double), ("lds_bytes", C.c_double),
-                ("block_threads", C.c_int)]
+                ("block_threads", C.c_int), ("num_cloths", C.c_int), ("padded_verts", C.c_int)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("num_candidates", C.c_int), ("tile_verts", C.c_int * 8),
+                ("block_threads", C.c_int * 8), ("us_per_launch", C.c_float * 8),
+                ("best", C.c_int), ("from_cache", C.c_int), ("key", C.c_ulonglong)]
 
 
 class SceneSpec(C.Structure):
@@ -95,6 +102,7 @@ CLH_SYMBOLS = [
     "clh_solver_step", "clh_solver_get_stats", "clh_solver_residual_history",
     "clh_solver_set_stream", "clh_solver_run_frames", "clh_solver_time_iteration_kernel",
     "clh_solver_plan_info", "clh_measure_copy", "clh_fused_plan_check",
+    "clh_batch_create", "clh_batch_cloth_residuals", "clh_solver_autotune",
 ]
 
 _hip = None
@@ -132,7 +140,11 @@ def _load():
     hip.clh_solver_plan_info.argtypes = [_vp, C.POINTER(PlanInfo)]
     hip.clh_measure_copy.argtypes = [_vp, _vp, C.c_size_t, C.c_int, _fp]
     hip.clh_fused_plan_check.argtypes = [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int,
-                                         C.c_int, C.POINTER(PlanInfo)]
+                                         C.c_int, C.c_int, _vp, C.POINTER(PlanInfo)]
+    hip.clh_batch_create.argtypes = [C.POINTER(Config), C.POINTER(MeshDesc), C.c_int, _vp,
+                                     C.POINTER(_vp)]
+    hip.clh_batch_cloth_residuals.argtypes = [_vp, C.POINTER(C.c_double), C.c_int, _ip]
+    hip.clh_solver_autotune.argtypes = [_vp, C.POINTER(StepArgs), C.c_int, C.POINTER(TuneReport)]
     hip.clh_get_device_info.argtypes = [C.c_int, C.POINTER(DeviceInfo)]
     hip.clh_device_count.argtypes = [_ip]
 
@@ -142,6 +154,12 @@ def _load():
     core.cc_scene_build.restype = _vp
     core.cc_scene_build_obj.argtypes = [C.c_char_p, C.POINTER(SceneSpec)]
     core.cc_scene_build_obj.restype = _vp
+    core.cc_batch_build.argtypes = [C.POINTER(SceneSpec), C.c_int]
+    core.cc_batch_build.restype = _vp
+    core.cc_scene_num_cloths.argtypes = [_vp]
+    core.cc_scene_num_cloths.restype = C.c_int
+    core.cc_scene_cloth_offsets.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    core.cc_scene_cloth_offsets.restype = C.c_int
     core.cc_scene_destroy.argtypes = [_vp]
     core.cc_scene_destroy.restype = None
     core.cc_scene_counts.argtypes = [_vp, _ip, _ip, _ip, _ip]
@@ -275,16 +293,26 @@ ARRAY_SPECS = {
 
 
 class Scene:
-    """A cloth built through the C++ API (mesh, topology, constraints, pins)."""
+    """A cloth built through the C++ API (mesh, topology, constraints, pins);
+    with `specs`, a batch of cloths concatenated into one scene."""
 
-    def __init__(self, obj_path=None, **kw):
-        self.spec = scene_spec(**kw)
-        if obj_path is not None:
+    def __init__(self, obj_path=None, specs=None, **kw):
+        if specs is not None:
+            arr = (SceneSpec * len(specs))(*[scene_spec(**sp) for sp in specs])
+            self.spec = arr[0]
+            self.h = core().cc_batch_build(arr, len(specs))
+        elif obj_path is not None:
+            self.spec = scene_spec(**kw)
             self.h = core().cc_scene_build_obj(str(obj_path).encode(), C.byref(self.spec))
         else:
+            self.spec = scene_spec(**kw)
             self.h = core().cc_scene_build(C.byref(self.spec))
         if not self.h:
             raise ClothHipError("cc_scene_build failed")
+        self.C = core().cc_scene_num_cloths(self.h)
+        offs = [np.zeros(self.C + 1, np.int32) for _ in range(4)]
+        core().cc_scene_cloth_offsets(self.h, *[_ptr(o) for o in offs])
+        self.vert_off, self.tri_off, self.bend_off, self.pin_off = offs
         n, t, e, p = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         core().cc_scene_counts(self.h, C.byref(n), C.byref(t), C.byref(e), C.byref(p))
         self.N, self.T, self.E, self.P = n.value, t.value, e.value, p.value
@@ -364,15 +392,35 @@ def config(**kw) -> Config:
 
 
 class Solver:
-    """clh_solver through the C ABI."""
+    """clh_solver through the C ABI; with cloth_offsets, a batched solver
+    (clh_batch_create)."""
 
-    def __init__(self, cfg: Config, N: int, T: int, E: int, d_tris=None, d_quads=None):
+    def __init__(self, cfg: Config, N: int, T: int, E: int, d_tris=None, d_quads=None,
+                 cloth_offsets=None):
         desc = MeshDesc(N, T, E, d_tris, d_quads)
         h = _vp()
-        check(hip().clh_solver_create(C.byref(cfg), C.byref(desc), C.byref(h)),
-              "clh_solver_create")
+        if cloth_offsets is not None:
+            offs = np.ascontiguousarray(cloth_offsets, np.int32)
+            check(hip().clh_batch_create(C.byref(cfg), C.byref(desc), len(offs) - 1, _ptr(offs),
+                                         C.byref(h)), "clh_batch_create")
+        else:
+            check(hip().clh_solver_create(C.byref(cfg), C.byref(desc), C.byref(h)),
+                  "clh_solver_create")
         self.h = h.value
         self.cfg = cfg
+
+    def cloth_residuals(self, max_count=1 << 16) -> np.ndarray:
+        out = np.zeros(max_count, np.float64)
+        n = C.c_int(0)
+        check(hip().clh_batch_cloth_residuals(self.h, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                              max_count, C.byref(n)), "cloth_residuals")
+        return out[:min(n.value, max_count)]
+
+    def autotune(self, args: StepArgs, launches: int = 20) -> TuneReport:
+        rep = TuneReport()
+        check(hip().clh_solver_autotune(self.h, C.byref(args), launches, C.byref(rep)),
+              "clh_solver_autotune")
+        return rep
 
     def step(self, args: StepArgs):
         check(hip().clh_solver_step(self.h, C.byref(args)), "clh_solver_step")
@@ -466,15 +514,19 @@ class PDSolver:
             pass
 
 
-def fused_plan_check(N, tris, quads, pos, tile_verts=128, mode=MODE_PD) -> PlanInfo:
+def fused_plan_check(N, tris, quads, pos, tile_verts=128, mode=MODE_PD,
+                     cloth_offsets=None) -> PlanInfo:
     """Builds and validates the fused tile plan on the host (no device)."""
     tris = np.ascontiguousarray(tris, np.int32)
     quads = None if quads is None or len(quads) == 0 else np.ascontiguousarray(quads, np.int32)
     pos = np.ascontiguousarray(pos, np.float32)
+    offs = None if cloth_offsets is None else np.ascontiguousarray(cloth_offsets, np.int32)
     info = PlanInfo()
     check(hip().clh_fused_plan_check(N, len(tris), 0 if quads is None else len(quads),
                                      _ptr(tris), None if quads is None else _ptr(quads),
-                                     _ptr(pos), tile_verts, mode, C.byref(info)),
+                                     _ptr(pos), tile_verts, mode,
+                                     0 if offs is None else len(offs) - 1,
+                                     None if offs is None else _ptr(offs), C.byref(info)),
           "clh_fused_plan_check")
     return info
 

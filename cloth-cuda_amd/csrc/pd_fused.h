@@ -37,7 +37,9 @@ struct FusedIterArgs {
 };
 
 struct FusedPlan {
-    int N = 0;
+    int N = 0;             // tile-order length: every cloth's run padded to whole tiles
+    int num_verts = 0;     // the caller's vertex count (original order)
+    int num_cloths = 1;
     int num_tiles = 0;
     int tile_verts = 0;
     int threads = 256;                                     // workgroup size (256 or 512)
@@ -56,15 +58,21 @@ struct FusedPlan {
     DevBuf<float2> hin_c;                     // (n_rest, h^2 k)
     DevBuf<unsigned> gl_words;                // gather entries, two u16 per word
     DevBuf<unsigned> gl_meta;                 // [N] start in tile | n_tri << 16 | n_all << 24
+    DevBuf<int>   tile_cloth;                 // [num_tiles] cloth owning each tile
+    DevBuf<int>   cloth_tiles;                // [num_cloths + 1] first tile of each cloth
 
     // packed-constant provenance (repack when any pointer changes)
     const void* packed_src[8] = {};
     float packed_h2 = -1.0f;
 };
 
+// seg: cloth vertex offsets [C+1] (cloth c owns original vertices
+// [seg[c], seg[c+1])), or empty for one cloth. Every element must lie in one
+// cloth; each cloth's vertices are Morton-ordered on their own and padded to
+// whole tiles with inert vertices (no elements, zero mass and diagonal).
 int  fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
                       const std::vector<int>& quads, const std::vector<float>& pos,
-                      int tile_verts);
+                      int tile_verts, const std::vector<int>& seg);
 int  fused_pack_constants(FusedPlan& p, const clh_step_args& a, float h2, hipStream_t stream);
 int  fused_max_blocks(const FusedPlan& p);
 size_t fused_lds_bytes(const FusedPlan& p, int mode, bool check);
@@ -78,8 +86,11 @@ void fused_plan_release(FusedPlan& p);
 
 // Frame prologue / epilogue in tile order (gather from and scatter to the
 // caller's original-order arrays).
+// h2a = h^2 a for the frame's uniform acceleration a; with a.d_cloth_accel
+// the prediction uses h^2 (a + a_c) of each vertex's cloth c instead.
 int fused_prologue(const FusedPlan& p, const clh_step_args& a, float* xprev, float* y, void* x0,
-                   int4* vinfo, float dt, const float h2a[3], int disp, hipStream_t stream);
+                   int4* vinfo, float dt, float h2, const float acc[3], const float h2a[3],
+                   int disp, hipStream_t stream);
 int fused_epilogue(const FusedPlan& p, const clh_step_args& a, const void* xf, const float* xprev,
                    const int4* vinfo, float scale, int disp, hipStream_t stream);
 int fused_trace(const FusedPlan& p, const void* it, const float* xprev, float* out, int disp,

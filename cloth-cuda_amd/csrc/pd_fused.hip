@@ -32,12 +32,12 @@ uint64_t spread21(uint64_t v)
     return v;
 }
 
-// Vertex order: 3-D Morton code of the positions (21 bits per axis over the
-// bounding cube), ties by index.
-std::vector<int> morton_order(int N, const std::vector<float>& pos)
+// Vertex order of the original vertices [beg, end): 3-D Morton code of the
+// positions (21 bits per axis over their bounding cube), ties by index.
+std::vector<int> morton_order(const std::vector<float>& pos, int beg, int end)
 {
     float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-    for (int v = 0; v < N; ++v)
+    for (int v = beg; v < end; ++v)
         for (int k = 0; k < 3; ++k) {
             lo[k] = std::min(lo[k], pos[(size_t)v * 3 + k]);
             hi[k] = std::max(hi[k], pos[(size_t)v * 3 + k]);
@@ -46,20 +46,21 @@ std::vector<int> morton_order(int N, const std::vector<float>& pos)
     // cloth orders by its two in-plane axes
     double ext = 0.0;
     for (int k = 0; k < 3; ++k) ext = std::max(ext, (double)hi[k] - (double)lo[k]);
-    std::vector<uint64_t> key((size_t)N);
-    for (int v = 0; v < N; ++v) {
+    const int n = end - beg;
+    std::vector<uint64_t> key((size_t)n);
+    for (int v = beg; v < end; ++v) {
         uint64_t q[3];
         for (int k = 0; k < 3; ++k) {
             double f = ext > 0 ? ((double)pos[(size_t)v * 3 + k] - lo[k]) / ext : 0.0;
             f = std::min(std::max(f, 0.0), 1.0);
             q[k] = (uint64_t)(f * 2097151.0);
         }
-        key[(size_t)v] = spread21(q[0]) | (spread21(q[1]) << 1) | (spread21(q[2]) << 2);
+        key[(size_t)(v - beg)] = spread21(q[0]) | (spread21(q[1]) << 1) | (spread21(q[2]) << 2);
     }
-    std::vector<int> order((size_t)N);
-    std::iota(order.begin(), order.end(), 0);
+    std::vector<int> order((size_t)n);
+    std::iota(order.begin(), order.end(), beg);
     std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return key[(size_t)a] < key[(size_t)b]; });
+                     [&](int a, int b) { return key[(size_t)(a - beg)] < key[(size_t)(b - beg)]; });
     return order;
 }
 
@@ -106,7 +107,7 @@ void build_slots(const std::vector<int>& elems, int arity, int N, std::vector<in
 
 struct HostPlan {
     FusedPlan meta;  // sizes only; device buffers unused
-    std::vector<int> perm, inv, info, halo, tri_gid, hin_gid;
+    std::vector<int> perm, inv, info, halo, tri_gid, hin_gid, tile_cloth, cloth_tiles;
     std::vector<uint2> tri_idx, hin_idx;
     std::vector<unsigned short> gl_idx;
     std::vector<unsigned> gl_meta;
@@ -435,17 +436,36 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
 
 // ---- tile-order frame kernels ------------------------------------------------
 
+// Tile-order padding (perm < 0) gets a zero state and an inert record: no
+// gather list, zero mass and diagonal, so its iterate stays 0.
 __global__ void __launch_bounds__(256)
 k_fused_prologue(const int* __restrict__ perm, const float* __restrict__ pos,
                  const float* __restrict__ vel, const float* __restrict__ guess,
                  const float* __restrict__ mass_in, const float* __restrict__ diag_in,
-                 const unsigned* __restrict__ gl_meta, float* __restrict__ xprev,
+                 const unsigned* __restrict__ gl_meta, const int* __restrict__ tile_cloth,
+                 const float* __restrict__ cloth_accel, float* __restrict__ xprev,
                  float* __restrict__ y, void* __restrict__ x0, int4* __restrict__ vinfo, int N,
-                 float dt, float h2a0, float h2a1, float h2a2, int disp)
+                 int tile_verts, float dt, float h2, float acc0, float acc1, float acc2,
+                 float h2a0, float h2a1, float h2a2, int disp)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int o = perm[i];
+    if (o < 0) {
+        st3(xprev, i, mk(0.0f, 0.0f, 0.0f));
+        st3(y, i, mk(0.0f, 0.0f, 0.0f));
+        if (disp) st3d(static_cast<double*>(x0), i, d3{0.0, 0.0, 0.0});
+        else      st3(static_cast<float*>(x0), i, mk(0.0f, 0.0f, 0.0f));
+        vinfo[i] = make_int4(0, 0, -1, 0);
+        return;
+    }
+    if (cloth_accel) {
+        // per-cloth acceleration on top of the frame's uniform one
+        const f3 ca = ld3(cloth_accel, tile_cloth[i / tile_verts]);
+        h2a0 = h2 * (acc0 + ca.x);
+        h2a1 = h2 * (acc1 + ca.y);
+        h2a2 = h2 * (acc2 + ca.z);
+    }
     const f3 x = ld3(pos, o), u = ld3(vel, o);
     st3(xprev, i, x);
     const f3 dy = mk(dt * u.x + h2a0, dt * u.y + h2a1, dt * u.z + h2a2);
@@ -483,6 +503,7 @@ k_fused_epilogue(const int* __restrict__ perm, const void* __restrict__ xf,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int o = perm[i];
+    if (o < 0) return;  // padding
     const int slot = vinfo[i].z;
     if (slot >= 0) {
         st3(pos, o, ld3(target, slot));
@@ -506,7 +527,7 @@ k_fused_trace(const int* __restrict__ perm, const void* __restrict__ it,
               const float* __restrict__ xprev, float* __restrict__ out, int N, int disp)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+    if (i >= N || perm[i] < 0) return;
     st3(out, perm[i],
         disp ? to_f(to_d(ld3(xprev, i)) + ld3d(static_cast<const double*>(it), i))
              : ld3(static_cast<const float*>(it), i));
@@ -563,34 +584,67 @@ unsigned blocks_for(int n) { return (unsigned)((n + 255) / 256); }
 }  // namespace
 
 int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
-                    const std::vector<int>& quads, const std::vector<float>& pos, int tile_verts)
+                    const std::vector<int>& quads, const std::vector<float>& pos, int tile_verts,
+                    const std::vector<int>& seg_in)
 {
     if (N <= 0) return fail(CLH_ERR_UNSUPPORTED, "fused plan: empty mesh");
     const int T = (int)(tris.size() / 3), E = (int)(quads.size() / 4);
     const int V = std::max(32, std::min(tile_verts, kFB));  // one own vertex per thread
 
-    std::vector<int> perm = morton_order(N, pos);
+    // cloths: contiguous ranges of original vertices; elements stay inside one
+    const std::vector<int> seg = seg_in.empty() ? std::vector<int>{0, N} : seg_in;
+    const int C = (int)seg.size() - 1;
+    if (C < 1 || seg[0] != 0 || seg[(size_t)C] != N)
+        return fail(CLH_ERR_INVALID, "fused plan: cloth offsets must run from 0 to num_verts");
+    std::vector<int> vcloth((size_t)N);
+    for (int c = 0; c < C; ++c) {
+        if (seg[(size_t)c + 1] <= seg[(size_t)c])
+            return fail(CLH_ERR_INVALID, "fused plan: every cloth needs at least one vertex");
+        for (int v = seg[(size_t)c]; v < seg[(size_t)c + 1]; ++v) vcloth[(size_t)v] = c;
+    }
+    for (int e = 0; e < T; ++e)
+        for (int k = 1; k < 3; ++k)
+            if (vcloth[(size_t)tris[(size_t)e * 3 + k]] != vcloth[(size_t)tris[(size_t)e * 3]])
+                return fail(CLH_ERR_INVALID, "fused plan: a triangle spans two cloths");
+    for (int e = 0; e < E; ++e)
+        for (int k = 1; k < 4; ++k)
+            if (vcloth[(size_t)quads[(size_t)e * 4 + k]] != vcloth[(size_t)quads[(size_t)e * 4]])
+                return fail(CLH_ERR_INVALID, "fused plan: a hinge spans two cloths");
+
+    // tile order: each cloth's vertices in Morton order, padded with inert
+    // vertices (perm = -1) to a whole number of tiles
+    std::vector<int> perm, tile_cloth, cloth_tiles((size_t)C + 1, 0);
+    for (int c = 0; c < C; ++c) {
+        const std::vector<int> o = morton_order(pos, seg[(size_t)c], seg[(size_t)c + 1]);
+        perm.insert(perm.end(), o.begin(), o.end());
+        while (perm.size() % (size_t)V) perm.push_back(-1);
+        cloth_tiles[(size_t)c + 1] = (int)(perm.size() / (size_t)V);
+        while ((int)tile_cloth.size() < cloth_tiles[(size_t)c + 1]) tile_cloth.push_back(c);
+    }
+    const int Np = (int)perm.size();
     std::vector<int> inv((size_t)N);
-    for (int i = 0; i < N; ++i) inv[(size_t)perm[(size_t)i]] = i;
+    for (int i = 0; i < Np; ++i)
+        if (perm[(size_t)i] >= 0) inv[(size_t)perm[(size_t)i]] = i;
 
     std::vector<int> vt_ptr, vt_ids, ve_ptr, ve_ids;
     incidence(tris, 3, N, vt_ptr, vt_ids);
     incidence(quads, 4, N, ve_ptr, ve_ids);
 
-    const int ntiles = (N + V - 1) / V;
+    const int ntiles = Np / V;
     std::vector<int> info((size_t)ntiles * kTileInfo), halo_list, halo_beg((size_t)ntiles + 1, 0);
     std::vector<int> tri_gid, hin_gid;
     std::vector<uint2> tri_idx, hin_idx;
     std::vector<unsigned short> gl_idx;
-    std::vector<unsigned> gl_meta((size_t)N, 0);
-    std::vector<int> loc((size_t)N, -1);  // tile-order id -> local id in the current tile
+    std::vector<unsigned> gl_meta((size_t)Np, 0);
+    std::vector<int> loc((size_t)Np, -1);  // tile-order id -> local id in the current tile
     int lmax = 0, tmax = 0, hmax = 0, gmax = 0, hstride = 1;
 
     for (int t = 0; t < ntiles; ++t) {
-        const int vbeg = t * V, nown = std::min(N, vbeg + V) - vbeg;
+        const int vbeg = t * V, nown = V;
         std::vector<int> tl, hl, touched;
         for (int i = 0; i < nown; ++i) {
             const int o = perm[(size_t)vbeg + i];
+            if (o < 0) continue;  // padding
             for (int j = vt_ptr[(size_t)o]; j < vt_ptr[(size_t)o + 1]; ++j) tl.push_back(vt_ids[(size_t)j]);
             for (int j = ve_ptr[(size_t)o]; j < ve_ptr[(size_t)o + 1]; ++j) hl.push_back(ve_ids[(size_t)j]);
         }
@@ -637,6 +691,7 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
         const size_t gbeg = gl_idx.size();
         for (int i = 0; i < nown; ++i) {
             const int g = vbeg + i, o = perm[(size_t)g];
+            if (o < 0) continue;  // padding: no gather (meta 0)
             const size_t start = gl_idx.size() - gbeg;
             int nt = 0;
             for (int j = vt_ptr[(size_t)o]; j < vt_ptr[(size_t)o + 1]; ++j) {
@@ -684,7 +739,9 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
     }
 
     FusedPlan& p = h.meta;
-    p.N = N;
+    p.N = Np;
+    p.num_verts = N;
+    p.num_cloths = C;
     p.num_tiles = ntiles;
     p.tile_verts = V;
     p.hstride = hstride;
@@ -701,7 +758,7 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
     // vertex record; per tile its descriptor and padded halo id block; per
     // halo vertex its iterate and base; packed triangle (28 B) and hinge
     // (32 B) records; gather entries (2 B)
-    p.bytes_per_iter = (double)N * (3 * 24 + 2 * 12 + 16) + (double)halo_list.size() * 36 +
+    p.bytes_per_iter = (double)Np * (3 * 24 + 2 * 12 + 16) + (double)halo_list.size() * 36 +
                        (double)ntiles * (kTileInfo + hstride) * 4 +
                        (double)tri_gid.size() * 28 + (double)hin_gid.size() * 32 +
                        (double)gl_idx.size() * 2;
@@ -718,34 +775,62 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
     h.hin_idx.swap(hin_idx);
     h.gl_idx.swap(gl_idx);
     h.gl_meta.swap(gl_meta);
+    h.tile_cloth.swap(tile_cloth);
+    h.cloth_tiles.swap(cloth_tiles);
     return CLH_OK;
 }
 
-// Checks every index of a host plan against the mesh: own ranges, local ids
-// inside the tile's table, halo ids outside the own range, each local slot
-// naming the element's vertex, and each own vertex's gather list equal to
-// its incident (element, corner) slots in ascending order (the order of the
-// unfused pipeline and the oracle) and inside the tile's gather segment.
-int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::vector<int>& quads)
+// Checks every index of a host plan against the mesh: the tile order is a
+// permutation of the vertices plus padding, each cloth's vertices fill its
+// own whole tiles, local ids lie inside the tile's table, halo ids lie
+// outside the own range and in the tile's cloth, each local slot names the
+// element's vertex, padding has no gather list, and each own vertex's gather
+// list equals its incident (element, corner) slots in ascending order (the
+// order of the unfused pipeline and the oracle) inside the tile's segment.
+int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::vector<int>& quads,
+                    const std::vector<int>& seg_in)
 {
     const FusedPlan& p = h.meta;
-    const int N = p.N, V = p.tile_verts;
-    if ((int)h.perm.size() != N || (int)h.inv.size() != N) return fail(CLH_ERR_INVALID, "plan: perm size");
-    for (int i = 0; i < N; ++i)
-        if (h.perm[(size_t)i] < 0 || h.perm[(size_t)i] >= N || h.inv[(size_t)h.perm[(size_t)i]] != i)
+    const int Np = p.N, N = p.num_verts, V = p.tile_verts, C = p.num_cloths;
+    const std::vector<int> seg = seg_in.empty() ? std::vector<int>{0, N} : seg_in;
+    if ((int)seg.size() != C + 1 || (int)h.cloth_tiles.size() != C + 1 ||
+        (int)h.tile_cloth.size() != p.num_tiles)
+        return fail(CLH_ERR_INVALID, "plan: cloth tables");
+    if ((int)h.perm.size() != Np || (int)h.inv.size() != N) return fail(CLH_ERR_INVALID, "plan: perm size");
+    if ((long long)p.num_tiles * V != Np) return fail(CLH_ERR_INVALID, "plan: tile count");
+    std::vector<int> seen((size_t)N, 0);
+    for (int i = 0; i < Np; ++i) {
+        const int o = h.perm[(size_t)i];
+        if (o < 0) continue;
+        if (o >= N || h.inv[(size_t)o] != i || seen[(size_t)o]++)
             return fail(CLH_ERR_INVALID, "plan: perm is not a permutation");
-    if ((long long)p.num_tiles * V < N || (long long)(p.num_tiles - 1) * V >= N)
-        return fail(CLH_ERR_INVALID, "plan: tile count");
+    }
+    for (int v = 0; v < N; ++v)
+        if (!seen[(size_t)v]) return fail(CLH_ERR_INVALID, "plan: vertex missing from the tile order");
+    auto cloth_of = [&](int o) {
+        return (int)(std::upper_bound(seg.begin(), seg.end(), o) - seg.begin()) - 1;
+    };
+    if (h.cloth_tiles[0] != 0 || h.cloth_tiles[(size_t)C] != p.num_tiles)
+        return fail(CLH_ERR_INVALID, "plan: cloth tile ranges");
+    for (int c = 0; c < C; ++c)
+        for (int t = h.cloth_tiles[(size_t)c]; t < h.cloth_tiles[(size_t)c + 1]; ++t)
+            if (h.tile_cloth[(size_t)t] != c) return fail(CLH_ERR_INVALID, "plan: tile cloth");
+    for (int i = 0; i < Np; ++i) {
+        const int o = h.perm[(size_t)i];
+        if (o >= 0 && cloth_of(o) != h.tile_cloth[(size_t)(i / V)])
+            return fail(CLH_ERR_INVALID, "plan: vertex in a tile of another cloth");
+        if (o < 0 && h.gl_meta[(size_t)i] != 0) return fail(CLH_ERR_INVALID, "plan: padding gathers");
+    }
     if (h.halo.size() != (size_t)p.num_tiles * p.hstride) return fail(CLH_ERR_INVALID, "plan: halo size");
     for (int v : h.halo)
-        if (v < 0 || v >= N) return fail(CLH_ERR_INVALID, "plan: halo id range");
+        if (v < 0 || v >= Np) return fail(CLH_ERR_INVALID, "plan: halo id range");
     std::vector<int> vt_ptr, vt_ids, ve_ptr, ve_ids;
     build_slots(tris, 3, N, vt_ptr, vt_ids);
     build_slots(quads, 4, N, ve_ptr, ve_ids);
     const int T = (int)(tris.size() / 3), E = (int)(quads.size() / 4);
     for (int t = 0; t < p.num_tiles; ++t) {
         const int* ti = &h.info[(size_t)t * kTileInfo];
-        const int vbeg = t * V, nown = std::min(V, N - vbeg);
+        const int vbeg = t * V, nown = V;
         const int nhalo = ti[0], tbeg = ti[1], ntri = ti[2], ebeg = ti[3], nhin = ti[4];
         const size_t gbeg = (size_t)ti[5] * 2;
         const int glen = ti[6];
@@ -762,6 +847,8 @@ int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::
         for (int l = nown; l < nl; ++l) {
             const int g = tile_id(l);
             if (g >= vbeg && g < vbeg + nown) return fail(CLH_ERR_INVALID, "plan: halo id");
+            if (h.perm[(size_t)g] < 0 || cloth_of(h.perm[(size_t)g]) != h.tile_cloth[(size_t)t])
+                return fail(CLH_ERR_INVALID, "plan: halo vertex outside the tile's cloth");
         }
         for (int j = 0; j < ntri; ++j) {
             const uint2 id = h.tri_idx[(size_t)tbeg + j];
@@ -783,6 +870,7 @@ int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::
         }
         for (int i = 0; i < nown; ++i) {
             const int g = vbeg + i, o = h.perm[(size_t)g];
+            if (o < 0) continue;  // padding (checked above)
             const unsigned m = h.gl_meta[(size_t)g];
             const int j0 = (int)(m & 0xFFFFu), nt = (int)((m >> 16) & 0xFFu), na = (int)(m >> 24);
             if (nt > na || j0 + na > glen) return fail(CLH_ERR_INVALID, "plan: gather meta");
@@ -807,13 +895,14 @@ int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::
 }
 
 int fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
-                     const std::vector<int>& quads, const std::vector<float>& pos, int tile_verts)
+                     const std::vector<int>& quads, const std::vector<float>& pos, int tile_verts,
+                     const std::vector<int>& seg)
 {
     fused_plan_release(p);
     HostPlan h;
-    CLH_TRY(host_plan_build(h, N, tris, quads, pos, tile_verts));
+    CLH_TRY(host_plan_build(h, N, tris, quads, pos, tile_verts, seg));
     const FusedPlan& m = h.meta;
-    p.N = m.N; p.num_tiles = m.num_tiles; p.tile_verts = m.tile_verts; p.hstride = m.hstride;
+    p.N = m.N; p.num_verts = m.num_verts; p.num_cloths = m.num_cloths; p.num_tiles = m.num_tiles; p.tile_verts = m.tile_verts; p.hstride = m.hstride;
     p.lmax = m.lmax; p.tmax = m.tmax; p.hmax = m.hmax; p.gmax = m.gmax; p.vmax = m.vmax;
     p.tri_factor = m.tri_factor; p.bend_factor = m.bend_factor; p.xcd_remap = m.xcd_remap;
     p.bytes_per_iter = m.bytes_per_iter;
@@ -833,6 +922,8 @@ int fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
     std::memcpy(words.data(), h.gl_idx.data(), words.size() * sizeof(unsigned));
     CLH_TRY(p.gl_words.upload(words));
     CLH_TRY(p.gl_meta.upload(h.gl_meta));
+    CLH_TRY(p.tile_cloth.upload(h.tile_cloth));
+    CLH_TRY(p.cloth_tiles.upload(h.cloth_tiles));
     std::memset(p.packed_src, 0, sizeof(p.packed_src));
     p.packed_h2 = -1.0f;
     return CLH_OK;
@@ -932,11 +1023,13 @@ int fused_enable_lds(int& bytes)
 }
 
 int fused_prologue(const FusedPlan& p, const clh_step_args& a, float* xprev, float* y, void* x0,
-                   int4* vinfo, float dt, const float h2a[3], int disp, hipStream_t stream)
+                   int4* vinfo, float dt, float h2, const float acc[3], const float h2a[3],
+                   int disp, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_fused_prologue, dim3(blocks_for(p.N)), dim3(256), 0, stream, p.perm.ptr,
                        a.d_pos, a.d_vel, a.d_init_guess, a.d_mass, a.d_jacobi_diag, p.gl_meta.ptr,
-                       xprev, y, x0, vinfo, p.N, dt, h2a[0], h2a[1], h2a[2], disp);
+                       p.tile_cloth.ptr, a.d_cloth_accel, xprev, y, x0, vinfo, p.N, p.tile_verts,
+                       dt, h2, acc[0], acc[1], acc[2], h2a[0], h2a[1], h2a[2], disp);
     if (a.num_pinned > 0)
         hipLaunchKernelGGL(k_fused_pins, dim3(blocks_for(a.num_pinned)), dim3(256), 0, stream,
                            p.inv_perm.ptr, a.d_pin_idx, a.d_pin_target, xprev, vinfo, x0,
@@ -963,20 +1056,23 @@ int fused_trace(const FusedPlan& p, const void* it, const float* xprev, float* o
 }  // namespace clh
 
 extern "C" int clh_fused_plan_check(int N, int T, int E, const int* tris, const int* quads,
-                                    const float* pos, int tile_verts, int mode,
-                                    clh_plan_info* out)
+                                    const float* pos, int tile_verts, int mode, int num_cloths,
+                                    const int* cloth_vert_offsets, clh_plan_info* out)
 {
     using namespace clh;
-    if (N <= 0 || T < 0 || E < 0 || !pos || (T > 0 && !tris) || (E > 0 && !quads))
+    if (N <= 0 || T < 0 || E < 0 || !pos || (T > 0 && !tris) || (E > 0 && !quads) ||
+        num_cloths < 0 || (num_cloths > 0 && !cloth_vert_offsets))
         return fail(CLH_ERR_INVALID, "clh_fused_plan_check: bad arguments");
     std::vector<int> t(tris, tris + (size_t)T * 3), q;
     if (E > 0) q.assign(quads, quads + (size_t)E * 4);
     for (int v : t) if (v < 0 || v >= N) return fail(CLH_ERR_INVALID, "triangle vertex index out of range");
     for (int v : q) if (v < 0 || v >= N) return fail(CLH_ERR_INVALID, "hinge vertex index out of range");
+    std::vector<int> seg;
+    if (num_cloths > 0) seg.assign(cloth_vert_offsets, cloth_vert_offsets + num_cloths + 1);
     std::vector<float> x(pos, pos + (size_t)N * 3);
     HostPlan h;
-    CLH_TRY(host_plan_build(h, N, t, q, x, tile_verts));
-    CLH_TRY(host_plan_check(h, t, q));
+    CLH_TRY(host_plan_build(h, N, t, q, x, tile_verts, seg));
+    CLH_TRY(host_plan_check(h, t, q, seg));
     if (out) {
         std::memset(out, 0, sizeof(*out));
         out->variant = CLH_KERNEL_FUSED;
@@ -990,6 +1086,8 @@ namespace clh {
 
 void fused_plan_info(const FusedPlan& p, clh_plan_info* out)
 {
+    out->num_cloths = p.num_cloths;
+    out->padded_verts = p.N;
     out->num_tiles = p.num_tiles;
     out->tile_verts = p.tile_verts;
     out->halo_tri_factor = p.tri_factor;
@@ -1014,6 +1112,8 @@ void fused_plan_release(FusedPlan& p)
     p.hin_c.release();
     p.gl_words.release();
     p.gl_meta.release();
+    p.tile_cloth.release();
+    p.cloth_tiles.release();
     p.num_tiles = 0;
 }
 

@@ -395,10 +395,12 @@ __device__ __forceinline__ bool ctl_begin(FrameCtl* c, int par, float& omega, in
 }
 
 // After the residual of a check iteration (slot `par`) has been summed:
-// history, stop on convergence or a non-finite residual, and a Chebyshev
-// restart when the residual grew since the previous check. Every third
-// restart in a frame halves 1 - rho. One thread.
-__device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, double b2, double* hist)
+// history, stop on convergence (`conv`: every cloth meets the tolerance) or a
+// non-finite residual, and a Chebyshev restart when the residual grew since
+// the previous check. Every third restart in a frame halves 1 - rho. One
+// thread.
+__device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, double b2, double* hist,
+                                          bool conv)
 {
     const int slot = c->checks;
     hist[2 * slot + 0] = r2;
@@ -409,7 +411,7 @@ __device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, doubl
         c->nonfinite = 1;
         c->stop = 1;
         c->iters = k + 1;
-    } else if (r2 <= c->tol2 * b2) {
+    } else if (conv) {
         c->converged = 1;
         c->stop = 1;
         c->iters = k + 1;

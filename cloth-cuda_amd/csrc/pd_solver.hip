@@ -27,6 +27,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -45,15 +47,22 @@ inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 
 // x_prev = x, first iterate, pin map cleared. Reference mode: y = x + h v +
 // h^2 a and x0 = x (or the guess). PD mode: y holds dy = h v + h^2 a and x0
-// holds the displacement of the guess (0 without one).
+// holds the displacement of the guess (0 without one). With cloth_accel (one
+// cloth) the acceleration is acc + cloth_accel[0..2], else h2a = h^2 acc.
 __global__ void __launch_bounds__(kBlock)
 k_prologue(const float* __restrict__ pos, const float* __restrict__ vel,
-           const float* __restrict__ guess, float* __restrict__ xprev, float* __restrict__ y,
-           void* __restrict__ x0, int* __restrict__ pin_map, int N, float dt, float h2a0,
-           float h2a1, float h2a2, int disp)
+           const float* __restrict__ guess, const float* __restrict__ cloth_accel,
+           float* __restrict__ xprev, float* __restrict__ y, void* __restrict__ x0,
+           int* __restrict__ pin_map, int N, float dt, float h2, float acc0, float acc1,
+           float acc2, float h2a0, float h2a1, float h2a2, int disp)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
+    if (cloth_accel) {
+        h2a0 = h2 * (acc0 + cloth_accel[0]);
+        h2a1 = h2 * (acc1 + cloth_accel[1]);
+        h2a2 = h2 * (acc2 + cloth_accel[2]);
+    }
     const f3 x = ld3(pos, v);
     const f3 u = ld3(vel, v);
     st3(xprev, v, x);
@@ -332,10 +341,71 @@ k_reduce_partials(const double* __restrict__ partials, int nblocks, double* __re
     }
     if (threadIdx.x == 0) {
         if (ctl) {
-            ctl_check(ctl, par, sr[0], sb[0], hist);
+            ctl_check(ctl, par, sr[0], sb[0], hist, sr[0] <= ctl->tol2 * sb[0]);
         } else {
             hist[2 * slot + 0] = sr[0];
             hist[2 * slot + 1] = sb[0];
+        }
+    }
+}
+
+// Batched cloths: per-cloth sums of the tile partials (cloth c owns tiles
+// [cloth_tiles[c], cloth_tiles[c+1])), one wave per cloth with lanes striding
+// over its tiles and a fixed-order butterfly, so every sum is reproducible.
+// Writes (r^2, b^2) of each cloth to cloth_res, the batch totals to the
+// history, and converges only when every cloth meets the tolerance.
+constexpr int kReduceCloths = 1024;
+__global__ void __launch_bounds__(kReduceCloths)
+k_reduce_cloths(const double* __restrict__ partials, const int* __restrict__ cloth_tiles, int C,
+                double* __restrict__ cloth_res, double* __restrict__ hist, int slot,
+                FrameCtl* __restrict__ ctl, int par)
+{
+    if (ctl && ctl->stop) return;
+    constexpr int kWaves = kReduceCloths / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double tol2 = ctl ? ctl->tol2 : 0.0;
+    double wr = 0.0, wb = 0.0;
+    int wconv = 1;
+    for (int c = wave; c < C; c += kWaves) {
+        const int t0 = cloth_tiles[c], t1 = cloth_tiles[c + 1];
+        double r = 0.0, b = 0.0;
+        for (int t = t0 + lane; t < t1; t += 64) {
+            r += partials[2 * t + 0];
+            b += partials[2 * t + 1];
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            r += __shfl_xor(r, off, 64);
+            b += __shfl_xor(b, off, 64);
+        }
+        if (lane == 0) {
+            cloth_res[2 * c + 0] = r;
+            cloth_res[2 * c + 1] = b;
+        }
+        wr += r;
+        wb += b;
+        if (!(r <= tol2 * b)) wconv = 0;
+    }
+    __shared__ double sr[kWaves], sb[kWaves];
+    __shared__ int sc[kWaves];
+    if (lane == 0) {
+        sr[wave] = wr;
+        sb[wave] = wb;
+        sc[wave] = wconv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tr = 0.0, tb = 0.0;
+        int conv = 1;
+        for (int i = 0; i < kWaves; ++i) {
+            tr += sr[i];
+            tb += sb[i];
+            conv &= sc[i];
+        }
+        if (ctl) {
+            ctl_check(ctl, par, tr, tb, hist, conv != 0);
+        } else {
+            hist[2 * slot + 0] = tr;
+            hist[2 * slot + 1] = tb;
         }
     }
 }
@@ -449,6 +519,10 @@ struct clh_solver {
     bool plan_ready = false;
     DevBuf<int> vt_ptr, vt_idx, ve_ptr, ve_idx;
     FusedPlan fused;
+    std::vector<int> seg;        // cloth vertex offsets [C+1] (batched solver), else empty
+    int num_cloths = 1;
+    unsigned long long shape_key = 0;  // autotuner cache key of the mesh shape
+    DevBuf<double> cloth_res;    // per-cloth (r^2, b^2) of the last residual evaluation
     int variant = CLH_KERNEL_UNFUSED;
     int tile_verts = 128;
     int block_threads = 256;
@@ -503,6 +577,29 @@ void drop_chunk_graphs(clh_solver* s)
     }
 }
 
+// FNV-1a over the counts that decide the fused kernel's cost: vertices,
+// triangles, hinges, cloths, mode, and the histogram of (triangle, hinge)
+// valence per vertex.
+unsigned long long shape_key(const clh_solver& s, const std::vector<int>& tptr,
+                             const std::vector<int>& eptr)
+{
+    std::vector<long long> words = {s.N, s.T, s.E, s.num_cloths, s.cfg.mode};
+    std::vector<long long> hist(32 * 32, 0);
+    for (int v = 0; v < s.N; ++v) {
+        const int nt = std::min(31, tptr[(size_t)v + 1] - tptr[(size_t)v]);
+        const int ne = std::min(31, eptr[(size_t)v + 1] - eptr[(size_t)v]);
+        hist[(size_t)nt * 32 + ne]++;
+    }
+    words.insert(words.end(), hist.begin(), hist.end());
+    unsigned long long h = 1469598103934665603ull;
+    for (long long w : words)
+        for (int b = 0; b < 8; ++b) {
+            h ^= (unsigned long long)((w >> (8 * b)) & 0xFF);
+            h *= 1099511628211ull;
+        }
+    return h;
+}
+
 int ensure_plan(clh_solver* s, const clh_step_args& a)
 {
     if (s->plan_ready && s->plan_tris == a.d_tris && s->plan_quads == a.d_bend_quads)
@@ -522,13 +619,15 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
     for (int v : quads)
         if (v < 0 || v >= N) return fail(CLH_ERR_INVALID, "hinge vertex index out of range");
 
-    std::vector<int> ptr, idx;
+    std::vector<int> ptr, idx, tptr;
     build_incidence(tris, 3, N, ptr, idx);
     CLH_TRY(s->vt_ptr.upload(ptr));
     CLH_TRY(s->vt_idx.upload(idx));
+    tptr.swap(ptr);
     build_incidence(quads, 4, N, ptr, idx);
     CLH_TRY(s->ve_ptr.upload(ptr));
     CLH_TRY(s->ve_idx.upload(idx));
+    s->shape_key = shape_key(*s, tptr, ptr);
 
     s->variant = CLH_KERNEL_UNFUSED;
     if (s->cfg.kernel_variant != CLH_KERNEL_UNFUSED && N > 0 && a.d_pos) {
@@ -540,15 +639,17 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
         CLH_TRY(fused_enable_lds(lds_limit));
         int st = CLH_ERR_UNSUPPORTED;
         for (int tv = s->tile_verts; tv >= 32; tv /= 2) {
-            st = fused_plan_build(s->fused, N, tris, quads, pos, tv);
+            st = fused_plan_build(s->fused, N, tris, quads, pos, tv, s->seg);
             if (st != CLH_OK) break;
             s->fused.threads = s->block_threads;
             if (fused_lds_bytes(s->fused, s->cfg.mode, true) <= (size_t)lds_limit) break;
             st = fail(CLH_ERR_UNSUPPORTED, "fused plan: tile does not fit in LDS");
         }
         if (st == CLH_OK) s->variant = CLH_KERNEL_FUSED;
-        else if (s->cfg.kernel_variant == CLH_KERNEL_FUSED) return st;
+        else if (s->cfg.kernel_variant == CLH_KERNEL_FUSED || s->num_cloths > 1) return st;
     }
+    if (s->num_cloths > 1 && s->variant != CLH_KERNEL_FUSED)
+        return fail(CLH_ERR_UNSUPPORTED, "batched cloths need the fused kernel variant");
 
     s->plan_tris = a.d_tris;
     s->plan_quads = a.d_bend_quads;
@@ -561,14 +662,17 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
 int ensure_scratch(clh_solver* s)
 {
     const size_t N = (size_t)s->N, T = (size_t)s->T, E = (size_t)s->E;
-    CLH_TRY(s->y.resize(N * 3));
-    CLH_TRY(s->xprev.resize(N * 3));
+    // the fused variant keeps per-vertex state in tile order, padded
+    const size_t Nt = s->variant == CLH_KERNEL_FUSED ? std::max(N, (size_t)s->fused.N) : N;
+    CLH_TRY(s->y.resize(Nt * 3));
+    CLH_TRY(s->xprev.resize(Nt * 3));
     // float iterates in reference mode need half of this
-    for (auto& b : s->it) CLH_TRY(b.resize(N * 3));
+    for (auto& b : s->it) CLH_TRY(b.resize(Nt * 3));
     CLH_TRY(s->pin_map.resize(N));
     if (s->variant == CLH_KERNEL_FUSED) {
-        CLH_TRY(s->vinfo.resize(N));
+        CLH_TRY(s->vinfo.resize(Nt));
     }
+    CLH_TRY(s->cloth_res.resize((size_t)s->num_cloths * 2));
     CLH_TRY(s->fr.resize(T * 9));
     CLH_TRY(s->bp.resize(E * 3));
     if (s->cfg.mode == CLH_MODE_PD) {
@@ -622,13 +726,13 @@ int enqueue_prologue(clh_solver* s, const clh_step_args& a)
         const float h2a[3] = {h2a0, h2a1, h2a2};
         CLH_TRY(fused_pack_constants(s->fused, a, h2, s->stream));
         return fused_prologue(s->fused, a, s->xprev.ptr, s->y.ptr, s->it[0].ptr, s->vinfo.ptr, c.dt,
-                              h2a, disp, s->stream);
+                              h2, acc, h2a, disp, s->stream);
     }
     if (s->N > 0)
         hipLaunchKernelGGL(k_prologue, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream, a.d_pos,
-                           a.d_vel, a.d_init_guess, s->xprev.ptr, s->y.ptr, s->it[0].ptr,
-                           s->pin_map.ptr, s->N, c.dt, h2a0, h2a1, h2a2,
-                           c.mode == CLH_MODE_PD ? 1 : 0);
+                           a.d_vel, a.d_init_guess, a.d_cloth_accel, s->xprev.ptr, s->y.ptr,
+                           s->it[0].ptr, s->pin_map.ptr, s->N, c.dt, h2, acc[0], acc[1], acc[2],
+                           h2a0, h2a1, h2a2, c.mode == CLH_MODE_PD ? 1 : 0);
     if (a.num_pinned > 0)
         hipLaunchKernelGGL(k_pin_scatter, dim3(grid_for(a.num_pinned)), dim3(kBlock), 0,
                            s->stream, a.d_pin_idx, a.d_pin_target, s->xprev.ptr, s->pin_map.ptr,
@@ -666,7 +770,11 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
         f.ctl = ctl;
         f.par = k & 1;
         CLH_TRY(fused_launch_iteration(s->fused, f, c.mode, s->stream));
-        if (check)
+        if (check && s->num_cloths > 1)
+            hipLaunchKernelGGL(k_reduce_cloths, dim3(1), dim3(kReduceCloths), 0, s->stream,
+                               s->partials.ptr, s->fused.cloth_tiles.ptr, s->num_cloths,
+                               s->cloth_res.ptr, s->hist.ptr, check_slot, ctl, k & 1);
+        else if (check)
             hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->stream,
                                s->partials.ptr, s->fused.num_tiles, s->hist.ptr, check_slot, ctl,
                                k & 1);
@@ -1127,5 +1235,135 @@ extern "C" int clh_solver_plan_info(clh_solver* s, clh_plan_info* out)
     std::memset(out, 0, sizeof(*out));
     out->variant = s->plan_ready ? s->variant : CLH_KERNEL_AUTO;
     if (s->plan_ready && s->variant == CLH_KERNEL_FUSED) fused_plan_info(s->fused, out);
+    return CLH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched cloths and the autotuner
+// ---------------------------------------------------------------------------
+
+extern "C" int clh_batch_create(const clh_config* cfg, const clh_mesh_desc* mesh, int num_cloths,
+                                const int* offs, clh_solver** out)
+{
+    if (!cfg || !mesh || !out || !offs || num_cloths < 1)
+        return fail(CLH_ERR_INVALID, "clh_batch_create: bad arguments");
+    *out = nullptr;
+    if (cfg->kernel_variant == CLH_KERNEL_UNFUSED)
+        return fail(CLH_ERR_UNSUPPORTED, "clh_batch_create: batched cloths need the fused kernel");
+    if (offs[0] != 0 || offs[num_cloths] != mesh->num_verts)
+        return fail(CLH_ERR_INVALID, "clh_batch_create: offsets must run from 0 to num_verts");
+    for (int c = 0; c < num_cloths; ++c)
+        if (offs[c + 1] <= offs[c])
+            return fail(CLH_ERR_INVALID, "clh_batch_create: every cloth needs at least one vertex");
+    clh_solver* s = nullptr;
+    CLH_TRY(clh_solver_create(cfg, mesh, &s));
+    s->seg.assign(offs, offs + num_cloths + 1);
+    s->num_cloths = num_cloths;
+    *out = s;
+    return CLH_OK;
+}
+
+extern "C" int clh_batch_cloth_residuals(clh_solver* s, double* out, int max_count, int* count)
+{
+    if (!s || !count) return fail(CLH_ERR_INVALID, "clh_batch_cloth_residuals: null pointer");
+    CLH_HIP(hipStreamSynchronize(s->stream));
+    *count = s->num_cloths;
+    if (s->cfg.mode != CLH_MODE_PD || s->last_checks == 0 || !out) return CLH_OK;
+    const int n = std::min(s->num_cloths, std::max(max_count, 0));
+    std::vector<double> h((size_t)s->num_cloths * 2);
+    if (s->num_cloths > 1) {
+        CLH_HIP(hipMemcpy(h.data(), s->cloth_res.ptr, h.size() * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    } else {
+        // one cloth: the last entry of the history
+        CLH_HIP(hipMemcpy(h.data(), s->hist.ptr + 2 * (size_t)(s->last_checks - 1),
+                          2 * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    for (int c = 0; c < n; ++c)
+        out[c] = h[2 * (size_t)c + 1] > 0.0 ? std::sqrt(h[2 * (size_t)c] / h[2 * (size_t)c + 1]) : 0.0;
+    return CLH_OK;
+}
+
+namespace {
+std::mutex g_tune_mu;
+std::map<unsigned long long, std::pair<int, int>> g_tune_cache;  // key -> (tile_verts, threads)
+
+// Rebuilds the plan with the given launch shape; returns the tile size the
+// plan actually got (smaller when the tile did not fit in LDS).
+int replan(clh_solver* s, const clh_step_args& a, int tile_verts, int threads, int& got)
+{
+    s->tile_verts = tile_verts;
+    s->block_threads = threads;
+    s->plan_ready = false;
+    CLH_TRY(ensure_plan(s, a));
+    CLH_TRY(ensure_scratch(s));
+    s->fused.threads = threads;
+    got = s->variant == CLH_KERNEL_FUSED ? s->fused.tile_verts : 0;
+    return CLH_OK;
+}
+}  // namespace
+
+extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int num_launches,
+                                   clh_tune_report* out)
+{
+    if (!s || !out || num_launches <= 0)
+        return fail(CLH_ERR_INVALID, "clh_solver_autotune: bad arguments");
+    CLH_TRY(validate_args(s, a));
+    if (s->cfg.kernel_variant == CLH_KERNEL_UNFUSED)
+        return fail(CLH_ERR_UNSUPPORTED, "clh_solver_autotune: the unfused variant has no launch shape");
+    std::memset(out, 0, sizeof(*out));
+    CLH_TRY(ensure_plan(s, *a));
+    if (s->variant != CLH_KERNEL_FUSED)
+        return fail(CLH_ERR_UNSUPPORTED, "clh_solver_autotune: the mesh has no fused plan");
+    out->key = s->shape_key;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(s->shape_key);
+        if (it != g_tune_cache.end()) {
+            int got = 0;
+            if (s->fused.tile_verts != it->second.first || s->fused.threads != it->second.second)
+                CLH_TRY(replan(s, *a, it->second.first, it->second.second, got));
+            out->num_candidates = 1;
+            out->tile_verts[0] = it->second.first;
+            out->block_threads[0] = it->second.second;
+            out->best = 0;
+            out->from_cache = 1;
+            return CLH_OK;
+        }
+    }
+    const int tiles[3] = {64, 128, 256}, threads[2] = {256, 512};
+    int n = 0, best = -1;
+    for (int tv : tiles) {
+        int got = 0;
+        CLH_TRY(replan(s, *a, tv, 256, got));
+        for (int th : threads) {
+            out->tile_verts[n] = tv;
+            out->block_threads[n] = th;
+            out->us_per_launch[n] = 0.0f;
+            if (got == tv) {
+                s->fused.threads = th;
+                CLH_TRY(enqueue_prologue(s, *a));
+                CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, false, 0));  // warm-up
+                CLH_HIP(hipStreamSynchronize(s->stream));
+                CLH_HIP(hipEventRecord(s->ev_t0, s->stream));
+                for (int i = 0; i < num_launches; ++i)
+                    CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, false, 0));
+                CLH_HIP(hipEventRecord(s->ev_t1, s->stream));
+                CLH_HIP(hipEventSynchronize(s->ev_t1));
+                float ms = 0.0f;
+                CLH_HIP(hipEventElapsedTime(&ms, s->ev_t0, s->ev_t1));
+                out->us_per_launch[n] = 1e3f * ms / (float)num_launches;
+                if (best < 0 || out->us_per_launch[n] < out->us_per_launch[best]) best = n;
+            }
+            ++n;
+        }
+    }
+    out->num_candidates = n;
+    if (best < 0) return fail(CLH_ERR_UNSUPPORTED, "clh_solver_autotune: no candidate fits");
+    out->best = best;
+    int got = 0;
+    CLH_TRY(replan(s, *a, out->tile_verts[best], out->block_threads[best], got));
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[s->shape_key] = {out->tile_verts[best], out->block_threads[best]};
     return CLH_OK;
 }

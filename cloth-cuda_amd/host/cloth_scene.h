@@ -43,6 +43,14 @@ cc_scene* cc_scene_build(const cc_scene_spec* spec);
  * grid fields of spec are used only for pin selection (ncols). */
 cc_scene* cc_scene_build_obj(const char* path, const cc_scene_spec* spec);
 void cc_scene_destroy(cc_scene* s);
+/* Batched workload: each spec built as cc_scene_build builds it, then the
+ * cloths concatenated into one scene (vertex and triangle ids shifted; the
+ * specs must share dt). The per-cloth offsets give each cloth's range. */
+cc_scene* cc_batch_build(const cc_scene_spec* specs, int count);
+int  cc_scene_num_cloths(const cc_scene* s);
+/* Offsets [C+1] of each cloth's vertices, triangles, hinges and pins in the
+ * concatenated arrays; a null pointer skips that table. */
+int  cc_scene_cloth_offsets(const cc_scene* s, int* vert, int* tri, int* bend, int* pin);
 void cc_scene_counts(const cc_scene* s, int* N, int* T, int* E, int* P);
 /* Host export of a CPU array by name into dst (bytes must match exactly);
  * returns 0 on success. Names: rest_pos pos0 tris Dm_inv rest_area mass

@@ -23,6 +23,8 @@ struct cc_scene {
     SimConstraints sim;
     Constraints    pins;
     std::vector<float> pos0;  // initial positions [N*3]
+    // per-cloth offsets [C+1] into the vertex, triangle, hinge and pin arrays
+    std::vector<int> cloth_vert{0}, cloth_tri{0}, cloth_bend{0}, cloth_pin{0};
     bool uploaded = false;
     ~cc_scene() { pins.free_gpu(); }
 };
@@ -131,6 +133,10 @@ void finish_scene(cc_scene* s)
     s->sim.jacobi_diag = s->sim.compute_jacobi_diag(s->mesh, sp.dt);
 
     const int N = s->mesh.num_verts;
+    s->cloth_vert = {0, N};
+    s->cloth_tri = {0, s->mesh.num_tris};
+    s->cloth_bend = {0, s->sim.num_bend_cons};
+    s->cloth_pin = {0, (int)s->pins.pinned_indices.size()};
     s->pos0.resize((size_t)N * 3);
     for (int i = 0; i < N; ++i)
         for (int k = 0; k < 3; ++k) s->pos0[(size_t)i * 3 + k] = s->mesh.rest_pos[i](k);
@@ -147,6 +153,93 @@ void finish_scene(cc_scene* s)
 }  // namespace
 
 extern "C" void cc_scene_destroy(cc_scene* s) { delete s; }
+
+namespace {
+// Appends cloth `c` to the batch `b`, shifting its vertex and triangle ids.
+void append_cloth(cc_scene& b, const cc_scene& c)
+{
+    const int vo = b.mesh.num_verts, to = b.mesh.num_tris;
+    ClothMesh& m = b.mesh;
+    const ClothMesh& cm = c.mesh;
+    m.rest_pos.insert(m.rest_pos.end(), cm.rest_pos.begin(), cm.rest_pos.end());
+    for (const auto& t : cm.triangles) m.triangles.push_back(cloth::Vec3i(t(0) + vo, t(1) + vo, t(2) + vo));
+    m.Dm_inv.insert(m.Dm_inv.end(), cm.Dm_inv.begin(), cm.Dm_inv.end());
+    m.deltaUV.insert(m.deltaUV.end(), cm.deltaUV.begin(), cm.deltaUV.end());
+    m.dF_dx.insert(m.dF_dx.end(), cm.dF_dx.begin(), cm.dF_dx.end());
+    m.rest_area.insert(m.rest_area.end(), cm.rest_area.begin(), cm.rest_area.end());
+    m.mass.insert(m.mass.end(), cm.mass.begin(), cm.mass.end());
+    m.pos_cpu.insert(m.pos_cpu.end(), cm.pos_cpu.begin(), cm.pos_cpu.end());
+    m.vel_cpu.insert(m.vel_cpu.end(), cm.vel_cpu.begin(), cm.vel_cpu.end());
+    m.num_verts += cm.num_verts;
+    m.num_tris += cm.num_tris;
+
+    SimConstraints& sc = b.sim;
+    const SimConstraints& cs = c.sim;
+    sc.tri_stretch_k.insert(sc.tri_stretch_k.end(), cs.tri_stretch_k.begin(), cs.tri_stretch_k.end());
+    for (const auto& q : cs.bend_quads)
+        sc.bend_quads.push_back(cloth::Vec4i(q(0) + vo, q(1) + vo, q(2) + vo, q(3) + vo));
+    sc.bend_w.insert(sc.bend_w.end(), cs.bend_w.begin(), cs.bend_w.end());
+    sc.bend_n.insert(sc.bend_n.end(), cs.bend_n.begin(), cs.bend_n.end());
+    sc.bend_stiffness.insert(sc.bend_stiffness.end(), cs.bend_stiffness.begin(), cs.bend_stiffness.end());
+    sc.jacobi_diag.insert(sc.jacobi_diag.end(), cs.jacobi_diag.begin(), cs.jacobi_diag.end());
+    sc.num_tris += cs.num_tris;
+    sc.num_bend_cons += cs.num_bend_cons;
+
+    for (auto e : c.topo.edges) {
+        e.v0 += vo;
+        e.v1 += vo;
+        e.tri_a += to;
+        if (e.tri_b >= 0) e.tri_b += to;
+        b.topo.edges.push_back(e);
+    }
+    for (const auto& l : c.topo.vert_to_tris) {
+        std::vector<int> s2;
+        for (int t : l) s2.push_back(t + to);
+        b.topo.vert_to_tris.push_back(s2);
+    }
+    for (const auto& t : c.topo.tri_to_tris)
+        b.topo.tri_to_tris.push_back(cloth::Vec3i(t(0) >= 0 ? t(0) + to : t(0),
+                                                  t(1) >= 0 ? t(1) + to : t(1),
+                                                  t(2) >= 0 ? t(2) + to : t(2)));
+
+    for (int v : c.pins.pinned_indices) b.pins.pinned_indices.push_back(v + vo);
+    b.pins.target_positions.insert(b.pins.target_positions.end(), c.pins.target_positions.begin(),
+                                   c.pins.target_positions.end());
+    b.pos0.insert(b.pos0.end(), c.pos0.begin(), c.pos0.end());
+
+    b.cloth_vert.push_back(m.num_verts);
+    b.cloth_tri.push_back(m.num_tris);
+    b.cloth_bend.push_back(sc.num_bend_cons);
+    b.cloth_pin.push_back((int)b.pins.pinned_indices.size());
+}
+}  // namespace
+
+extern "C" cc_scene* cc_batch_build(const cc_scene_spec* specs, int count)
+{
+    if (!specs || count < 1) return nullptr;
+    for (int c = 1; c < count; ++c)
+        if (specs[c].dt != specs[0].dt) return nullptr;  // one Jacobi diagonal time step
+    cc_scene* b = new cc_scene();
+    b->spec = specs[0];
+    for (int c = 0; c < count; ++c) {
+        cc_scene* one = cc_scene_build(&specs[c]);
+        append_cloth(*b, *one);
+        cc_scene_destroy(one);
+    }
+    return b;
+}
+
+extern "C" int cc_scene_num_cloths(const cc_scene* s) { return (int)s->cloth_vert.size() - 1; }
+
+extern "C" int cc_scene_cloth_offsets(const cc_scene* s, int* vert, int* tri, int* bend, int* pin)
+{
+    const size_t n = s->cloth_vert.size();
+    if (vert) std::memcpy(vert, s->cloth_vert.data(), n * sizeof(int));
+    if (tri) std::memcpy(tri, s->cloth_tri.data(), n * sizeof(int));
+    if (bend) std::memcpy(bend, s->cloth_bend.data(), n * sizeof(int));
+    if (pin) std::memcpy(pin, s->cloth_pin.data(), n * sizeof(int));
+    return 0;
+}
 
 extern "C" void cc_scene_counts(const cc_scene* s, int* N, int* T, int* E, int* P)
 {

@@ -160,6 +160,8 @@ typedef struct clh_step_args {
     int          num_pinned;
     const float* d_init_guess;   /* optional [N*3] first iterate (warm start); NULL = x_n */
     float*       d_trace;        /* optional [max_iterations*N*3] every iterate */
+    const float* d_cloth_accel;  /* optional [C*3] per-cloth acceleration added to
+                                    ext_accel (C = 1 for a solver of one cloth) */
 } clh_step_args;
 
 /* Replaces PDSolver::step (src/pd_solver.cu:496-621). Work is enqueued on the
@@ -212,19 +214,60 @@ typedef struct clh_plan_info {
     double bytes_per_iter;   /* bytes the kernel streams per iteration (model) */
     double lds_bytes;        /* LDS per workgroup of the residual-check variant */
     int    block_threads;    /* workgroup size of the fused kernel */
+    int    num_cloths;       /* cloths in the plan (1 unless batched) */
+    int    padded_verts;     /* tile-order length: each cloth padded to whole tiles */
 } clh_plan_info;
 int clh_solver_plan_info(clh_solver* s, clh_plan_info* out);
 
 /* Builds the fused kernel's tile plan for a mesh from host arrays (no device
- * needed) and checks every index in it: local ids inside each tile's LDS
- * table, halo ids outside the tile, each local slot naming the element's
- * vertex, and each vertex's gather list equal to its incident element slots
- * in ascending order. Returns CLH_OK and the plan statistics, or an error
- * naming the first violated property. Also the tile-size probe of the
- * autotuner. */
+ * needed) and checks every index in it: the tile order is a permutation of
+ * the vertices plus padding, each cloth fills whole tiles of its own, local
+ * ids lie inside each tile's LDS table, halo ids outside the tile and inside
+ * its cloth, each local slot names the element's vertex, and each vertex's
+ * gather list equals its incident element slots in ascending order. Returns
+ * CLH_OK and the plan statistics, or an error naming the first violated
+ * property. num_cloths = 0 means one cloth; otherwise cloth_vert_offsets
+ * [num_cloths + 1] delimits the cloths as in clh_batch_create. */
 int clh_fused_plan_check(int num_verts, int num_tris, int num_bend, const int* tris,
                          const int* bend_quads, const float* pos, int tile_verts, int mode,
-                         clh_plan_info* out);
+                         int num_cloths, const int* cloth_vert_offsets, clh_plan_info* out);
+
+/* ---- batched cloths -------------------------------------------------------
+ * Many independent cloths of uneven size and material in one solver: the
+ * arrays of clh_step_args are the concatenation of the cloths' arrays, with
+ * vertex indices global (cloth c owns vertices [cloth_vert_offsets[c],
+ * cloth_vert_offsets[c+1]) and every triangle, hinge and pin of it indexes
+ * only those). One launch of the fused iteration kernel per iteration covers
+ * every cloth; tiles never straddle two cloths. Per-cloth stiffness comes
+ * from the per-element arrays, per-cloth wind from d_cloth_accel. A frame run
+ * to a tolerance stops when every cloth's relative residual meets it.
+ * No counterpart in the reference, whose PDSolver holds one mesh
+ * (src/pd_solver.cu:445-460). */
+int clh_batch_create(const clh_config* cfg, const clh_mesh_desc* mesh, int num_cloths,
+                     const int* cloth_vert_offsets, clh_solver** out);
+
+/* FP64 relative residual ||b_c - K_c x_c|| / ||b_c|| of each cloth at the last
+ * residual evaluation of the last frame (CLH_MODE_PD); *count gets the number
+ * of cloths. */
+int clh_batch_cloth_residuals(clh_solver* s, double* out, int max_count, int* count);
+
+/* Autotuner of the fused kernel's launch shape: times every (tile_verts,
+ * block_threads) candidate on this solver's mesh (num_launches iteration
+ * launches each, after one warm-up) and keeps the fastest. Results are
+ * cached per process under a key of the vertex, triangle, hinge and cloth
+ * counts, the mode and the histogram of vertex valences, so another solver
+ * on an equally shaped mesh reuses the choice without timing. */
+typedef struct clh_tune_report {
+    int   num_candidates;
+    int   tile_verts[8];
+    int   block_threads[8];
+    float us_per_launch[8];   /* 0 when the candidate did not fit */
+    int   best;               /* index of the chosen candidate */
+    int   from_cache;         /* 1 when the choice came from the cache */
+    unsigned long long key;   /* cache key of the mesh shape */
+} clh_tune_report;
+int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int num_launches,
+                        clh_tune_report* out);
 
 /* Streams bytes of device memory through a copy kernel num_launches times and
  * returns the average time per launch (measured copy peak). */

@@ -21,6 +21,23 @@ CFG3 = dict(nrows=1024, ncols=1024, size=1 / 1023, type=ch.MESH_IRREGULAR, rest_
 CFG2_WIND = (0.0, 0.0, 5.0)
 
 
+def cfg4_specs(count=512, seed=4, side_max=256, dt=1 / 60):
+    """The batched workload: cloth sides round(16 * 16^U) (capped at
+    side_max), stretch log-uniform in [1e2, 1e4], bend log-uniform in
+    [1e-3, 1e-1], 1-4 random boundary pins, each cloth 1 m wide, +-1e-3 m
+    jitter, seeded."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(min(side_max, round(16 * 16 ** rng.uniform())))
+        out.append(dict(nrows=n, ncols=n, size=1 / (n - 1), type=0, seed=1000 + i, density=0.1,
+                        stretch_k=float(np.exp(rng.uniform(np.log(1e2), np.log(1e4)))),
+                        bend_k=float(np.exp(rng.uniform(np.log(1e-3), np.log(1e-1)))),
+                        pin_mode=ch.PIN_RANDOM_BOUNDARY, pin_count=int(rng.integers(1, 5)),
+                        pos_jitter=1e-3, dt=dt))
+    return out
+
+
 def scene_arrays(scene):
     out = {}
     for k in ARRAYS:
