@@ -103,7 +103,20 @@ CLH_SYMBOLS = [
     "clh_solver_set_stream", "clh_solver_run_frames", "clh_solver_time_iteration_kernel",
     "clh_solver_plan_info", "clh_measure_copy", "clh_fused_plan_check",
     "clh_batch_create", "clh_batch_cloth_residuals", "clh_solver_autotune",
+    "clh_ws_create", "clh_ws_destroy", "clh_ws_set_stream", "clh_ws_param_count",
+    "clh_ws_get_params", "clh_ws_set_params", "clh_ws_predict", "clh_ws_features",
+    "clh_ws_dataset_reserve", "clh_ws_dataset_add", "clh_ws_dataset_size", "clh_ws_dataset_clear",
+    "clh_ws_dataset_read", "clh_ws_fit_output_scale", "clh_ws_loss_grad", "clh_ws_train",
+    "clh_ws_adam_step", "clh_ws_set_smoothing", "clh_solver_start_residual",
 ]
+
+
+class WsDesc(C.Structure):
+    _fields_ = [("num_verts", C.c_int), ("num_tris", C.c_int), ("d_tris", _vp),
+                ("num_cloths", C.c_int), ("cloth_vert_offsets", _vp), ("d_pin_idx", _vp),
+                ("num_pinned", C.c_int), ("dt", C.c_float), ("gravity", C.c_float),
+                ("ext_accel", C.c_float * 3), ("seed", C.c_ulonglong),
+                ("ext_feature", C.c_int)]
 
 _hip = None
 _core = None
@@ -145,6 +158,26 @@ def _load():
                                      C.POINTER(_vp)]
     hip.clh_batch_cloth_residuals.argtypes = [_vp, C.POINTER(C.c_double), C.c_int, _ip]
     hip.clh_solver_autotune.argtypes = [_vp, C.POINTER(StepArgs), C.c_int, C.POINTER(TuneReport)]
+    _dp = C.POINTER(C.c_double)
+    hip.clh_ws_create.argtypes = [C.POINTER(WsDesc), C.POINTER(_vp)]
+    hip.clh_ws_destroy.argtypes = [_vp]
+    hip.clh_ws_set_stream.argtypes = [_vp, _vp]
+    hip.clh_ws_param_count.argtypes = []
+    hip.clh_ws_get_params.argtypes = [_vp, _fp, C.c_int, _fp]
+    hip.clh_ws_set_params.argtypes = [_vp, _fp, C.c_int, C.c_float]
+    hip.clh_ws_predict.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int]
+    hip.clh_ws_features.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+    hip.clh_ws_dataset_reserve.argtypes = [_vp, C.c_longlong]
+    hip.clh_ws_dataset_add.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    hip.clh_ws_dataset_size.argtypes = [_vp, C.POINTER(C.c_longlong)]
+    hip.clh_ws_dataset_clear.argtypes = [_vp]
+    hip.clh_ws_dataset_read.argtypes = [_vp, C.c_longlong, C.c_int, _vp, _vp]
+    hip.clh_ws_fit_output_scale.argtypes = [_vp, _fp]
+    hip.clh_ws_loss_grad.argtypes = [_vp, C.c_longlong, C.c_int, _fp, _fp]
+    hip.clh_ws_train.argtypes = [_vp, C.c_int, C.c_int, C.c_float, _fp]
+    hip.clh_ws_adam_step.argtypes = [_vp, _fp, C.c_float]
+    hip.clh_ws_set_smoothing.argtypes = [_vp, C.c_int]
+    hip.clh_solver_start_residual.argtypes = [_vp, C.POINTER(StepArgs), C.POINTER(C.c_double)]
     hip.clh_get_device_info.argtypes = [C.c_int, C.POINTER(DeviceInfo)]
     hip.clh_device_count.argtypes = [_ip]
 
@@ -409,6 +442,12 @@ class Solver:
         self.h = h.value
         self.cfg = cfg
 
+    def start_residual(self, args: StepArgs) -> float:
+        r = C.c_double(0)
+        check(hip().clh_solver_start_residual(self.h, C.byref(args), C.byref(r)),
+              "clh_solver_start_residual")
+        return r.value
+
     def cloth_residuals(self, max_count=1 << 16) -> np.ndarray:
         out = np.zeros(max_count, np.float64)
         n = C.c_int(0)
@@ -542,3 +581,124 @@ def measure_copy(nbytes: int, launches: int = 20) -> float:
         src.free()
         dst.free()
     return ms.value
+
+
+def _dptr(a):
+    """Device pointer of a DeviceArray or a raw pointer value."""
+    if a is None:
+        return None
+    return a.ptr if isinstance(a, DeviceArray) else a
+
+
+class WarmStart:
+    """The learned warm start (clh_ws_*): per-vertex BF16 MFMA MLP predicting
+    the converged positions of the next frame."""
+
+    def __init__(self, N, T, d_tris, dt, gravity=-9.81, ext_accel=(0.0, 0.0, 0.0),
+                 d_pin_idx=None, num_pinned=0, cloth_offsets=None, seed=1, ext_feature=False):
+        d = WsDesc()
+        d.num_verts, d.num_tris, d.d_tris = N, T, d_tris
+        self._offs = None
+        if cloth_offsets is not None and len(cloth_offsets) > 2:
+            self._offs = np.ascontiguousarray(cloth_offsets, np.int32)
+            d.num_cloths, d.cloth_vert_offsets = len(self._offs) - 1, _ptr(self._offs)
+        else:
+            d.num_cloths, d.cloth_vert_offsets = 1, None
+        d.d_pin_idx, d.num_pinned = d_pin_idx, num_pinned
+        d.dt, d.gravity = dt, gravity
+        for i in range(3):
+            d.ext_accel[i] = float(ext_accel[i])
+        d.seed = seed
+        d.ext_feature = 1 if ext_feature else 0
+        h = _vp()
+        check(hip().clh_ws_create(C.byref(d), C.byref(h)), "clh_ws_create")
+        self.h = h.value
+        self.N = N
+        self.P = hip().clh_ws_param_count()
+
+    @classmethod
+    def for_scene(cls, scene, dt, gravity=-9.81, ext_accel=(0.0, 0.0, 0.0), seed=1,
+                  ext_feature=False):
+        return cls(scene.N, scene.T, scene.device("tris"), dt, gravity, ext_accel,
+                   scene.device("pin_idx"), scene.P, scene.vert_off, seed, ext_feature)
+
+    def params(self):
+        out = np.zeros(self.P, np.float32)
+        sc = C.c_float(0)
+        check(hip().clh_ws_get_params(self.h, out.ctypes.data_as(_fp), self.P, C.byref(sc)),
+              "clh_ws_get_params")
+        return out, sc.value
+
+    def set_params(self, p, scale):
+        p = np.ascontiguousarray(p, np.float32)
+        check(hip().clh_ws_set_params(self.h, p.ctypes.data_as(_fp), len(p), scale),
+              "clh_ws_set_params")
+
+    def predict(self, xm2, xm1, x, guess, cloth_accel=None, use_network=True):
+        check(hip().clh_ws_predict(self.h, _dptr(xm2), _dptr(xm1), _dptr(x), _dptr(cloth_accel),
+                                   _dptr(guess), 1 if use_network else 0), "clh_ws_predict")
+
+    def set_smoothing(self, passes):
+        check(hip().clh_ws_set_smoothing(self.h, passes), "clh_ws_set_smoothing")
+
+    def features(self, xm2, xm1, x, cloth_accel=None):
+        out = DeviceArray(self.N * 16 * 2)
+        check(hip().clh_ws_features(self.h, _dptr(xm2), _dptr(xm1), _dptr(x), _dptr(cloth_accel),
+                                    out.ptr), "clh_ws_features")
+        return out.download(np.uint16, (self.N, 16))
+
+    def dataset_reserve(self, n):
+        check(hip().clh_ws_dataset_reserve(self.h, n), "clh_ws_dataset_reserve")
+
+    def dataset_add(self, xm2, xm1, x, xnext, cloth_accel=None, cloth_weight=None):
+        check(hip().clh_ws_dataset_add(self.h, _dptr(xm2), _dptr(xm1), _dptr(x), _dptr(xnext),
+                                       _dptr(cloth_accel), _dptr(cloth_weight)),
+              "clh_ws_dataset_add")
+
+    def dataset_size(self):
+        n = C.c_longlong(0)
+        check(hip().clh_ws_dataset_size(self.h, C.byref(n)), "clh_ws_dataset_size")
+        return n.value
+
+    def dataset_clear(self):
+        check(hip().clh_ws_dataset_clear(self.h), "clh_ws_dataset_clear")
+
+    def dataset_read(self, first, count):
+        F = np.zeros((count, 16), np.uint16)
+        T = np.zeros((count, 4), np.float32)
+        check(hip().clh_ws_dataset_read(self.h, first, count, _ptr(F), _ptr(T)),
+              "clh_ws_dataset_read")
+        return F, T
+
+    def fit_output_scale(self):
+        sc = C.c_float(0)
+        check(hip().clh_ws_fit_output_scale(self.h, C.byref(sc)), "clh_ws_fit_output_scale")
+        return sc.value
+
+    def loss_grad(self, first, count):
+        g = np.zeros(self.P, np.float32)
+        loss = C.c_float(0)
+        check(hip().clh_ws_loss_grad(self.h, first, count, C.byref(loss),
+                                     g.ctypes.data_as(_fp)), "clh_ws_loss_grad")
+        return loss.value, g
+
+    def train(self, steps, batch, lr):
+        losses = np.zeros(max(steps, 1), np.float32)
+        check(hip().clh_ws_train(self.h, steps, batch, lr, losses.ctypes.data_as(_fp)),
+              "clh_ws_train")
+        return losses[:steps]
+
+    def adam_step(self, grad, lr):
+        g = np.ascontiguousarray(grad, np.float32)
+        check(hip().clh_ws_adam_step(self.h, g.ctypes.data_as(_fp), lr), "clh_ws_adam_step")
+
+    def close(self):
+        if self.h:
+            hip().clh_ws_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1367,3 +1367,26 @@ extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int nu
     g_tune_cache[s->shape_key] = {out->tile_verts[best], out->block_threads[best]};
     return CLH_OK;
 }
+
+extern "C" int clh_solver_start_residual(clh_solver* s, const clh_step_args* a, double* rel)
+{
+    if (!s || !rel) return fail(CLH_ERR_INVALID, "clh_solver_start_residual: null pointer");
+    CLH_TRY(validate_args(s, a));
+    if (s->cfg.mode != CLH_MODE_PD)
+        return fail(CLH_ERR_UNSUPPORTED, "clh_solver_start_residual: PD mode only");
+    CLH_TRY(ensure_plan(s, *a));
+    CLH_TRY(ensure_scratch(s));
+    if (!s->user_stream) {
+        CLH_HIP(hipEventRecord(s->ev_in, 0));
+        CLH_HIP(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+    }
+    // the prologue writes only solver scratch; iteration 0 with a residual
+    // check measures its first iterate and writes x_1 to scratch as well
+    CLH_TRY(enqueue_prologue(s, *a));
+    CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, true, 0));
+    double h[2] = {0.0, 0.0};
+    CLH_HIP(hipMemcpyAsync(h, s->hist.ptr, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    CLH_HIP(hipStreamSynchronize(s->stream));
+    *rel = h[1] > 0.0 ? std::sqrt(h[0] / h[1]) : 0.0;
+    return CLH_OK;
+}

@@ -178,6 +178,13 @@ typedef struct clh_step_stats {
     int    frames;       /* frames stepped since creation or reset           */
 } clh_step_stats;
 
+/* FP64 relative residual ||b - K x_0|| / ||b|| of the frame's first iterate
+ * (x_0 = a->d_init_guess, or x_n when it is NULL; pinned vertices at their
+ * targets) without stepping: the mesh state is left as it is. CLH_MODE_PD.
+ * Lets a caller judge candidate starts by the measure the frame stops on.
+ * No counterpart in the reference. */
+int clh_solver_start_residual(clh_solver* s, const clh_step_args* a, double* rel);
+
 /* Synchronizes the solver's stream and returns statistics of the last frame. */
 int clh_solver_get_stats(clh_solver* s, clh_step_stats* out);
 /* Copies up to max_count FP64 relative residuals ||r_k||/||b_k|| of the last
@@ -273,6 +280,89 @@ int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int num_launches,
  * returns the average time per launch (measured copy peak). */
 int clh_measure_copy(void* d_dst, const void* d_src, size_t bytes,
                      int num_launches, float* avg_ms);
+
+/* ---- learned warm start ----------------------------------------------------
+ * A per-vertex message-passing MLP that predicts each frame's converged
+ * positions from the three previous frames, used as the first iterate of
+ * clh_solver_step (clh_step_args.d_init_guess); the unchanged iterations then
+ * run to the same residual tolerance, so the plain method stays the judge.
+ * guess = 3 x_n - 3 x_{n-1} + x_{n-2} + S(h^2 g0 s_o MLP(features)), with
+ * g0 = 9.81 m/s^2, s_o the output scale, 16 BF16 features per vertex
+ * (velocity and acceleration of the last frames in units of h^2 g0, the
+ * external acceleration over g0 when ext_feature is set, the pin flag, and
+ * 1-ring means of velocity
+ * and acceleration minus the vertex's own), two SiLU hidden layers of 64 and
+ * a linear head of 3, on BF16 MFMA with FP32 accumulation, and S optional
+ * passes of 1-ring averaging of the correction; trained on the
+ * device by the library's own forward and backward kernels with Adam on FP32
+ * master weights. Pinned vertices keep x_n (the solver sets their targets).
+ * No counterpart in the reference, which starts each frame from x_n
+ * (src/pd_solver.cu:528). All device work goes to the handle's stream (the
+ * null stream unless clh_ws_set_stream was called). */
+typedef struct clh_ws clh_ws;
+
+typedef struct clh_ws_desc {
+    int          num_verts;
+    int          num_tris;
+    const int*   d_tris;              /* [T*3] device: the triangles' edges are the message graph */
+    int          num_cloths;          /* 0 or 1: one cloth; else cloth_vert_offsets is read */
+    const int*   cloth_vert_offsets;  /* host [num_cloths + 1], as clh_batch_create */
+    const int*   d_pin_idx;           /* [P] device */
+    int          num_pinned;
+    float        dt;                  /* frame time step */
+    float        gravity;             /* along component 1, as clh_config.gravity */
+    float        ext_accel[3];        /* uniform external acceleration, as clh_config */
+    unsigned long long seed;          /* initial weights and minibatch sampling */
+    int          ext_feature;         /* 1: the external acceleration / g0 is among the
+                                         features (6..8); 0: those features are zero */
+} clh_ws_desc;
+
+int clh_ws_create(const clh_ws_desc* desc, clh_ws** out);
+int clh_ws_destroy(clh_ws* w);
+int clh_ws_set_stream(clh_ws* w, void* stream);
+/* Number of FP32 parameters: W1 [64x16], b1 [64], W2 [64x64], b2 [64],
+ * W3 [16x64] (rows 3..15 zero), b3 [16] (entries 3..15 zero), row-major. */
+int clh_ws_param_count(void);
+int clh_ws_get_params(clh_ws* w, float* out, int count, float* out_scale);
+int clh_ws_set_params(clh_ws* w, const float* in, int count, float out_scale);
+/* d_guess [N*3] from the positions of the three previous frames (device
+ * [N*3] each) and the optional per-cloth acceleration [C*3];
+ * use_network = 0 gives the extrapolation alone. */
+int clh_ws_predict(clh_ws* w, const float* d_xm2, const float* d_xm1, const float* d_x,
+                   const float* d_cloth_accel, float* d_guess, int use_network);
+/* Passes of 1-ring averaging applied to the network's correction before it
+ * is added to the extrapolation (default 0 = none). */
+int clh_ws_set_smoothing(clh_ws* w, int passes);
+/* The 16 BF16 features per vertex [N*16] (device) of a state. */
+int clh_ws_features(clh_ws* w, const float* d_xm2, const float* d_xm1, const float* d_x,
+                    const float* d_cloth_accel, unsigned short* d_features);
+/* Training set on the device: each call appends one frame (N samples padded
+ * to a multiple of 16 with weight-0 samples): features of (x_{n-2}, x_{n-1},
+ * x_n) and the target, the change of acceleration to the converged x_{n+1}
+ * over s_o. A sample's weight is its cloth's entry of d_cloth_weight [C]
+ * (device, NULL = 1; 0 leaves out a cloth whose frame did not converge);
+ * pinned vertices have weight 0. */
+int clh_ws_dataset_reserve(clh_ws* w, long long samples);
+int clh_ws_dataset_add(clh_ws* w, const float* d_xm2, const float* d_xm1, const float* d_x,
+                       const float* d_xnext, const float* d_cloth_accel,
+                       const float* d_cloth_weight);
+int clh_ws_dataset_size(clh_ws* w, long long* samples);
+int clh_ws_dataset_clear(clh_ws* w);
+/* Host copies of samples [first, first + count): features [count*16] BF16
+ * bits, targets [count*4] (3 components over s_o, weight). */
+int clh_ws_dataset_read(clh_ws* w, long long first, int count, unsigned short* features,
+                        float* targets);
+/* Sets s_o to the RMS of the weighted targets and rescales them. */
+int clh_ws_fit_output_scale(clh_ws* w, float* out_scale);
+/* Loss sum_s w_s |o_s - t_s|^2 / (3 count) over samples [first, first + count)
+ * (first and count multiples of 16) and its gradient [param_count] (host). */
+int clh_ws_loss_grad(clh_ws* w, long long first, int count, float* loss, float* grad);
+/* steps Adam steps (beta 0.9 / 0.999, eps 1e-8, learning rate decayed from lr
+ * to lr / 10 on a cosine) on minibatches of batch samples (a multiple of 16)
+ * drawn as random 16-sample blocks; losses [steps] (host, optional). */
+int clh_ws_train(clh_ws* w, int steps, int batch, float lr, float* losses);
+/* One Adam step at learning rate lr with a given gradient [param_count] (host). */
+int clh_ws_adam_step(clh_ws* w, const float* grad, float lr);
 
 #ifdef __cplusplus
 }  /* extern "C" */
