@@ -138,6 +138,9 @@ def warm_start_study(ch, n, rho, args, pos0=None, vel0=None, scene=None, gen_tol
         warm = None
         if mode != "plain":
             warm = ch.Solver(ch.config(max_iterations=cap, **common), scene.N, scene.T, scene.E)
+        for sol_m in (full, warm):
+            if sol_m is not None:
+                sol_m.autotune(scene.step_args(), 50)
         w1 = ch.WarmStart.for_scene(scene, dt, -9.81, ext_accel=CFG2_WIND)
         w1.set_params(params, scale)
         fl = FrameLoop(scene, warm or full, w1, fallback=full if warm else None)
@@ -214,6 +217,12 @@ def main():
                   ext_accel=CFG2_WIND, kernel_variant=args.variant)
     sol_tol = ch.Solver(ch.config(max_iterations=args.max_iters, tolerance=args.tol,
                                   check_every=args.check_every, **common), N, T, E)
+    rep256 = sol_tol.autotune(args_c, 200)
+    tune256 = [{"tile_verts": rep256.tile_verts[i], "block_threads": rep256.block_threads[i],
+                "us_per_launch": rep256.us_per_launch[i]} for i in range(rep256.num_candidates)]
+    log("256x256 autotune: " + ", ".join(f"{c['tile_verts']}/{c['block_threads']}: "
+                                         f"{c['us_per_launch']:.2f} us" for c in tune256))
+    s.set_state(pos0, vel0)
     log(f"256x256: N={N} T={T} E={E}; warmup {args.warmup} frames")
     for _ in range(args.warmup):
         sol_tol.step(args_c)
@@ -238,6 +247,8 @@ def main():
     # ---- 256x256: fixed 20 iterations per frame --------------------------------
     s.set_state(pos0, vel0)
     sol20 = ch.Solver(ch.config(max_iterations=20, **common), N, T, E)
+    sol20.autotune(args_c, 200)  # the cached choice
+    s.set_state(pos0, vel0)
     for _ in range(3):
         sol20.step(args_c)
     ms20 = sol20.run_frames(args_c, 50)
@@ -264,6 +275,11 @@ def main():
         sol3 = ch.Solver(ch.config(max_iterations=30, mode=ch.MODE_PD, gravity=-9.81,
                                    dt=CFG3["dt"], rho=0.99, cheb_delay=1,
                                    kernel_variant=args.variant), s3.N, s3.T, s3.E)
+        rep3 = sol3.autotune(a3, 20)
+        tune1024 = [{"tile_verts": rep3.tile_verts[i], "block_threads": rep3.block_threads[i],
+                     "us_per_launch": rep3.us_per_launch[i]} for i in range(rep3.num_candidates)]
+        log("1024x1024 autotune: " + ", ".join(f"{c['tile_verts']}/{c['block_threads']}: "
+                                               f"{c['us_per_launch']:.1f} us" for c in tune1024))
         sol3.step(a3)
         t3 = sol3.time_iteration_kernel(a3, 50)
         b3 = algorithmic_bytes(s3.N, s3.T, s3.E)
@@ -273,7 +289,8 @@ def main():
         roof_1024 = {"workload": "1024x1024 irregular mesh, 30 iterations/frame",
                      "N": s3.N, "T": s3.T, "E": s3.E, "us_per_launch": t3 * 1e3,
                      "bytes_per_launch": b3, "achieved": bw3, "frac": bw3 / HBM_PEAK_GBS,
-                     "traffic": tr3, "frames_per_s_30_iterations": 1e3 / ms3}
+                     "traffic": tr3, "frames_per_s_30_iterations": 1e3 / ms3,
+                     "autotune": tune1024}
         log(f"1024x1024 iteration: {t3 * 1e3:.1f} us, {bw3:.0f} GB/s "
             f"({100 * bw3 / HBM_PEAK_GBS:.1f}% of peak); {1e3 / ms3:.1f} frames/s at 30 iterations")
         del sol3, s3
@@ -380,7 +397,7 @@ def main():
                    "frames_per_s_fixed_20_iterations": fps20,
                    "fp64_oracle_error_20_iterations": err64,
                    "kernel_variant": int(plan.variant), "tile_verts": int(plan.tile_verts),
-                   "block_threads": int(plan.block_threads)},
+                   "block_threads": int(plan.block_threads), "autotune_256": tune256},
         "roofline": {"bound": "hbm", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": bw / HBM_PEAK_GBS, "traffic": profiled_traffic(TRAFFIC_FILE),
                      "kernel": "k_fused_iter (one PD iteration, 256x256)",

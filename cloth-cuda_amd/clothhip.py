@@ -108,6 +108,7 @@ CLH_SYMBOLS = [
     "clh_ws_dataset_reserve", "clh_ws_dataset_add", "clh_ws_dataset_size", "clh_ws_dataset_clear",
     "clh_ws_dataset_read", "clh_ws_fit_output_scale", "clh_ws_loss_grad", "clh_ws_train",
     "clh_ws_adam_step", "clh_ws_set_smoothing", "clh_solver_start_residual",
+    "clh_ws_dataset_append",
 ]
 
 
@@ -170,6 +171,7 @@ def _load():
     hip.clh_ws_dataset_reserve.argtypes = [_vp, C.c_longlong]
     hip.clh_ws_dataset_add.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
     hip.clh_ws_dataset_size.argtypes = [_vp, C.POINTER(C.c_longlong)]
+    hip.clh_ws_dataset_append.argtypes = [_vp, _vp, _vp, C.c_int]
     hip.clh_ws_dataset_clear.argtypes = [_vp]
     hip.clh_ws_dataset_read.argtypes = [_vp, C.c_longlong, C.c_int, _vp, _vp]
     hip.clh_ws_fit_output_scale.argtypes = [_vp, _fp]
@@ -654,6 +656,12 @@ class WarmStart:
         check(hip().clh_ws_dataset_add(self.h, _dptr(xm2), _dptr(xm1), _dptr(x), _dptr(xnext),
                                        _dptr(cloth_accel), _dptr(cloth_weight)),
               "clh_ws_dataset_add")
+
+    def dataset_append(self, features, targets):
+        F = np.ascontiguousarray(features, np.uint16)
+        T = np.ascontiguousarray(targets, np.float32)
+        check(hip().clh_ws_dataset_append(self.h, _ptr(F), _ptr(T), len(F)),
+              "clh_ws_dataset_append")
 
     def dataset_size(self):
         n = C.c_longlong(0)

@@ -1331,7 +1331,7 @@ extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int nu
             return CLH_OK;
         }
     }
-    const int tiles[3] = {64, 128, 256}, threads[2] = {256, 512};
+    const int tiles[4] = {32, 64, 128, 256}, threads[2] = {256, 512};
     int n = 0, best = -1;
     for (int tv : tiles) {
         int got = 0;

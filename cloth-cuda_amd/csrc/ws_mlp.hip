@@ -875,6 +875,28 @@ extern "C" int clh_ws_dataset_add(clh_ws* w, const float* xm2, const float* xm1,
     return CLH_OK;
 }
 
+extern "C" int clh_ws_dataset_append(clh_ws* w, const unsigned short* features,
+                                     const float* targets, int count)
+{
+    if (!w || count < 0 || (count > 0 && (!features || !targets)))
+        return fail(CLH_ERR_INVALID, "clh_ws_dataset_append: bad arguments");
+    const long long padded = ((long long)count + 15) / 16 * 16;
+    const long long need = w->size + padded;
+    if (need > w->cap) CLH_TRY(clh_ws_dataset_reserve(w, std::max(need, 2 * w->cap)));
+    CLH_HIP(hipStreamSynchronize(w->stream));
+    std::vector<unsigned short> F((size_t)padded * kF, 0);
+    std::vector<float4> T((size_t)padded, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    if (count > 0) {
+        std::memcpy(F.data(), features, (size_t)count * kF * sizeof(unsigned short));
+        std::memcpy(T.data(), targets, (size_t)count * sizeof(float4));
+    }
+    CLH_HIP(hipMemcpy(w->F.ptr + (size_t)w->size * kF, F.data(), F.size() * sizeof(unsigned short),
+                      hipMemcpyHostToDevice));
+    CLH_HIP(hipMemcpy(w->T.ptr + w->size, T.data(), T.size() * sizeof(float4), hipMemcpyHostToDevice));
+    w->size = need;
+    return CLH_OK;
+}
+
 extern "C" int clh_ws_dataset_size(clh_ws* w, long long* samples)
 {
     if (!w || !samples) return fail(CLH_ERR_INVALID, "clh_ws_dataset_size: null pointer");

@@ -258,8 +258,8 @@ int clh_batch_create(const clh_config* cfg, const clh_mesh_desc* mesh, int num_c
  * of cloths. */
 int clh_batch_cloth_residuals(clh_solver* s, double* out, int max_count, int* count);
 
-/* Autotuner of the fused kernel's launch shape: times every (tile_verts,
- * block_threads) candidate on this solver's mesh (num_launches iteration
+/* Autotuner of the fused kernel's launch shape: times every (tile_verts in
+ * {32, 64, 128, 256}, block_threads in {256, 512}) candidate on this solver's mesh (num_launches iteration
  * launches each, after one warm-up) and keeps the fastest. Results are
  * cached per process under a key of the vertex, triangle, hinge and cloth
  * counts, the mode and the histogram of vertex valences, so another solver
@@ -346,6 +346,11 @@ int clh_ws_dataset_reserve(clh_ws* w, long long samples);
 int clh_ws_dataset_add(clh_ws* w, const float* d_xm2, const float* d_xm1, const float* d_x,
                        const float* d_xnext, const float* d_cloth_accel,
                        const float* d_cloth_weight);
+/* Appends count samples from host arrays: features [count*16] BF16 bits and
+ * targets [count*4] (3 components over s_o, weight), padded to a multiple of
+ * 16 with weight-0 samples (a stored training set, or test vectors). */
+int clh_ws_dataset_append(clh_ws* w, const unsigned short* features, const float* targets,
+                          int count);
 int clh_ws_dataset_size(clh_ws* w, long long* samples);
 int clh_ws_dataset_clear(clh_ws* w);
 /* Host copies of samples [first, first + count): features [count*16] BF16

@@ -138,8 +138,8 @@ def test_autotune_caches_and_keeps_results():
     b.set_state(pos0, vel0)
     sol = ch.Solver(c, b.N, b.T, b.E, cloth_offsets=b.vert_off)
     rep = sol.autotune(b.step_args(), 10)
-    assert rep.num_candidates == 6 and rep.from_cache == 0
-    times = [rep.us_per_launch[i] for i in range(6)]
+    assert rep.num_candidates == 8 and rep.from_cache == 0
+    times = [rep.us_per_launch[i] for i in range(8)]
     assert rep.us_per_launch[rep.best] == min(t for t in times if t > 0)
     info = sol.plan_info()
     assert (info.tile_verts, info.block_threads) == (rep.tile_verts[rep.best],
