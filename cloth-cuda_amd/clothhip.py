@@ -223,6 +223,8 @@ def _load():
     core.cc_pdsolver_native.restype = _vp
     core.cc_pdsolver_destroy.argtypes = [_vp]
     core.cc_pdsolver_destroy.restype = None
+    core.cc_scene_energy.argtypes = [_vp, C.c_float, _vp, _vp, _vp]
+    core.cc_scene_energy.restype = None
     core.cc_scene_apply_pins.argtypes = [_vp]
     core.cc_scene_apply_pins.restype = None
     _hip, _core = hip, core
@@ -386,6 +388,14 @@ class Scene:
         if core().cc_scene_get_state(self.h, _ptr(pos), _ptr(vel)) != 0:
             raise ClothHipError("cc_scene_get_state failed")
         return pos, vel
+
+    def energy(self, pos, vel, gravity=-9.81):
+        """(kinetic, potential, stretch, bend) of a host state."""
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        out = np.zeros(4, np.float64)
+        core().cc_scene_energy(self.h, gravity, _ptr(pos), _ptr(vel), _ptr(out))
+        return out
 
     def step_args(self, stretch=True, bend=True) -> StepArgs:
         a = StepArgs()

@@ -93,6 +93,11 @@ void  cc_pdsolver_step(void* solver, cc_scene* s);
 void  cc_pdsolver_reset(void* solver);
 void* cc_pdsolver_native(void* solver);  /* clh_solver* */
 void  cc_pdsolver_destroy(void* solver);
+/* Energies of a state given as host arrays pos, vel [N*3] (no device):
+ * out[0..3] = kinetic, gravitational potential (gravity along y), stretch
+ * sum 1/2 k A |F - R|^2, bend sum 1/2 k |e - p|^2 (host/cloth_energy.h). */
+void  cc_scene_energy(const cc_scene* s, float gravity, const float* pos, const float* vel,
+                      double* out);
 /* Constraints::apply_gpu on the scene's device state. */
 void  cc_scene_apply_pins(cc_scene* s);
 

@@ -2,6 +2,7 @@
 #include "cloth_scene.h"
 
 #include "check.h"
+#include "cloth_energy.h"
 #include "cloth_mesh.h"
 #include "cloth_rng.h"
 #include "constraints.h"
@@ -429,5 +430,17 @@ extern "C" void* cc_pdsolver_native(void* solver)
 }
 
 extern "C" void cc_pdsolver_destroy(void* solver) { delete static_cast<PDSolver*>(solver); }
+
+extern "C" void cc_scene_energy(const cc_scene* s, float gravity, const float* pos,
+                                const float* vel, double* out)
+{
+    const size_t n = (size_t)s->mesh.num_verts * 3;
+    const std::vector<float> p(pos, pos + n), v(vel, vel + n);
+    const ClothEnergy e = cloth_energy(s->mesh, s->sim, gravity, p, v);
+    out[0] = e.kinetic;
+    out[1] = e.potential;
+    out[2] = e.stretch;
+    out[3] = e.bend;
+}
 
 extern "C" void cc_scene_apply_pins(cc_scene* s) { s->pins.apply_gpu(s->mesh.d_pos, s->mesh.d_vel); }
