@@ -21,6 +21,7 @@ CLH_OK = 0
 CLH_H2H, CLH_H2D, CLH_D2H, CLH_D2D = 0, 1, 2, 3
 MODE_REF_CUDA, MODE_PD = 0, 1
 KERNEL_AUTO, KERNEL_UNFUSED, KERNEL_FUSED = 0, 1, 2
+PERSIST_OFF, PERSIST_ON = 0, 1
 PIN_NONE, PIN_TOP_ROW, PIN_CORNERS, PIN_RANDOM_BOUNDARY = 0, 1, 2, 3
 MESH_IRREGULAR = 4
 
@@ -41,7 +42,7 @@ class Config(C.Structure):
                 ("damping", C.c_float), ("tolerance", C.c_float), ("check_every", C.c_int),
                 ("cheb_delay", C.c_int), ("ext_accel", C.c_float * 3),
                 ("kernel_variant", C.c_int), ("use_graph", C.c_int), ("tile_verts", C.c_int),
-                ("block_threads", C.c_int)]
+                ("block_threads", C.c_int), ("persistent", C.c_int)]
 
 
 class MeshDesc(C.Structure):
@@ -69,7 +70,8 @@ class PlanInfo(C.Structure):
     _fields_ = [("variant", C.c_int), ("num_tiles", C.c_int), ("tile_verts", C.c_int),
                 ("halo_tri_factor", C.c_double), ("halo_bend_factor", C.c_double),
                 ("bytes_per_iter", C.c_double), ("lds_bytes", C.c_double),
-                ("block_threads", C.c_int), ("num_cloths", C.c_int), ("padded_verts", C.c_int)]
+                ("block_threads", C.c_int), ("num_cloths", C.c_int), ("padded_verts", C.c_int),
+                ("persistent", C.c_int)]
 
 
 class TuneReport(C.Structure):

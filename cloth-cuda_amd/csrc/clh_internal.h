@@ -50,7 +50,7 @@ struct FrameCtl {
     int    restarts;        // Chebyshev restarts this frame
     int    converged;
     int    nonfinite;
-    int    pad;
+    int    barrier_error;   // a grid barrier of the persistent kernel timed out
     double last_r2;         // residual^2 at the previous check (-1: none)
     double tol2;            // tolerance^2
 };

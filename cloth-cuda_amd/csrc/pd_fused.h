@@ -81,6 +81,15 @@ size_t fused_lds_bytes(const FusedPlan& p, int mode, bool check);
 int  fused_enable_lds(int& bytes);
 int  fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
                             hipStream_t stream);
+
+// Persistent PD iterations [k0, k1) in one cooperative launch (device control
+// through ctl; iterates in it[k % 3]; grid barrier words bar[2], zero at
+// rest). fused_persist_fits tells whether every tile can be resident at once.
+size_t fused_persist_lds_bytes(const FusedPlan& p);
+bool fused_persist_fits(const FusedPlan& p);
+int  fused_launch_persist(const FusedPlan& p, const FusedIterArgs& a, double* const it[3], int k0,
+                          int k1, int K, int check_every, FrameCtl* ctl, double* partials,
+                          double* hist, unsigned* bar, hipStream_t stream);
 void fused_plan_info(const FusedPlan& p, clh_plan_info* out);
 void fused_plan_release(FusedPlan& p);
 

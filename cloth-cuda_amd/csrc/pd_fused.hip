@@ -434,6 +434,297 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
     if (kCheck) fused_block_sum2<kT>(rr, bb, a.partials, tile);
 }
 
+// ---- persistent PD iteration kernel ------------------------------------------
+//
+// One cooperative launch runs iterations [k0, k1) of a frame under device
+// control (FrameCtl): every tile's workgroup stays resident, keeps its element
+// records, gather list, halo ids and halo base positions in LDS and its own
+// vertices' state (x_k, x_{k-1}, constants) in registers, and meets the other
+// workgroups at a grid barrier after each iteration. Only the iterate goes
+// through memory: each iteration writes x_{k+1} of the own vertices and reads
+// x_k of the halo. A check iteration adds the fixed-order FP64 residual sum
+// and the control decision (workgroup 0) between two barriers. The arithmetic
+// and every summation order are those of k_fused_iter and k_reduce_partials,
+// so both paths give the same iterates bit for bit.
+//
+// Measured on MI355X at 256x256 this is slower than a launch per iteration:
+// 68 us per iteration with 512 workgroups and 122 us with 1024, against 7-8
+// us, growing with the number of workgroups: the device-scope arrival
+// atomics on one word serialize across the XCDs, so the kernel boundary is
+// the cheaper global barrier here. The path is therefore opt-in.
+
+struct PersistArgs {
+    FusedKernelArgs A;   // plan and frame arrays (A.it.x / xkm1 / out unused)
+    double* it[3];       // the rotating iterate buffers (tile order)
+    int k0, k1, K, check_every;
+    FrameCtl* ctl;
+    double* partials;    // [num_tiles * 2]
+    double* hist;        // residual history
+    unsigned* bar;       // arrivals, generation
+    int off_rt, off_rh, off_hb;  // LDS byte offsets: tri records, hinge records, halo ids + base
+};
+
+constexpr long kSpinLimit = 1l << 24;
+
+// Grid-wide barrier of a cooperative launch: release of this workgroup's
+// writes at agent scope, arrival count, and a wait on the generation word;
+// acquire on the way out. A wait longer than kSpinLimit sleeps raises the
+// error flag `err`, which every later barrier honours, so a stuck launch
+// still ends. Returns false once the flag is up.
+__device__ __forceinline__ bool grid_barrier(unsigned* bar, int* err, unsigned nblocks, int* sflag)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned arrived =
+            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        if (arrived == nblocks) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (++spins > kSpinLimit) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __threadfence();
+        *sflag = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 1 : 0;
+    }
+    __syncthreads();
+    return *sflag == 0;
+}
+
+__device__ __forceinline__ int ctl_load_int(const int* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int kT>
+__global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
+{
+    extern __shared__ double smem[];
+    __shared__ double red_r[kT], red_b[kT];
+    __shared__ int s_flag;
+    const FusedKernelArgs& A = P.A;
+    const FusedIterArgs& a = A.it;
+    const int tid = threadIdx.x;
+    const int L = A.lmax;
+    int tile = blockIdx.x;
+    if (A.xcd_remap) {
+        const int per = A.num_tiles >> 3;
+        tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    }
+    const int vbeg = tile * A.tile_verts;
+    const int nown = min(A.tile_verts, A.N - vbeg);
+
+    char* lds = reinterpret_cast<char*>(smem);
+    double* sX = smem;
+    unsigned* sgw = reinterpret_cast<unsigned*>(lds + A.off_g);
+    const unsigned short* sg = reinterpret_cast<const unsigned short*>(sgw);
+    float* st = reinterpret_cast<float*>(lds + A.off_r);
+    float* st2 = st + A.tmax * 9;
+    float* skw = st;
+    float* sq = skw + A.hmax * 4;
+    float* sp = sq + A.hmax * 3;
+    uint2* rti = reinterpret_cast<uint2*>(lds + P.off_rt);
+    float4* rtg = reinterpret_cast<float4*>(rti + A.tmax + (A.tmax & 1));
+    float* rtw = reinterpret_cast<float*>(rtg + A.tmax);
+    uint2* rhi = reinterpret_cast<uint2*>(lds + P.off_rh);
+    float4* rhw = reinterpret_cast<float4*>(rhi + A.hmax + (A.hmax & 1));
+    float2* rhc = reinterpret_cast<float2*>(rhw + A.hmax);
+    int* shid = reinterpret_cast<int*>(lds + P.off_hb);
+    float* shb = reinterpret_cast<float*>(shid + A.hstride);
+
+    // ---- setup: tile data into LDS, own vertices into registers --------------
+    const int* ti = A.tile_info + (size_t)tile * kTileInfo;
+    const int nhalo = ti[0], tbeg = ti[1], ntri = ti[2], ebeg = ti[3], nhin = ti[4];
+    const int gbeg = ti[5], glen = ti[6];
+    for (int j = tid; j < ntri; j += kT) {
+        rti[j] = A.tri_idx[tbeg + j];
+        rtg[j] = A.tri_c0[tbeg + j];
+        rtw[j] = A.tri_c1[tbeg + j];
+    }
+    for (int j = tid; j < nhin; j += kT) {
+        rhi[j] = A.hin_idx[ebeg + j];
+        rhw[j] = A.hin_w[ebeg + j];
+        rhc[j] = A.hin_c[ebeg + j];
+    }
+    for (int j = tid; j < (glen + 1) >> 1; j += kT) sgw[j] = A.gl_words[gbeg + j];
+    const int* hids = A.halo + (size_t)tile * A.hstride;
+    for (int i = tid; i < nhalo; i += kT) {
+        const int hg = hids[i];
+        shid[i] = hg;
+        const f3 hb = ld3(a.base, hg);
+        shb[3 * i + 0] = hb.x;
+        shb[3 * i + 1] = hb.y;
+        shb[3 * i + 2] = hb.z;
+    }
+    // own vertices: constants in LDS, the two FP64 iterates in registers
+    int4* sov = reinterpret_cast<int4*>(shb + 3 * A.hstride);  // 16 * hstride bytes in
+    float* sob = reinterpret_cast<float*>(sov + A.tile_verts);
+    float* soy = sob + 3 * A.tile_verts;
+    const bool own = tid < nown;
+    const size_t g = (size_t)vbeg + (own ? tid : 0);
+    d3 od = d3{0.0, 0.0, 0.0}, oxp = d3{0.0, 0.0, 0.0};
+    if (own) {
+        sov[tid] = a.vinfo[g];
+        sts3(sob, tid, ld3(a.base, g));
+        sts3(soy, tid, ld3(a.y, g));
+        od = ld3d(P.it[P.k0 % 3], g);
+        oxp = ld3d(P.it[(P.k0 + 2) % 3], g);
+    }
+    __syncthreads();
+
+    for (int k = P.k0; k < P.k1; ++k) {
+        // ---- control state of iteration k (written before the last barrier)
+        if (ctl_load_int(&P.ctl->stop)) break;
+        const int par = k & 1;
+        CtlSlot cs;
+        cs.omega = __int_as_float(ctl_load_int(reinterpret_cast<const int*>(&P.ctl->slot[par].omega)));
+        cs.blend = ctl_load_int(&P.ctl->slot[par].blend);
+        cs.ck = ctl_load_int(&P.ctl->slot[par].ck);
+        cs.k = ctl_load_int(&P.ctl->slot[par].k);
+        ctl_write_next(P.ctl, par, cs);
+        const bool check = ((k + 1) % P.check_every == 0) || (k == P.K - 1);
+        const double* xk = P.it[k % 3];
+        double* out = P.it[(k + 1) % 3];
+
+        // ---- positions of the tile: own from registers, halo from memory
+        for (int i = tid; i < nhalo; i += kT) {
+            const d3 hx = ld3d(xk, shid[i]);
+            const d3 hb = d3{(double)shb[3 * i], (double)shb[3 * i + 1], (double)shb[3 * i + 2]};
+            stsX(sX, L, nown + i, hb + hx);
+        }
+        if (own) stsX(sX, L, tid, to_d(lds3(sob, tid)) + od);
+        __syncthreads();
+
+        // ---- triangles -> corner forces
+        if (a.use_stretch) {
+            for (int j = tid; j < ntri; j += kT) {
+                const uint2 id = rti[j];
+                const int l0 = id.x & 0xFFFF, l1 = id.x >> 16, l2 = id.y & 0xFFFF;
+                f3 c[3], cr[3];
+                pd_triangle(ldsX(sX, L, l0), ldsX(sX, L, l1), ldsX(sX, L, l2), rtg[j], rtw[j], cr,
+                            c, check);
+                if (check) {
+                    float* q = st2 + (size_t)j * 9;
+                    sts3(q, 0, cr[0]);
+                    sts3(q, 1, cr[1]);
+                    sts3(q, 2, cr[2]);
+                }
+                float* o = st + (size_t)j * 9;
+                sts3(o, 0, c[0]);
+                sts3(o, 1, c[1]);
+                sts3(o, 2, c[2]);
+            }
+        }
+        __syncthreads();
+        const int4 vi = own ? sov[tid] : make_int4(0, 0, -1, 0);
+        const unsigned meta = (unsigned)vi.w;
+        const int j0 = (int)(meta & 0xFFFFu), jt = j0 + (int)((meta >> 16) & 0xFFu),
+                  j1 = j0 + (int)(meta >> 24);
+        const bool gather = own && vi.z < 0;
+        f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
+        if (gather && a.use_stretch) {
+            for (int j = j0; j < jt; ++j) {
+                const int s = sg[j];
+                ar = ar + lds3(st, s);
+                if (check) ab = ab + lds3(st2, s);
+            }
+        }
+        if (a.use_bend) {
+            __syncthreads();
+            for (int j = tid; j < nhin; j += kT) {
+                const uint2 id = rhi[j];
+                const float4 w = rhw[j];
+                const float2 c = rhc[j];
+                const int q0 = id.x & 0xFFFF, q1 = id.x >> 16, q2 = id.y & 0xFFFF, q3 = id.y >> 16;
+                const d3 X[4] = {ldsX(sX, L, q0), ldsX(sX, L, q1), ldsX(sX, L, q2), ldsX(sX, L, q3)};
+                f3 pp, pe;
+                pd_hinge(X, w, c.x, pp, pe);
+                skw[j] = c.y * w.x;
+                skw[A.hmax + j] = c.y * w.y;
+                skw[2 * A.hmax + j] = c.y * w.z;
+                skw[3 * A.hmax + j] = c.y * w.w;
+                sts3(sq, j, pe);
+                if (check) sts3(sp, j, pp);
+            }
+            __syncthreads();
+            if (gather) {
+                for (int j = jt; j < j1; ++j) {
+                    const int s = sg[j];
+                    const float kw = skw[(s & 3) * A.hmax + (s >> 2)];
+                    const f3 d = lds3(sq, s >> 2);
+                    ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+                    if (check) {
+                        const f3 q = lds3(sp, s >> 2);
+                        ab = ab + mk(kw * q.x, kw * q.y, kw * q.z);
+                    }
+                }
+            }
+        }
+
+        // ---- own vertices: update, pins, residual
+        double rr = 0.0, bb = 0.0;
+        if (own) {
+            const f3 ob = lds3(sob, tid);
+            const float om = __int_as_float(vi.x), odg = __int_as_float(vi.y);
+            d3 xn;
+            if (vi.z >= 0) {
+                xn = to_d(ld3(a.pin_target, vi.z)) - to_d(ob);
+            } else {
+                const f3 oy = lds3(soy, tid);
+                const double md = om;
+                const d3 r = d3{(double)ar.x + md * ((double)oy.x - od.x),
+                                (double)ar.y + md * ((double)oy.y - od.y),
+                                (double)ar.z + md * ((double)oy.z - od.z)};
+                const double inv = (odg > 1e-10f) ? 1.0 / (double)odg : 0.0;
+                xn = od + r * inv;
+                if (cs.blend) xn = (xn - oxp) * (double)cs.omega + oxp;
+                if (check) {
+                    const f3 ya = ob + oy;
+                    const f3 b = mk(ab.x + om * ya.x, ab.y + om * ya.y, ab.z + om * ya.z);
+                    rr = r.x * r.x + r.y * r.y + r.z * r.z;
+                    bb = (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
+                }
+            }
+            st3d(out, g, xn);
+            oxp = od;
+            od = xn;
+        }
+        if (check) fused_block_sum2<kT>(rr, bb, P.partials, tile);
+        if (!grid_barrier(P.bar, &P.ctl->barrier_error, gridDim.x, &s_flag)) break;
+        if (check) {
+            if (blockIdx.x == 0) {
+                // fixed-order sum of the tile partials (k_reduce_partials)
+                double r = 0.0, b = 0.0;
+                for (int i = tid; i < A.num_tiles; i += kT) {
+                    r += P.partials[2 * i + 0];
+                    b += P.partials[2 * i + 1];
+                }
+                red_r[tid] = r;
+                red_b[tid] = b;
+                __syncthreads();
+                for (int s2 = kT / 2; s2 > 0; s2 >>= 1) {
+                    if (tid < s2) {
+                        red_r[tid] += red_r[tid + s2];
+                        red_b[tid] += red_b[tid + s2];
+                    }
+                    __syncthreads();
+                }
+                if (tid == 0) ctl_check(P.ctl, par, red_r[0], red_b[0], P.hist, red_r[0] <= P.ctl->tol2 * red_b[0]);
+            }
+            if (!grid_barrier(P.bar, &P.ctl->barrier_error, gridDim.x, &s_flag)) break;
+        }
+    }
+}
+
 // ---- tile-order frame kernels ------------------------------------------------
 
 // Tile-order padding (perm < 0) gets a zero state and an inert record: no
@@ -578,6 +869,27 @@ LdsLayout lds_layout(const FusedPlan& p, int mode, bool check)
 }
 
 size_t lds_bytes(const FusedPlan& p, int mode, bool check) { return lds_layout(p, mode, check).total; }
+
+// LDS of the persistent kernel: the iteration kernel's layout (PD, residual
+// variant), then the tile's triangle and hinge records, halo ids and halo
+// base positions.
+struct PersistLayout {
+    LdsLayout it;
+    size_t off_rt, off_rh, off_hb, total;
+};
+
+PersistLayout persist_layout(const FusedPlan& p)
+{
+    PersistLayout o;
+    o.it = lds_layout(p, CLH_MODE_PD, true);
+    const size_t t = (size_t)p.tmax, h = (size_t)p.hmax;
+    o.off_rt = align16(o.it.total);
+    o.off_rh = align16(o.off_rt + (t + (t & 1)) * 8 + t * 16 + t * 4);
+    o.off_hb = align16(o.off_rh + (h + (h & 1)) * 8 + h * 16 + h * 8);
+    // halo ids and base positions, then the own vertices' records, base and y
+    o.total = align16(o.off_hb + (size_t)p.hstride * 16 + (size_t)p.tile_verts * (16 + 12 + 12));
+    return o;
+}
 
 unsigned blocks_for(int n) { return (unsigned)((n + 255) / 256); }
 
@@ -951,6 +1263,72 @@ int fused_pack_constants(FusedPlan& p, const clh_step_args& a, float h2, hipStre
 
 int fused_max_blocks(const FusedPlan& p) { return p.num_tiles; }
 
+size_t fused_persist_lds_bytes(const FusedPlan& p) { return persist_layout(p).total; }
+
+bool fused_persist_fits(const FusedPlan& p)
+{
+    if (p.threads != 256 || p.num_tiles <= 0) return false;
+    int dev = 0, cus = 0, coop = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (!coop) return false;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_fused_persist<256>, 256,
+                                                     fused_persist_lds_bytes(p)) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return (long long)per * cus >= p.num_tiles;
+}
+
+int fused_launch_persist(const FusedPlan& p, const FusedIterArgs& a, double* const it[3], int k0,
+                         int k1, int K, int check_every, FrameCtl* ctl, double* partials,
+                         double* hist, unsigned* bar, hipStream_t stream)
+{
+    PersistArgs P;
+    FusedKernelArgs& A = P.A;
+    A.it = a;
+    A.tile_info = p.tile_info.ptr;
+    A.halo = p.halo.ptr;
+    A.tri_idx = p.tri_idx.ptr;
+    A.tri_c0 = p.tri_c0.ptr;
+    A.tri_c1 = p.tri_c1.ptr;
+    A.hin_idx = p.hin_idx.ptr;
+    A.hin_w = p.hin_w.ptr;
+    A.hin_c = p.hin_c.ptr;
+    A.gl_words = p.gl_words.ptr;
+    A.N = p.N;
+    A.tile_verts = p.tile_verts;
+    A.hstride = p.hstride;
+    A.lmax = p.lmax;
+    A.tmax = p.tmax;
+    A.hmax = p.hmax;
+    A.num_tiles = p.num_tiles;
+    A.xcd_remap = p.xcd_remap;
+    const PersistLayout L = persist_layout(p);
+    A.off_g = (int)L.it.off_g;
+    A.off_r = (int)L.it.off_r;
+    for (int i = 0; i < 3; ++i) P.it[i] = it[i];
+    P.k0 = k0;
+    P.k1 = k1;
+    P.K = K;
+    P.check_every = check_every;
+    P.ctl = ctl;
+    P.partials = partials;
+    P.hist = hist;
+    P.bar = bar;
+    P.off_rt = (int)L.off_rt;
+    P.off_rh = (int)L.off_rh;
+    P.off_hb = (int)L.off_hb;
+    void* args[] = {&P};
+    return hip_status(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_fused_persist<256>),
+                                                 dim3(p.num_tiles), dim3(256), args, L.total, stream),
+                      "persistent iteration launch");
+}
+
 int fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
                            hipStream_t stream)
 {
@@ -1011,7 +1389,8 @@ int fused_enable_lds(int& bytes)
                          reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false, 256>),
                          reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 512>),
                          reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 512>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false, 512>)};
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false, 512>),
+                         reinterpret_cast<const void*>(&k_fused_persist<256>)};
     for (const void* f : fns) {
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {
             (void)hipGetLastError();

@@ -430,7 +430,7 @@ __global__ void k_ctl_frame_init(FrameCtl* __restrict__ c, float rho, int use_ch
     f.restarts = 0;
     f.converged = 0;
     f.nonfinite = 0;
-    f.pad = 0;
+    f.barrier_error = 0;
     f.last_r2 = -1.0;
     f.tol2 = tol2;
     *c = f;
@@ -522,6 +522,8 @@ struct clh_solver {
     std::vector<int> seg;        // cloth vertex offsets [C+1] (batched solver), else empty
     int num_cloths = 1;
     unsigned long long shape_key = 0;  // autotuner cache key of the mesh shape
+    bool persist = false;        // tolerance frames in persistent cooperative launches
+    DevBuf<unsigned> bar;        // grid barrier words of the persistent kernel
     DevBuf<double> cloth_res;    // per-cloth (r^2, b^2) of the last residual evaluation
     int variant = CLH_KERNEL_UNFUSED;
     int tile_verts = 128;
@@ -650,6 +652,8 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
     }
     if (s->num_cloths > 1 && s->variant != CLH_KERNEL_FUSED)
         return fail(CLH_ERR_UNSUPPORTED, "batched cloths need the fused kernel variant");
+    s->persist = s->cfg.persistent == CLH_PERSIST_ON && s->variant == CLH_KERNEL_FUSED &&
+                 s->cfg.mode == CLH_MODE_PD && s->num_cloths == 1 && fused_persist_fits(s->fused);
 
     s->plan_tris = a.d_tris;
     s->plan_quads = a.d_bend_quads;
@@ -673,6 +677,10 @@ int ensure_scratch(clh_solver* s)
         CLH_TRY(s->vinfo.resize(Nt));
     }
     CLH_TRY(s->cloth_res.resize((size_t)s->num_cloths * 2));
+    if (!s->bar.ptr) {
+        CLH_TRY(s->bar.resize(2));
+        CLH_HIP(hipMemset(s->bar.ptr, 0, 2 * sizeof(unsigned)));
+    }
     CLH_TRY(s->fr.resize(T * 9));
     CLH_TRY(s->bp.resize(E * 3));
     if (s->cfg.mode == CLH_MODE_PD) {
@@ -978,7 +986,18 @@ int step_impl(clh_solver* s, const clh_step_args& a)
     };
     auto issue = [&](int ci) -> int {
         const int k0 = ci * P, k1 = std::min(K, k0 + P);
-        if (graphs && k1 - k0 == P) {
+        if (s->persist && !a.d_trace) {
+            FusedIterArgs f = {};
+            f.base = s->xprev.ptr;
+            f.y = s->y.ptr;
+            f.vinfo = s->vinfo.ptr;
+            f.pin_target = a.d_pin_target;
+            f.use_stretch = (s->T > 0 && a.d_stretch_k != nullptr) ? 1 : 0;
+            f.use_bend = (s->E > 0 && a.d_bend_quads != nullptr) ? 1 : 0;
+            double* const its[3] = {s->it[0].ptr, s->it[1].ptr, s->it[2].ptr};
+            CLH_TRY(fused_launch_persist(s->fused, f, its, k0, k1, K, P, s->ctl.ptr,
+                                         s->partials.ptr, s->hist.ptr, s->bar.ptr, s->stream));
+        } else if (graphs && k1 - k0 == P) {
             hipGraphExec_t& ex = s->chunk_exec[k0 % 6];
             if (!ex) {
                 CLH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -1012,6 +1031,7 @@ int step_impl(clh_solver* s, const clh_step_args& a)
     for (int ci = 0; ci < nchunks; ++ci) {
         CLH_HIP(hipEventSynchronize(s->ev_chunk[ci & 1]));
         const FrameCtl h = s->h_ctl[ci & 1];
+        if (h.barrier_error) return fail(CLH_ERR_DEVICE, "persistent kernel: a grid barrier timed out");
         checks = h.checks;
         if (h.stop) {
             iters = h.iters;
@@ -1235,6 +1255,7 @@ extern "C" int clh_solver_plan_info(clh_solver* s, clh_plan_info* out)
     std::memset(out, 0, sizeof(*out));
     out->variant = s->plan_ready ? s->variant : CLH_KERNEL_AUTO;
     if (s->plan_ready && s->variant == CLH_KERNEL_FUSED) fused_plan_info(s->fused, out);
+    out->persistent = s->plan_ready && s->persist ? 1 : 0;
     return CLH_OK;
 }
 

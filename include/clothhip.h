@@ -112,7 +112,13 @@ typedef struct clh_config {
     int   use_graph;        /* capture fixed-length frames in a hipGraph        */
     int   tile_verts;       /* fused variant: vertices per tile (0 = default)   */
     int   block_threads;    /* fused variant: workgroup size, 256 or 512 (0 = default) */
+    int   persistent;       /* CLH_PERSIST_ON: frames to a tolerance in one cooperative
+                               launch per chunk (grid barrier between iterations) when
+                               every tile fits resident; off by default: on MI355X the
+                               barrier costs more than a launch */
 } clh_config;
+
+enum { CLH_PERSIST_OFF = 0, CLH_PERSIST_ON = 1 };
 
 /* Fills *cfg with the defaults of PDSolverConfig (include/pd_solver.h:5-14)
  * and CLH_MODE_REF_CUDA. */
@@ -223,6 +229,7 @@ typedef struct clh_plan_info {
     int    block_threads;    /* workgroup size of the fused kernel */
     int    num_cloths;       /* cloths in the plan (1 unless batched) */
     int    padded_verts;     /* tile-order length: each cloth padded to whole tiles */
+    int    persistent;       /* frames to a tolerance use the persistent kernel */
 } clh_plan_info;
 int clh_solver_plan_info(clh_solver* s, clh_plan_info* out);
 

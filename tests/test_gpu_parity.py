@@ -395,3 +395,32 @@ def test_fused_matches_unfused(mode, spec):
     L = max(extent(pos), 1.0)
     assert rel_err(outs[1][0], outs[0][0], L) < 1e-6
     assert rel_err(outs[1][1], outs[0][1], L / REF_CASES[spec]["dt"]) < 1e-6
+
+
+@pytest.mark.parametrize("spec", [dict(CFG1, pin_mode=ch.PIN_TOP_ROW), CFG2])
+def test_persistent_frames_equal_per_iteration_frames(spec):
+    """Frames to a tolerance in persistent cooperative launches (grid barrier
+    between iterations, residual reduction and control in workgroup 0) equal
+    the per-iteration launches bit for bit: positions, velocities, iteration
+    counts and residual histories."""
+    out = {}
+    for persist in (ch.PERSIST_ON, ch.PERSIST_OFF):
+        s, _ = make(spec)
+        cfg = pd_cfg(20000, spec["dt"], rho=0.999, tolerance=1e-5, check_every=25,
+                     persistent=persist)
+        sol = solver_for(s, **cfg)
+        args = s.step_args()
+        its, hist = [], []
+        for _ in range(3):
+            sol.step(args)
+            its.append(sol.stats().iterations)
+            hist.append(sol.residual_history(2000))
+        out[persist] = (s.get_state(), its, hist, sol.plan_info().persistent)
+    (pa, va), ita, ha, pera = out[ch.PERSIST_ON]
+    (pb, vb), itb, hb, perb = out[ch.PERSIST_OFF]
+    assert pera == 1 and perb == 0
+    assert ita == itb
+    for x, y in zip(ha, hb):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(va, vb)
