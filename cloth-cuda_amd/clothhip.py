@@ -71,7 +71,8 @@ class PlanInfo(C.Structure):
                 ("halo_tri_factor", C.c_double), ("halo_bend_factor", C.c_double),
                 ("bytes_per_iter", C.c_double), ("lds_bytes", C.c_double),
                 ("block_threads", C.c_int), ("num_cloths", C.c_int), ("padded_verts", C.c_int),
-                ("persistent", C.c_int)]
+                ("persistent", C.c_int), ("tri_max", C.c_int), ("hinge_max", C.c_int),
+                ("halo_max", C.c_int)]
 
 
 class TuneReport(C.Structure):

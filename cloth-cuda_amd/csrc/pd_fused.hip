@@ -1467,6 +1467,9 @@ void fused_plan_info(const FusedPlan& p, clh_plan_info* out)
 {
     out->num_cloths = p.num_cloths;
     out->padded_verts = p.N;
+    out->tri_max = p.tmax;
+    out->hinge_max = p.hmax;
+    out->halo_max = p.hstride;
     out->num_tiles = p.num_tiles;
     out->tile_verts = p.tile_verts;
     out->halo_tri_factor = p.tri_factor;

@@ -230,6 +230,9 @@ typedef struct clh_plan_info {
     int    num_cloths;       /* cloths in the plan (1 unless batched) */
     int    padded_verts;     /* tile-order length: each cloth padded to whole tiles */
     int    persistent;       /* frames to a tolerance use the persistent kernel */
+    int    tri_max;          /* largest triangle count of a tile (halo included) */
+    int    hinge_max;        /* largest hinge count of a tile */
+    int    halo_max;         /* largest halo of a tile */
 } clh_plan_info;
 int clh_solver_plan_info(clh_solver* s, clh_plan_info* out);
 
