@@ -14,7 +14,7 @@ config.warm_start compares the plain start with the second-order
 extrapolation and the learned warm start (trained here, on the GPU, by the
 library's own kernels, on frames the batched solver generates for 64 cloths
 with random wind) on the same workload, at 256x256 and at 64x64, each frame
-to the same tolerance with the plain start as judge and fallback.
+to the same tolerance with the plain start as arbiter and fallback.
 
 The roofline object describes the kernel that dominates the run, the fused
 per-iteration kernel at 256x256: algorithmic bytes 56 N + 32 T + 40 E per
@@ -340,7 +340,7 @@ def main():
                                          gen_tol=1e-5, gen_mode="plain"),
               "at_64": warm_start_study(ch, 64, 0.9999, args),
               "at_32": warm_start_study(ch, 32, 0.999, args)}
-        ws["plain_is_judge_and_fallback"] = (
+        ws["plain_start_fallback"] = (
             "a warm-started frame runs under a cap of 1.5x the plain frames' mean iterations; a "
             "frame missing the tolerance is redone from (x_n, v_n) with the plain start, and "
             "3 misses in a row pause the warm start for 20 frames")

@@ -8,7 +8,7 @@ and the frame then iterates to the solver's residual tolerance, so every
 mode ends at the same accuracy. All state stays on the device: the previous
 positions live in three rotating buffers, one device-to-device copy per frame.
 
-The plain method is both the judge and the fallback: with a fallback solver
+The plain method is both the arbiter and the fallback: with a fallback solver
 (the same configuration, plain start, full iteration cap) a warm-started
 frame runs under the warm solver's own smaller cap, and a frame that misses
 the tolerance within it is redone from (x_n, v_n) by the fallback solver, so

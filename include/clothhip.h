@@ -187,7 +187,7 @@ typedef struct clh_step_stats {
 /* FP64 relative residual ||b - K x_0|| / ||b|| of the frame's first iterate
  * (x_0 = a->d_init_guess, or x_n when it is NULL; pinned vertices at their
  * targets) without stepping: the mesh state is left as it is. CLH_MODE_PD.
- * Lets a caller judge candidate starts by the measure the frame stops on.
+ * Lets a caller compare candidate starts by the measure the frame stops on.
  * No counterpart in the reference. */
 int clh_solver_start_residual(clh_solver* s, const clh_step_args* a, double* rel);
 
@@ -295,7 +295,7 @@ int clh_measure_copy(void* d_dst, const void* d_src, size_t bytes,
  * A per-vertex message-passing MLP that predicts each frame's converged
  * positions from the three previous frames, used as the first iterate of
  * clh_solver_step (clh_step_args.d_init_guess); the unchanged iterations then
- * run to the same residual tolerance, so the plain method stays the judge.
+ * run to the same residual tolerance, so the plain method stays the arbiter.
  * guess = 3 x_n - 3 x_{n-1} + x_{n-2} + S(h^2 g0 s_o MLP(features)), with
  * g0 = 9.81 m/s^2, s_o the output scale, 16 BF16 features per vertex
  * (velocity and acceleration of the last frames in units of h^2 g0, the

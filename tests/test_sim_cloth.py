@@ -81,7 +81,7 @@ def test_sim_cloth_run_exports_frames(tmp_path):
     bend 1): PLY frames in the reference's format, the pinned row stays at
     y = 0 while the free vertices move off it (the reference's qualitative
     check, "pinned row y stays 0, free vertices fall and swing back",
-    DESIGN.md:274-277 of the reference), and the appended PD mode reports its
+    the reference's DESIGN.md:274-277), and the appended PD mode reports its
     iterations and residuals."""
     out = tmp_path / "frames"
     out.mkdir()
