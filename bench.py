@@ -17,12 +17,16 @@ with random wind) on the same workload, at 256x256 and at 64x64, each frame
 to the same tolerance with the plain start as arbiter and fallback.
 
 The roofline object describes the kernel that dominates the run, the fused
-per-iteration kernel at 256x256: algorithmic bytes 56 N + 32 T + 40 E per
-launch over its average launch time, measured here with HIP events on the
-solver's stream, against the MI355X HBM peak of 8 TB/s. `traffic` is the HBM
+iteration kernel at 256x256 in the launch shape the autotuner chose (one or
+two iterations per launch): algorithmic bytes per launch, 56 N + 32 T + 40 E
+for one iteration and 68 N + 32 T + 40 E for two, over its average launch
+time, measured here with HIP events on the solver's stream, against the
+MI355X HBM peak of 8 TB/s. `traffic` is the HBM
 bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a copy
-kernel), read from the committed profile summary. The same figures for the
-1024x1024 irregular mesh are reported under roofline.mesh_1024.
+kernel), read from the committed profile summary for the launch shape the
+autotuner chose (null when that shape was not profiled). The same figures for
+the 1024x1024 irregular mesh and the 512-cloth batch are reported under
+roofline.mesh_1024 and roofline.batched_512.
 """
 import argparse
 import json
@@ -40,24 +44,45 @@ os.environ.setdefault("CLOTH_QUIET", "1")
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_iter_256x256_pd_traffic.md")
 TRAFFIC_FILE_1024 = os.path.join(ROOT, "profiles", "fused_iter_1024x1024_pd_traffic.md")
+TRAFFIC_FILE_BATCH = os.path.join(ROOT, "profiles", "fused_iter_batch512_pd_traffic.md")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(N, T, E):
-    return 56 * N + 32 * T + 40 * E
+def algorithmic_bytes(N, T, E, iters_per_launch=1):
+    """Bytes one launch of the iteration kernel must move at the least: per
+    vertex x_k, x_{k-1} and y (12 B each), mass and Jacobi diagonal (4 B each)
+    read once and 12 B written per iteration; per triangle 32 B and per hinge
+    40 B of constants and indices read once. 56 N + 32 T + 40 E for one
+    iteration per launch, 68 N + 32 T + 40 E for two."""
+    return (44 + 12 * iters_per_launch) * N + 32 * T + 40 * E
 
 
-def profiled_traffic(path):
-    """HBM bytes per launch recorded in a committed rocprofv3 summary."""
+def tune_line(cands, digits):
+    return ", ".join(f"{c['tile_verts']}/{c['block_threads']}x{c['iters_per_launch']}: "
+                     f"{c['us_per_iteration']:.{digits}f}" for c in cands) + " us/iteration"
+
+
+def profiled_traffic(path, plan):
+    """HBM bytes per launch of the plan's launch shape (tile vertices, block
+    threads, iterations per launch), as recorded in a committed rocprofv3
+    summary."""
     try:
         with open(path) as f:
-            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
-        return float(m.group(1)) if m else None
+            text = f.read()
     except OSError:
         return None
+    m = re.search(r"traffic_bytes_per_launch\[%d,%d,%d\]:\s*([0-9.eE+]+)"
+                  % (plan.tile_verts, plan.block_threads, plan.iters_per_launch), text)
+    return float(m.group(1)) if m else None
+
+
+def kernel_name(plan, mesh):
+    if plan.iters_per_launch == 2:
+        return f"k_fused_pair (two PD iterations per launch, {mesh})"
+    return f"k_fused_iter (one PD iteration per launch, {mesh})"
 
 
 def build_scene(spec):
@@ -218,10 +243,8 @@ def main():
     sol_tol = ch.Solver(ch.config(max_iterations=args.max_iters, tolerance=args.tol,
                                   check_every=args.check_every, **common), N, T, E)
     rep256 = sol_tol.autotune(args_c, 200)
-    tune256 = [{"tile_verts": rep256.tile_verts[i], "block_threads": rep256.block_threads[i],
-                "us_per_launch": rep256.us_per_launch[i]} for i in range(rep256.num_candidates)]
-    log("256x256 autotune: " + ", ".join(f"{c['tile_verts']}/{c['block_threads']}: "
-                                         f"{c['us_per_launch']:.2f} us" for c in tune256))
+    tune256 = rep256.candidates()
+    log("256x256 autotune: " + tune_line(tune256, 2))
     s.set_state(pos0, vel0)
     log(f"256x256: N={N} T={T} E={E}; warmup {args.warmup} frames")
     for _ in range(args.warmup):
@@ -255,13 +278,15 @@ def main():
     fps20 = 50 / (ms20 / 1e3)
     log(f"fixed 20 iterations: {fps20:.1f} frames/s")
 
-    # ---- roofline of the per-iteration kernel ------------------------------------
-    t_iter = sol20.time_iteration_kernel(args_c, 500)
-    b_iter = algorithmic_bytes(N, T, E)
-    bw = b_iter / (t_iter * 1e-3) / 1e9
+    # ---- roofline of the iteration kernel -----------------------------------------
+    t_iter = sol20.time_iteration_kernel(args_c, 500)  # ms per launch
     plan = sol20.plan_info()
-    log(f"iteration kernel: {t_iter * 1e3:.2f} us, {bw:.0f} GB/s algorithmic "
-        f"(variant {plan.variant}, tile {plan.tile_verts}, {plan.block_threads} threads)")
+    ipl = plan.iters_per_launch
+    b_iter = algorithmic_bytes(N, T, E, ipl)
+    bw = b_iter / (t_iter * 1e-3) / 1e9
+    log(f"iteration kernel: {t_iter * 1e3:.2f} us per launch of {ipl} iteration(s), "
+        f"{bw:.0f} GB/s algorithmic (variant {plan.variant}, tile {plan.tile_verts}, "
+        f"{plan.block_threads} threads)")
     copy_ms = ch.measure_copy(256 << 20, 20)
     copy_gbs = 2 * (256 << 20) / (copy_ms * 1e-3) / 1e9
     log(f"copy probe: {copy_gbs:.0f} GB/s")
@@ -276,23 +301,27 @@ def main():
                                    dt=CFG3["dt"], rho=0.99, cheb_delay=1,
                                    kernel_variant=args.variant), s3.N, s3.T, s3.E)
         rep3 = sol3.autotune(a3, 20)
-        tune1024 = [{"tile_verts": rep3.tile_verts[i], "block_threads": rep3.block_threads[i],
-                     "us_per_launch": rep3.us_per_launch[i]} for i in range(rep3.num_candidates)]
-        log("1024x1024 autotune: " + ", ".join(f"{c['tile_verts']}/{c['block_threads']}: "
-                                               f"{c['us_per_launch']:.1f} us" for c in tune1024))
+        tune1024 = rep3.candidates()
+        log("1024x1024 autotune: " + tune_line(tune1024, 1))
         sol3.step(a3)
         t3 = sol3.time_iteration_kernel(a3, 50)
-        b3 = algorithmic_bytes(s3.N, s3.T, s3.E)
+        pi3 = sol3.plan_info()
+        b3 = algorithmic_bytes(s3.N, s3.T, s3.E, pi3.iters_per_launch)
         ms3 = sol3.run_frames(a3, 5) / 5
-        tr3 = profiled_traffic(TRAFFIC_FILE_1024)
+        tr3 = profiled_traffic(TRAFFIC_FILE_1024, pi3)
         bw3 = b3 / (t3 * 1e-3) / 1e9
         roof_1024 = {"workload": "1024x1024 irregular mesh, 30 iterations/frame",
+                     "kernel": kernel_name(pi3, "1024x1024"),
                      "N": s3.N, "T": s3.T, "E": s3.E, "us_per_launch": t3 * 1e3,
+                     "iters_per_launch": pi3.iters_per_launch,
+                     "us_per_iteration": t3 * 1e3 / pi3.iters_per_launch,
                      "bytes_per_launch": b3, "achieved": bw3, "frac": bw3 / HBM_PEAK_GBS,
                      "traffic": tr3, "frames_per_s_30_iterations": 1e3 / ms3,
+                     "tile_verts": pi3.tile_verts, "block_threads": pi3.block_threads,
                      "autotune": tune1024}
-        log(f"1024x1024 iteration: {t3 * 1e3:.1f} us, {bw3:.0f} GB/s "
-            f"({100 * bw3 / HBM_PEAK_GBS:.1f}% of peak); {1e3 / ms3:.1f} frames/s at 30 iterations")
+        log(f"1024x1024: {t3 * 1e3:.1f} us per launch of {pi3.iters_per_launch} iteration(s), "
+            f"{bw3:.0f} GB/s ({100 * bw3 / HBM_PEAK_GBS:.1f}% of peak); "
+            f"{1e3 / ms3:.1f} frames/s at 30 iterations")
         del sol3, s3
 
     # ---- 512 batched cloths of uneven size, one launch per iteration -----------
@@ -309,28 +338,31 @@ def main():
                          cloth_offsets=sb.vert_off)
         t0 = time.time()
         rep = solb.autotune(ab, 10)
-        cands = [{"tile_verts": rep.tile_verts[i], "block_threads": rep.block_threads[i],
-                  "us_per_launch": rep.us_per_launch[i]} for i in range(rep.num_candidates)]
-        log(f"autotune ({time.time() - t0:.1f} s): " + ", ".join(
-            f"{c['tile_verts']}/{c['block_threads']}: {c['us_per_launch']:.0f} us" for c in cands))
+        cands = rep.candidates()
+        log(f"autotune ({time.time() - t0:.1f} s): " + tune_line(cands, 0))
         for _ in range(2):
             solb.step(ab)
         msb = solb.run_frames(ab, 10) / 10
         tb = solb.time_iteration_kernel(ab, 20)
-        bb = algorithmic_bytes(sb.N, sb.T, sb.E)
-        bwb = bb / (tb * 1e-3) / 1e9
         pib = solb.plan_info()
+        bb = algorithmic_bytes(sb.N, sb.T, sb.E, pib.iters_per_launch)
+        bwb = bb / (tb * 1e-3) / 1e9
         batch = {"workload": "512 cloths, sides round(16*16^U) in [16, 256], stretch log-U "
                              "[1e2, 1e4], bend log-U [1e-3, 1e-1], 1-4 random boundary pins, "
                              "15 iterations/frame, graph-captured frame",
                  "cloths": sb.C, "N": sb.N, "T": sb.T, "E": sb.E,
                  "frames_per_s": 1e3 / msb, "cloth_frames_per_s": sb.C * 1e3 / msb,
-                 "ms_per_frame": msb, "us_per_iteration_launch": tb * 1e3,
+                 "ms_per_frame": msb, "kernel": kernel_name(pib, "512 cloths"),
+                 "us_per_launch": tb * 1e3, "iters_per_launch": pib.iters_per_launch,
+                 "us_per_iteration": tb * 1e3 / pib.iters_per_launch,
                  "bytes_per_launch": bb, "achieved": bwb, "frac": bwb / HBM_PEAK_GBS,
+                 "traffic": profiled_traffic(TRAFFIC_FILE_BATCH, pib),
+                 "tile_verts": pib.tile_verts, "block_threads": pib.block_threads,
                  "autotune": {"best": cands[rep.best], "candidates": cands},
                  "padded_verts": int(pib.padded_verts)}
-        log(f"batch: {1e3 / msb:.1f} frames/s ({sb.C * 1e3 / msb:.0f} cloth-frames/s), iteration "
-            f"{tb * 1e3:.0f} us = {bwb:.0f} GB/s ({100 * bwb / HBM_PEAK_GBS:.1f}% of peak)")
+        log(f"batch: {1e3 / msb:.1f} frames/s ({sb.C * 1e3 / msb:.0f} cloth-frames/s), launch of "
+            f"{pib.iters_per_launch} iteration(s) {tb * 1e3:.0f} us = {bwb:.0f} GB/s "
+            f"({100 * bwb / HBM_PEAK_GBS:.1f}% of peak)")
         del solb, sb
 
     # ---- learned warm start: train on batched cloths, evaluate on the workload --
@@ -396,12 +428,14 @@ def main():
                    "final_rel_residual_mean": float(np.mean(resid)),
                    "frames_per_s_fixed_20_iterations": fps20,
                    "fp64_oracle_error_20_iterations": err64,
-                   "kernel_variant": int(plan.variant), "tile_verts": int(plan.tile_verts),
+                   "kernel_variant": int(plan.variant), "iters_per_launch": int(ipl),
+                   "tile_verts": int(plan.tile_verts),
                    "block_threads": int(plan.block_threads), "autotune_256": tune256},
         "roofline": {"bound": "hbm", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": bw / HBM_PEAK_GBS, "traffic": profiled_traffic(TRAFFIC_FILE),
-                     "kernel": "k_fused_iter (one PD iteration, 256x256)",
-                     "us_per_launch": t_iter * 1e3, "bytes_per_launch": b_iter,
+                     "frac": bw / HBM_PEAK_GBS, "traffic": profiled_traffic(TRAFFIC_FILE, plan),
+                     "kernel": kernel_name(plan, "256x256"),
+                     "us_per_launch": t_iter * 1e3, "iters_per_launch": ipl,
+                     "us_per_iteration": t_iter * 1e3 / ipl, "bytes_per_launch": b_iter,
                      "copy_probe_GBs": copy_gbs, "mesh_1024": roof_1024,
                      "batched_512": batch},
         "cpu_baseline": cpu,

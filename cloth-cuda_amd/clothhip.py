@@ -42,7 +42,8 @@ class Config(C.Structure):
                 ("damping", C.c_float), ("tolerance", C.c_float), ("check_every", C.c_int),
                 ("cheb_delay", C.c_int), ("ext_accel", C.c_float * 3),
                 ("kernel_variant", C.c_int), ("use_graph", C.c_int), ("tile_verts", C.c_int),
-                ("block_threads", C.c_int), ("persistent", C.c_int)]
+                ("block_threads", C.c_int), ("persistent", C.c_int),
+                ("iters_per_launch", C.c_int)]
 
 
 class MeshDesc(C.Structure):
@@ -72,13 +73,24 @@ class PlanInfo(C.Structure):
                 ("bytes_per_iter", C.c_double), ("lds_bytes", C.c_double),
                 ("block_threads", C.c_int), ("num_cloths", C.c_int), ("padded_verts", C.c_int),
                 ("persistent", C.c_int), ("tri_max", C.c_int), ("hinge_max", C.c_int),
-                ("halo_max", C.c_int)]
+                ("halo_max", C.c_int), ("pair", C.c_int), ("iters_per_launch", C.c_int),
+                ("pair_halo_max", C.c_int), ("pair_tri_max", C.c_int),
+                ("pair_hinge_max", C.c_int), ("pair_bytes_per_launch", C.c_double),
+                ("pair_compute_max", C.c_int), ("pair_lds_bytes", C.c_double)]
 
 
 class TuneReport(C.Structure):
-    _fields_ = [("num_candidates", C.c_int), ("tile_verts", C.c_int * 8),
-                ("block_threads", C.c_int * 8), ("us_per_launch", C.c_float * 8),
+    _fields_ = [("num_candidates", C.c_int), ("tile_verts", C.c_int * 16),
+                ("block_threads", C.c_int * 16), ("iters_per_launch", C.c_int * 16),
+                ("us_per_iteration", C.c_float * 16),
                 ("best", C.c_int), ("from_cache", C.c_int), ("key", C.c_ulonglong)]
+
+    def candidates(self):
+        """[{tile_verts, block_threads, iters_per_launch, us_per_iteration}]"""
+        return [{"tile_verts": self.tile_verts[i], "block_threads": self.block_threads[i],
+                 "iters_per_launch": self.iters_per_launch[i],
+                 "us_per_iteration": self.us_per_iteration[i]}
+                for i in range(self.num_candidates)]
 
 
 class SceneSpec(C.Structure):

@@ -19,6 +19,12 @@
 
 namespace clh {
 
+// Rotating iterate buffers of a frame: iteration k reads x_k from it[k % 4]
+// and x_{k-1} from it[(k + 3) % 4] and writes x_{k+1} to it[(k + 1) % 4]; a
+// two-iteration launch also writes x_{k+2} to it[(k + 2) % 4], which no tile
+// reads during that launch.
+constexpr int kIterBufs = 4;
+
 struct FusedIterArgs {
     const void*  x;        // iterate k (tile order): float (reference) or double (PD)
     const void*  xkm1;     // iterate k-1
@@ -61,6 +67,27 @@ struct FusedPlan {
     DevBuf<int>   tile_cloth;                 // [num_tiles] cloth owning each tile
     DevBuf<int>   cloth_tiles;                // [num_cloths + 1] first tile of each cloth
 
+    // Two-iteration launches (PD): each tile also updates its ring-1 halo
+    // (the vertices sharing an element with an own vertex) in the first
+    // iteration, from the positions of ring 1 and ring 2, so the second
+    // iteration of its own vertices needs no other tile's result.
+    int pair = 0;          // 1 when the extension below was built
+    int pstride = 0;       // ring-1 + ring-2 ids per tile (padded)
+    int plmax = 0;         // local vertex table (own + ring 1 + ring 2)
+    int ptmax = 0, phmax = 0;  // triangles / hinges touching own or ring-1 vertices
+    int pgmax = 0;         // ring-1 gather entries
+    int pcmax = 0;         // vertices updated in the first iteration (own + ring 1)
+    double pbytes_per_launch = 0.0;
+    DevBuf<int>   pinfo;                      // [num_tiles * kPairInfo]
+    DevBuf<int>   phalo;                      // [num_tiles * pstride] ring 1, then ring 2
+    DevBuf<unsigned> pmeta;                   // [num_tiles * hstride] ring-1 gather meta
+    DevBuf<int>   ptri_gid, phin_gid;         // ring-1-only elements (original ids)
+    DevBuf<uint2> ptri_idx, phin_idx;
+    DevBuf<float4> ptri_c0, phin_w;
+    DevBuf<float> ptri_c1;
+    DevBuf<float2> phin_c;
+    DevBuf<unsigned> pgl_words;               // ring-1 gather entries, two u16 per word
+
     // packed-constant provenance (repack when any pointer changes)
     const void* packed_src[8] = {};
     float packed_h2 = -1.0f;
@@ -70,9 +97,11 @@ struct FusedPlan {
 // [seg[c], seg[c+1])), or empty for one cloth. Every element must lie in one
 // cloth; each cloth's vertices are Morton-ordered on their own and padded to
 // whole tiles with inert vertices (no elements, zero mass and diagonal).
+// pair: also build the two-iteration extension (kept off, p.pair = 0, when a
+// tile exceeds its limits).
 int  fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
                       const std::vector<int>& quads, const std::vector<float>& pos,
-                      int tile_verts, const std::vector<int>& seg);
+                      int tile_verts, const std::vector<int>& seg, bool pair);
 int  fused_pack_constants(FusedPlan& p, const clh_step_args& a, float h2, hipStream_t stream);
 int  fused_max_blocks(const FusedPlan& p);
 size_t fused_lds_bytes(const FusedPlan& p, int mode, bool check);
@@ -81,13 +110,24 @@ size_t fused_lds_bytes(const FusedPlan& p, int mode, bool check);
 int  fused_enable_lds(int& bytes);
 int  fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
                             hipStream_t stream);
+// Two PD iterations k, k+1 in one launch: a.x = x_k, a.xkm1 = x_{k-1},
+// a.out receives x_{k+1} and out2 x_{k+2}; a.omega / a.blend weight
+// iteration k and omega1 / blend1 iteration k+1 (or both come from a.ctl:
+// slot a.par and the state after it; block 0 writes the state of k+2 to the
+// other slot). A residual check (a.partials) measures iteration k+1.
+// Needs the pair extension with every tile's own + ring-1 count within the
+// workgroup (fused_pair_usable).
+bool fused_pair_usable(const FusedPlan& p);
+size_t fused_pair_lds_bytes(const FusedPlan& p, bool check);
+int  fused_launch_pair(const FusedPlan& p, const FusedIterArgs& a, void* out2, float omega1,
+                       int blend1, hipStream_t stream);
 
 // Persistent PD iterations [k0, k1) in one cooperative launch (device control
-// through ctl; iterates in it[k % 3]; grid barrier words bar[2], zero at
+// through ctl; iterates in it[k % kIterBufs]; grid barrier words bar[2], zero at
 // rest). fused_persist_fits tells whether every tile can be resident at once.
 size_t fused_persist_lds_bytes(const FusedPlan& p);
 bool fused_persist_fits(const FusedPlan& p);
-int  fused_launch_persist(const FusedPlan& p, const FusedIterArgs& a, double* const it[3], int k0,
+int  fused_launch_persist(const FusedPlan& p, const FusedIterArgs& a, double* const it[kIterBufs], int k0,
                           int k1, int K, int check_every, FrameCtl* ctl, double* partials,
                           double* hist, unsigned* bar, hipStream_t stream);
 void fused_plan_info(const FusedPlan& p, clh_plan_info* out);

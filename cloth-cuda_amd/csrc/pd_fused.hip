@@ -16,6 +16,8 @@ namespace {
 constexpr int kFB = 256;  // largest tile (one own vertex per thread)
 constexpr int kTileInfo = 8;  // nhalo, tri_beg, ntri, hin_beg, nhin, gather word begin,
                                // gather entries, vbeg
+constexpr int kPairInfo = 8;  // ring-2 count, tri_beg, ntri, hin_beg, nhin of the ring-1-only
+                               // elements, ring-1 gather word begin, gather entries, 0
 
 // ---------------------------------------------------------------------------
 // host: plan
@@ -111,6 +113,12 @@ struct HostPlan {
     std::vector<uint2> tri_idx, hin_idx;
     std::vector<unsigned short> gl_idx;
     std::vector<unsigned> gl_meta;
+    // two-iteration extension (meta.pair): ring-2 halo, the elements that
+    // touch ring 1 but no own vertex, and ring 1's gather lists
+    std::vector<int> pinfo, phalo, ptri_gid, phin_gid;
+    std::vector<uint2> ptri_idx, phin_idx;
+    std::vector<unsigned short> pgl_idx;
+    std::vector<unsigned> pmeta;
 };
 
 // ---------------------------------------------------------------------------
@@ -156,6 +164,23 @@ __device__ __forceinline__ void stsX(double* s, int n, int i, d3 v)
     s[i] = v.x;
     s[n + i] = v.y;
     s[2 * n + i] = v.z;
+}
+
+// PD update of a free vertex: residual r = f + m (dy - d) of the global
+// system at the displacement d, the Jacobi step d + r / D, then the Chebyshev
+// blend with d_{k-1} when `blend`. Every kernel that updates a vertex calls
+// this one function, so a vertex updated by two workgroups (as own vertex and
+// as another tile's ring-1 vertex) gets the same bits.
+__device__ __forceinline__ d3 pd_update(f3 ar, float om, float odg, f3 oy, d3 od, d3 oxp,
+                                        float omega, int blend, d3& r)
+{
+    const double md = om;
+    r = d3{(double)ar.x + md * ((double)oy.x - od.x), (double)ar.y + md * ((double)oy.y - od.y),
+           (double)ar.z + md * ((double)oy.z - od.z)};
+    const double inv = (odg > 1e-10f) ? 1.0 / (double)odg : 0.0;
+    d3 xn = od + r * inv;
+    if (blend) xn = (xn - oxp) * (double)omega + oxp;
+    return xn;
 }
 
 template <int kT>
@@ -415,13 +440,8 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
             const float inv = (odg > 1e-10f) ? (1.0f / odg) : 0.0f;
             st3(static_cast<float*>(a.out), g, madd_nc(ar, om, oy) * inv);
         } else {
-            const double md = om;
-            const d3 r = d3{(double)ar.x + md * ((double)oy.x - od.x),
-                            (double)ar.y + md * ((double)oy.y - od.y),
-                            (double)ar.z + md * ((double)oy.z - od.z)};
-            const double inv = (odg > 1e-10f) ? 1.0 / (double)odg : 0.0;
-            d3 xn = od + r * inv;
-            if (a.blend) xn = (xn - oxp) * (double)a.omega + oxp;
+            d3 r;
+            const d3 xn = pd_update(ar, om, odg, oy, od, oxp, a.omega, a.blend, r);
             st3d(static_cast<double*>(a.out), g, xn);
             if (kCheck) {
                 const f3 ya = ob + oy;
@@ -430,6 +450,377 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
                 bb = (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
             }
         }
+    }
+    if (kCheck) fused_block_sum2<kT>(rr, bb, a.partials, tile);
+}
+
+// ---- two PD iterations per launch ---------------------------------------------
+//
+// Iteration k updates the tile's own vertices and its ring-1 halo (the
+// vertices sharing an element with an own vertex) from the positions of own,
+// ring 1 and ring 2; iteration k+1 then updates the own vertices from the
+// first iteration's own and ring-1 results, all in LDS. A ring-1 vertex is
+// updated exactly as its own tile updates it: same element records, same
+// gather order (ascending element id), same update function, so x_{k+1} of
+// every vertex is the same in every tile that computes it, and the two
+// iterations give the bits of two single-iteration launches. Memory trips are
+// those of k_fused_iter: (1) descriptors, own vertex data, pair halo ids and
+// ring-1 gather metas; (2) halo positions, ring-1 vertex data, first element
+// records and both gather lists. The first element records of the second
+// iteration are the first iteration's own-list records, kept in registers.
+// x_{k+1} of own vertices goes to a.out (the next launch's x_{k-1}), x_{k+2}
+// to out2. The residual check, when requested, belongs to iteration k+1.
+//
+// LDS: position table (FP64 x_n + d of own, ring 1, ring 2 as three
+// component arrays), own gather list, ring-1 gather list, element region
+// (triangle corner forces of both element lists, then hinge weights and
+// vectors).
+
+struct PairKernelArgs {
+    FusedKernelArgs A;      // plan, frame arrays; A.it.out receives x_{k+1}
+    void* out2;             // x_{k+2}
+    float omega1;           // second iteration (host control)
+    int   blend1;
+    const int*      pinfo;
+    const int*      phalo;
+    const unsigned* pmeta;
+    const uint2*    ptri_idx;
+    const float4*   ptri_c0;
+    const float*    ptri_c1;
+    const uint2*    phin_idx;
+    const float4*   phin_w;
+    const float2*   phin_c;
+    const unsigned* pgl_words;
+    int pstride, plmax, ptmax, phmax;
+    int off_g2;             // LDS byte offset of the ring-1 gather list
+};
+
+template <bool kCheck, int kT>
+__global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
+{
+    extern __shared__ double smem[];
+    const FusedKernelArgs& A = P.A;
+    const FusedIterArgs& a = A.it;
+    const int tid = threadIdx.x;
+    int stopped = 0;
+    CtlSlot c0 = {a.omega, a.blend, 0, 0};
+    if (a.ctl) {
+        stopped = a.ctl->stop;
+        c0 = a.ctl->slot[a.par];
+    }
+
+    const int L = P.plmax;
+    int tile = blockIdx.x;
+    if (A.xcd_remap) {
+        const int per = A.num_tiles >> 3;
+        tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    }
+    const int vbeg = tile * A.tile_verts;
+    const int nown = min(A.tile_verts, A.N - vbeg);
+
+    char* lds = reinterpret_cast<char*>(smem);
+    double* sX = smem;
+    unsigned* sgw = reinterpret_cast<unsigned*>(lds + A.off_g);
+    unsigned* sgw2 = reinterpret_cast<unsigned*>(lds + P.off_g2);
+    const unsigned short* sg = reinterpret_cast<const unsigned short*>(sgw);
+    const unsigned short* sg2 = reinterpret_cast<const unsigned short*>(sgw2);
+    float* st = reinterpret_cast<float*>(lds + A.off_r);     // tri corner forces [ptmax*9]
+    float* st2 = st + P.ptmax * 9;                           // tri R part [tmax*9] (check)
+    float* skw = st;                                         // hinge k h^2 w_i [4][phmax]
+    float* sq = skw + P.phmax * 4;                           // hinge p-e [phmax*3]
+    float* sp = sq + P.phmax * 3;                            // hinge p [hmax*3] (check)
+
+    const double* xd = static_cast<const double*>(a.x);
+    const double* xm = static_cast<const double*>(a.xkm1);
+
+    // ---- trip 1: descriptors, own vertices, pair halo ids, ring-1 metas -----
+    const int* ti = A.tile_info + (size_t)tile * kTileInfo;
+    const int* pi = P.pinfo + (size_t)tile * kPairInfo;
+    const bool own = tid < nown;
+    const int r1 = tid - nown;  // ring-1 index of this thread's vertex
+    const size_t g = (size_t)vbeg + (own ? tid : 0);
+    int4 vi = make_int4(0, 0, -1, 0);
+    d3 od = d3{0.0, 0.0, 0.0}, oxp = d3{0.0, 0.0, 0.0};
+    f3 ob = mk(0.0f, 0.0f, 0.0f), oy = mk(0.0f, 0.0f, 0.0f);
+    if (own) {
+        vi = a.vinfo[g];
+        oy = ld3(a.y, g);
+        od = ld3d(xd, g);
+        ob = ld3(a.base, g);
+        if (a.blend || a.ctl) oxp = ld3d(xm, g);
+    }
+    const int* ph = P.phalo + (size_t)tile * P.pstride;
+    const int hid0 = tid < P.pstride ? ph[tid] : 0;
+    int rid = 0;
+    unsigned rmeta = 0u;
+    if (r1 >= 0 && r1 < A.hstride) {
+        rid = ph[r1];
+        rmeta = P.pmeta[(size_t)tile * A.hstride + r1];
+    }
+    const int nh1 = ti[0], tbeg = ti[1], ntri = ti[2], ebeg = ti[3], nhin = ti[4];
+    const int gbeg = ti[5], glen = ti[6];
+    const int nh2 = pi[0], tbeg2 = pi[1], ntri2 = pi[2], ebeg2 = pi[3], nhin2 = pi[4];
+    const int pgbeg = pi[5], pglen = pi[6];
+    CtlSlot c1 = {P.omega1, P.blend1, 0, 0};
+    if (a.ctl) {
+        if (stopped) return;
+        c1 = ctl_next(*a.ctl, c0);
+        // the launch after this one starts at iteration k + 2
+        ctl_write_next(a.ctl, a.par, c1);
+    }
+    const bool ring1 = r1 >= 0 && r1 < nh1;
+    const bool cv = own || ring1;  // updated in the first iteration
+
+    // ---- trip 2: halo positions, ring-1 data, first records, gather lists ---
+    for (int i = tid; i < nh1 + nh2; i += kT) {
+        const int hg = (i == tid) ? hid0 : ph[i];
+        stsX(sX, L, nown + i, to_d(ld3(a.base, hg)) + ld3d(xd, hg));
+    }
+    if (ring1) {
+        vi = a.vinfo[rid];
+        oy = ld3(a.y, rid);
+        od = ld3d(xd, rid);
+        ob = ld3(a.base, rid);
+        if (c0.blend) oxp = ld3d(xm, rid);
+    }
+    const int ntA = ntri + ntri2, nhA = nhin + nhin2;
+    // first records of both iterations: slot tid of the first iteration's
+    // list, which for tid < ntri is also slot tid of the second's
+    uint2 rt0 = make_uint2(0, 0);
+    float4 rG0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float rwA0 = 0.0f;
+    if (a.use_stretch && tid < ntA) {
+        if (tid < ntri) {
+            rt0 = A.tri_idx[tbeg + tid];
+            rG0 = A.tri_c0[tbeg + tid];
+            rwA0 = A.tri_c1[tbeg + tid];
+        } else {
+            const int q = tbeg2 + tid - ntri;
+            rt0 = P.ptri_idx[q];
+            rG0 = P.ptri_c0[q];
+            rwA0 = P.ptri_c1[q];
+        }
+    }
+    for (int j = tid; j < (glen + 1) >> 1; j += kT) sgw[j] = A.gl_words[gbeg + j];
+    for (int j = tid; j < (pglen + 1) >> 1; j += kT) sgw2[j] = P.pgl_words[pgbeg + j];
+    if (own) stsX(sX, L, tid, to_d(ob) + od);
+    __syncthreads();
+
+    const int opin = vi.z;
+    const float om = __int_as_float(vi.x), odg = __int_as_float(vi.y);
+    const bool gather = cv && opin < 0;
+
+    // ---- iteration k: elements of own and ring-1 vertices --------------------
+    if (a.use_stretch) {
+        uint2 rt = rt0;
+        float4 rG = rG0;
+        float rwA = rwA0;
+        for (int j = tid; j < ntA; j += kT) {
+            const uint2 id = rt;
+            const float4 G = rG;
+            const float wA = rwA;
+            const int jn = j + kT;
+            if (jn < ntA) {
+                if (jn < ntri) {
+                    rt = A.tri_idx[tbeg + jn];
+                    rG = A.tri_c0[tbeg + jn];
+                    rwA = A.tri_c1[tbeg + jn];
+                } else {
+                    const int q = tbeg2 + jn - ntri;
+                    rt = P.ptri_idx[q];
+                    rG = P.ptri_c0[q];
+                    rwA = P.ptri_c1[q];
+                }
+            }
+            const int l0 = id.x & 0xFFFF, l1 = id.x >> 16, l2 = id.y & 0xFFFF;
+            f3 cr[3], c[3];
+            pd_triangle(ldsX(sX, L, l0), ldsX(sX, L, l1), ldsX(sX, L, l2), G, wA, cr, c, false);
+            float* o = st + (size_t)j * 9;
+            sts3(o, 0, c[0]);
+            sts3(o, 1, c[1]);
+            sts3(o, 2, c[2]);
+        }
+    }
+    uint2 rh0 = make_uint2(0, 0);
+    float4 rw0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float2 rc0 = make_float2(0.0f, 0.0f);
+    if (a.use_bend && tid < nhA) {
+        if (tid < nhin) {
+            rh0 = A.hin_idx[ebeg + tid];
+            rw0 = A.hin_w[ebeg + tid];
+            rc0 = A.hin_c[ebeg + tid];
+        } else {
+            const int q = ebeg2 + tid - nhin;
+            rh0 = P.phin_idx[q];
+            rw0 = P.phin_w[q];
+            rc0 = P.phin_c[q];
+        }
+    }
+    __syncthreads();
+
+    const unsigned meta = own ? (unsigned)vi.w : rmeta;
+    const unsigned short* gl = own ? sg : sg2;
+    const int j0 = (int)(meta & 0xFFFFu), jt = j0 + (int)((meta >> 16) & 0xFFu),
+              j1 = j0 + (int)(meta >> 24);
+    f3 ar = mk(0.0f, 0.0f, 0.0f);
+    if (gather && a.use_stretch)
+        for (int j = j0; j < jt; ++j) ar = ar + lds3(st, gl[j]);
+    if (a.use_bend) {
+        __syncthreads();
+        uint2 rh = rh0;
+        float4 rw = rw0;
+        float2 rc = rc0;
+        for (int j = tid; j < nhA; j += kT) {
+            const uint2 id = rh;
+            const float4 w = rw;
+            const float2 c = rc;
+            const int jn = j + kT;
+            if (jn < nhA) {
+                if (jn < nhin) {
+                    rh = A.hin_idx[ebeg + jn];
+                    rw = A.hin_w[ebeg + jn];
+                    rc = A.hin_c[ebeg + jn];
+                } else {
+                    const int q = ebeg2 + jn - nhin;
+                    rh = P.phin_idx[q];
+                    rw = P.phin_w[q];
+                    rc = P.phin_c[q];
+                }
+            }
+            const int q0 = id.x & 0xFFFF, q1 = id.x >> 16, q2 = id.y & 0xFFFF, q3 = id.y >> 16;
+            f3 p, pe;
+            const d3 X[4] = {ldsX(sX, L, q0), ldsX(sX, L, q1), ldsX(sX, L, q2), ldsX(sX, L, q3)};
+            pd_hinge(X, w, c.x, p, pe);
+            skw[j] = c.y * w.x;
+            skw[P.phmax + j] = c.y * w.y;
+            skw[2 * P.phmax + j] = c.y * w.z;
+            skw[3 * P.phmax + j] = c.y * w.w;
+            sts3(sq, j, pe);
+        }
+        __syncthreads();
+        if (gather) {
+            for (int j = jt; j < j1; ++j) {
+                const int s = gl[j];
+                const float kw = skw[(s & 3) * P.phmax + (s >> 2)];
+                const f3 d = lds3(sq, s >> 2);
+                ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+            }
+        }
+    }
+    // x_{k+1} of own and ring-1 vertices; the position table is free (its
+    // last reads were in the element loops, before the barrier above)
+    d3 x1 = d3{0.0, 0.0, 0.0};
+    if (cv) {
+        if (opin >= 0) {
+            x1 = to_d(ld3(a.pin_target, opin)) - to_d(ob);
+        } else {
+            d3 r;
+            x1 = pd_update(ar, om, odg, oy, od, oxp, c0.omega, c0.blend, r);
+        }
+        if (own) st3d(static_cast<double*>(a.out), g, x1);
+        stsX(sX, L, tid, to_d(ob) + x1);
+    }
+    __syncthreads();
+
+    // ---- iteration k+1: elements of own vertices -------------------------------
+    ar = mk(0.0f, 0.0f, 0.0f);
+    f3 ab = mk(0.0f, 0.0f, 0.0f);
+    const bool gather2 = own && opin < 0;
+    if (a.use_stretch) {
+        uint2 rt = rt0;
+        float4 rG = rG0;
+        float rwA = rwA0;
+        for (int j = tid; j < ntri; j += kT) {
+            const uint2 id = rt;
+            const float4 G = rG;
+            const float wA = rwA;
+            if (j + kT < ntri) {
+                rt = A.tri_idx[tbeg + j + kT];
+                rG = A.tri_c0[tbeg + j + kT];
+                rwA = A.tri_c1[tbeg + j + kT];
+            }
+            const int l0 = id.x & 0xFFFF, l1 = id.x >> 16, l2 = id.y & 0xFFFF;
+            f3 cr[3], c[3];
+            pd_triangle(ldsX(sX, L, l0), ldsX(sX, L, l1), ldsX(sX, L, l2), G, wA, cr, c, kCheck);
+            if (kCheck) {
+                float* q = st2 + (size_t)j * 9;
+                sts3(q, 0, cr[0]);
+                sts3(q, 1, cr[1]);
+                sts3(q, 2, cr[2]);
+            }
+            float* o = st + (size_t)j * 9;
+            sts3(o, 0, c[0]);
+            sts3(o, 1, c[1]);
+            sts3(o, 2, c[2]);
+        }
+    }
+    __syncthreads();
+    const int k0 = (int)((unsigned)vi.w & 0xFFFFu), kt = k0 + (int)(((unsigned)vi.w >> 16) & 0xFFu),
+              k1 = k0 + (int)((unsigned)vi.w >> 24);
+    if (gather2 && a.use_stretch) {
+        for (int j = k0; j < kt; ++j) {
+            const int s = sg[j];
+            ar = ar + lds3(st, s);
+            if (kCheck) ab = ab + lds3(st2, s);
+        }
+    }
+    if (a.use_bend) {
+        __syncthreads();
+        uint2 rh = rh0;
+        float4 rw = rw0;
+        float2 rc = rc0;
+        for (int j = tid; j < nhin; j += kT) {
+            const uint2 id = rh;
+            const float4 w = rw;
+            const float2 c = rc;
+            if (j + kT < nhin) {
+                rh = A.hin_idx[ebeg + j + kT];
+                rw = A.hin_w[ebeg + j + kT];
+                rc = A.hin_c[ebeg + j + kT];
+            }
+            const int q0 = id.x & 0xFFFF, q1 = id.x >> 16, q2 = id.y & 0xFFFF, q3 = id.y >> 16;
+            f3 p, pe;
+            const d3 X[4] = {ldsX(sX, L, q0), ldsX(sX, L, q1), ldsX(sX, L, q2), ldsX(sX, L, q3)};
+            pd_hinge(X, w, c.x, p, pe);
+            skw[j] = c.y * w.x;
+            skw[P.phmax + j] = c.y * w.y;
+            skw[2 * P.phmax + j] = c.y * w.z;
+            skw[3 * P.phmax + j] = c.y * w.w;
+            sts3(sq, j, pe);
+            if (kCheck) sts3(sp, j, p);
+        }
+        __syncthreads();
+        if (gather2) {
+            for (int j = kt; j < k1; ++j) {
+                const int s = sg[j];
+                const float kw = skw[(s & 3) * P.phmax + (s >> 2)];
+                const f3 d = lds3(sq, s >> 2);
+                ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+                if (kCheck) {
+                    const f3 q = lds3(sp, s >> 2);
+                    ab = ab + mk(kw * q.x, kw * q.y, kw * q.z);
+                }
+            }
+        }
+    }
+
+    // ---- own vertices: x_{k+2}, pins, residual ---------------------------------
+    double rr = 0.0, bb = 0.0;
+    if (own) {
+        d3 x2;
+        if (opin >= 0) {
+            x2 = to_d(ld3(a.pin_target, opin)) - to_d(ob);
+        } else {
+            d3 r;
+            x2 = pd_update(ar, om, odg, oy, x1, od, c1.omega, c1.blend, r);
+            if (kCheck) {
+                const f3 ya = ob + oy;
+                const f3 b = mk(ab.x + om * ya.x, ab.y + om * ya.y, ab.z + om * ya.z);
+                rr = r.x * r.x + r.y * r.y + r.z * r.z;
+                bb = (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
+            }
+        }
+        st3d(static_cast<double*>(P.out2), g, x2);
     }
     if (kCheck) fused_block_sum2<kT>(rr, bb, a.partials, tile);
 }
@@ -455,7 +846,7 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
 
 struct PersistArgs {
     FusedKernelArgs A;   // plan and frame arrays (A.it.x / xkm1 / out unused)
-    double* it[3];       // the rotating iterate buffers (tile order)
+    double* it[kIterBufs];  // the rotating iterate buffers (tile order)
     int k0, k1, K, check_every;
     FrameCtl* ctl;
     double* partials;    // [num_tiles * 2]
@@ -576,8 +967,8 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
         sov[tid] = a.vinfo[g];
         sts3(sob, tid, ld3(a.base, g));
         sts3(soy, tid, ld3(a.y, g));
-        od = ld3d(P.it[P.k0 % 3], g);
-        oxp = ld3d(P.it[(P.k0 + 2) % 3], g);
+        od = ld3d(P.it[P.k0 % kIterBufs], g);
+        oxp = ld3d(P.it[(P.k0 + kIterBufs - 1) % kIterBufs], g);
     }
     __syncthreads();
 
@@ -592,8 +983,8 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
         cs.k = ctl_load_int(&P.ctl->slot[par].k);
         ctl_write_next(P.ctl, par, cs);
         const bool check = ((k + 1) % P.check_every == 0) || (k == P.K - 1);
-        const double* xk = P.it[k % 3];
-        double* out = P.it[(k + 1) % 3];
+        const double* xk = P.it[k % kIterBufs];
+        double* out = P.it[(k + 1) % kIterBufs];
 
         // ---- positions of the tile: own from registers, halo from memory
         for (int i = tid; i < nhalo; i += kT) {
@@ -680,13 +1071,8 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
                 xn = to_d(ld3(a.pin_target, vi.z)) - to_d(ob);
             } else {
                 const f3 oy = lds3(soy, tid);
-                const double md = om;
-                const d3 r = d3{(double)ar.x + md * ((double)oy.x - od.x),
-                                (double)ar.y + md * ((double)oy.y - od.y),
-                                (double)ar.z + md * ((double)oy.z - od.z)};
-                const double inv = (odg > 1e-10f) ? 1.0 / (double)odg : 0.0;
-                xn = od + r * inv;
-                if (cs.blend) xn = (xn - oxp) * (double)cs.omega + oxp;
+                d3 r;
+                xn = pd_update(ar, om, odg, oy, od, oxp, cs.omega, cs.blend, r);
                 if (check) {
                     const f3 ya = ob + oy;
                     const f3 b = mk(ab.x + om * ya.x, ab.y + om * ya.y, ab.z + om * ya.z);
@@ -870,6 +1256,48 @@ LdsLayout lds_layout(const FusedPlan& p, int mode, bool check)
 
 size_t lds_bytes(const FusedPlan& p, int mode, bool check) { return lds_layout(p, mode, check).total; }
 
+// LDS byte offsets of the two-iteration kernel: position table (own, ring 1,
+// ring 2), own gather list, ring-1 gather list, element region (triangles of
+// both lists plus the second iteration's R parts on a check; or hinges of
+// both lists plus the second iteration's projections).
+struct PairLdsLayout {
+    size_t off_g, off_g2, off_r, total;
+};
+
+PairLdsLayout pair_lds_layout(const FusedPlan& p, bool check)
+{
+    PairLdsLayout L;
+    L.off_g = align16((size_t)p.plmax * 24);
+    L.off_g2 = L.off_g + align16((size_t)(p.gmax + 1) / 2 * 4);
+    L.off_r = L.off_g2 + align16((size_t)(p.pgmax + 1) / 2 * 4);
+    const size_t ra = ((size_t)p.ptmax * 9 + (check ? (size_t)p.tmax * 9 : 0)) * sizeof(float);
+    const size_t rb = ((size_t)p.phmax * 7 + (check ? (size_t)p.hmax * 3 : 0)) * sizeof(float);
+    L.total = L.off_r + align16(std::max(ra, rb));
+    return L;
+}
+
+void fill_kernel_args(const FusedPlan& p, const FusedIterArgs& a, FusedKernelArgs& A)
+{
+    A.it = a;
+    A.tile_info = p.tile_info.ptr;
+    A.halo = p.halo.ptr;
+    A.tri_idx = p.tri_idx.ptr;
+    A.tri_c0 = p.tri_c0.ptr;
+    A.tri_c1 = p.tri_c1.ptr;
+    A.hin_idx = p.hin_idx.ptr;
+    A.hin_w = p.hin_w.ptr;
+    A.hin_c = p.hin_c.ptr;
+    A.gl_words = p.gl_words.ptr;
+    A.N = p.N;
+    A.tile_verts = p.tile_verts;
+    A.hstride = p.hstride;
+    A.lmax = p.lmax;
+    A.tmax = p.tmax;
+    A.hmax = p.hmax;
+    A.num_tiles = p.num_tiles;
+    A.xcd_remap = p.xcd_remap;
+}
+
 // LDS of the persistent kernel: the iteration kernel's layout (PD, residual
 // variant), then the tile's triangle and hinge records, halo ids and halo
 // base positions.
@@ -897,7 +1325,7 @@ unsigned blocks_for(int n) { return (unsigned)((n + 255) / 256); }
 
 int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
                     const std::vector<int>& quads, const std::vector<float>& pos, int tile_verts,
-                    const std::vector<int>& seg_in)
+                    const std::vector<int>& seg_in, bool want_pair)
 {
     if (N <= 0) return fail(CLH_ERR_UNSUPPORTED, "fused plan: empty mesh");
     const int T = (int)(tris.size() / 3), E = (int)(quads.size() / 4);
@@ -950,6 +1378,14 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
     std::vector<unsigned> gl_meta((size_t)Np, 0);
     std::vector<int> loc((size_t)Np, -1);  // tile-order id -> local id in the current tile
     int lmax = 0, tmax = 0, hmax = 0, gmax = 0, hstride = 1;
+    // two-iteration extension
+    bool pair_ok = want_pair;
+    std::vector<int> pinfo, phalo_list, phalo_beg((size_t)ntiles + 1, 0), ptri_gid, phin_gid;
+    std::vector<uint2> ptri_idx, phin_idx;
+    std::vector<unsigned short> pgl_idx;
+    std::vector<unsigned> pmeta_list;
+    int pstride = 1, plmax = 0, ptmax = 0, phmax = 0, pgmax = 0, pcmax = 0;
+    if (pair_ok) pinfo.assign((size_t)ntiles * kPairInfo, 0);
 
     for (int t = 0; t < ntiles; ++t) {
         const int vbeg = t * V, nown = V;
@@ -1036,6 +1472,105 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
         tmax = std::max(tmax, (int)tl.size());
         hmax = std::max(hmax, (int)hl.size());
         hstride = std::max(hstride, nl - nown);
+
+        if (pair_ok) {
+            // ring 1 = this tile's halo (local ids nown .. nl-1); the elements
+            // touching ring 1 but no own vertex, ascending; their vertices
+            // outside own and ring 1 form ring 2 (local ids nl ..)
+            const int nh1 = nl - nown;
+            const int* r1 = halo_list.data() + halo_beg[(size_t)t];
+            std::vector<int> tl2, hl2, r2;
+            for (int i = 0; i < nh1; ++i) {
+                const int o = perm[(size_t)r1[i]];
+                for (int j = vt_ptr[(size_t)o]; j < vt_ptr[(size_t)o + 1]; ++j) tl2.push_back(vt_ids[(size_t)j]);
+                for (int j = ve_ptr[(size_t)o]; j < ve_ptr[(size_t)o + 1]; ++j) hl2.push_back(ve_ids[(size_t)j]);
+            }
+            std::sort(tl2.begin(), tl2.end());
+            tl2.erase(std::unique(tl2.begin(), tl2.end()), tl2.end());
+            tl2.erase(std::remove_if(tl2.begin(), tl2.end(),
+                                     [&](int e) { return std::binary_search(tl.begin(), tl.end(), e); }),
+                      tl2.end());
+            std::sort(hl2.begin(), hl2.end());
+            hl2.erase(std::unique(hl2.begin(), hl2.end()), hl2.end());
+            hl2.erase(std::remove_if(hl2.begin(), hl2.end(),
+                                     [&](int e) { return std::binary_search(hl.begin(), hl.end(), e); }),
+                      hl2.end());
+            int nl2 = nl;
+            auto local2 = [&](int orig) {
+                const int g = inv[(size_t)orig];
+                if (loc[(size_t)g] < 0) {
+                    loc[(size_t)g] = nl2++;
+                    r2.push_back(g);
+                    touched.push_back(g);
+                }
+                return loc[(size_t)g];
+            };
+            const int tbeg2 = (int)ptri_gid.size();
+            for (int e : tl2) {
+                const int a = local2(tris[(size_t)e * 3]), b = local2(tris[(size_t)e * 3 + 1]),
+                          c = local2(tris[(size_t)e * 3 + 2]);
+                ptri_gid.push_back(e);
+                ptri_idx.push_back(make_uint2((unsigned)a | ((unsigned)b << 16), (unsigned)c | 0xFFFF0000u));
+            }
+            const int ebeg2 = (int)phin_gid.size();
+            for (int e : hl2) {
+                const int a = local2(quads[(size_t)e * 4]), b = local2(quads[(size_t)e * 4 + 1]),
+                          c = local2(quads[(size_t)e * 4 + 2]), d = local2(quads[(size_t)e * 4 + 3]);
+                phin_gid.push_back(e);
+                phin_idx.push_back(make_uint2((unsigned)a | ((unsigned)b << 16), (unsigned)c | ((unsigned)d << 16)));
+            }
+            // ring-1 gather lists over the slots of both element lists (the
+            // own vertices' elements first, then the ring-1-only ones), in
+            // ascending element id as for own vertices
+            const int nt1 = (int)tl.size(), ne1 = (int)hl.size();
+            if (pgl_idx.size() & 1) pgl_idx.push_back(0);
+            const size_t pgbeg = pgl_idx.size();
+            for (int i = 0; i < nh1 && pair_ok; ++i) {
+                const int o = perm[(size_t)r1[i]];
+                const size_t start = pgl_idx.size() - pgbeg;
+                int nt = 0;
+                for (int j = vt_ptr[(size_t)o]; j < vt_ptr[(size_t)o + 1]; ++j) {
+                    const int e = vt_ids[(size_t)j];
+                    auto it1 = std::lower_bound(tl.begin(), tl.end(), e);
+                    const int slot = (it1 != tl.end() && *it1 == e)
+                                         ? (int)(it1 - tl.begin())
+                                         : nt1 + (int)(std::lower_bound(tl2.begin(), tl2.end(), e) - tl2.begin());
+                    for (int c = 0; c < 3; ++c)
+                        if (tris[(size_t)e * 3 + c] == o) {
+                            pgl_idx.push_back((unsigned short)(slot * 3 + c));
+                            ++nt;
+                        }
+                }
+                for (int j = ve_ptr[(size_t)o]; j < ve_ptr[(size_t)o + 1]; ++j) {
+                    const int e = ve_ids[(size_t)j];
+                    auto it1 = std::lower_bound(hl.begin(), hl.end(), e);
+                    const int slot = (it1 != hl.end() && *it1 == e)
+                                         ? (int)(it1 - hl.begin())
+                                         : ne1 + (int)(std::lower_bound(hl2.begin(), hl2.end(), e) - hl2.begin());
+                    for (int c = 0; c < 4; ++c)
+                        if (quads[(size_t)e * 4 + c] == o) pgl_idx.push_back((unsigned short)(slot * 4 + c));
+                }
+                const size_t nall = pgl_idx.size() - pgbeg - start;
+                if (nall > 255 || start > 0xFFFF) pair_ok = false;
+                pmeta_list.push_back((unsigned)start | ((unsigned)nt << 16) | ((unsigned)nall << 24));
+            }
+            const size_t pglen = pgl_idx.size() - pgbeg;
+            if (nl2 > 65535 || (nt1 + (int)tl2.size()) * 3 > 65535 ||
+                (ne1 + (int)hl2.size()) * 4 > 65535 || pglen > 0xFFFF + 255 || nown + nh1 > 512)
+                pair_ok = false;
+            int* pi = &pinfo[(size_t)t * kPairInfo];
+            pi[0] = (int)r2.size(); pi[1] = tbeg2; pi[2] = (int)tl2.size(); pi[3] = ebeg2;
+            pi[4] = (int)hl2.size(); pi[5] = (int)(pgbeg / 2); pi[6] = (int)pglen; pi[7] = 0;
+            phalo_list.insert(phalo_list.end(), r1, r1 + nh1);
+            phalo_list.insert(phalo_list.end(), r2.begin(), r2.end());
+            phalo_beg[(size_t)t + 1] = (int)phalo_list.size();
+            pstride = std::max(pstride, nl2 - nown);
+            plmax = std::max(plmax, nl2);
+            ptmax = std::max(ptmax, nt1 + (int)tl2.size());
+            phmax = std::max(phmax, ne1 + (int)hl2.size());
+            pgmax = std::max(pgmax, (int)pglen);
+            pcmax = std::max(pcmax, nl);
+        }
         for (int g : touched) loc[(size_t)g] = -1;
     }
     if (gl_idx.size() & 1) gl_idx.push_back(0);
@@ -1050,7 +1585,52 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
             halo[(size_t)t * hstride + i] = i < n ? halo_list[(size_t)halo_beg[(size_t)t] + i] : t * V;
     }
 
+    if (pair_ok) {
+        // fixed strides, padded as the halo ids; ring-1 metas at the halo stride
+        h.phalo.assign((size_t)ntiles * pstride, 0);
+        h.pmeta.assign((size_t)ntiles * hstride, 0u);
+        size_t mo = 0;
+        for (int t = 0; t < ntiles; ++t) {
+            const int n = phalo_beg[(size_t)t + 1] - phalo_beg[(size_t)t];
+            for (int i = 0; i < pstride; ++i)
+                h.phalo[(size_t)t * pstride + i] = i < n ? phalo_list[(size_t)phalo_beg[(size_t)t] + i] : t * V;
+            const int nh1 = info[(size_t)t * kTileInfo];
+            for (int i = 0; i < nh1; ++i) h.pmeta[(size_t)t * hstride + i] = pmeta_list[mo++];
+        }
+        if (pgl_idx.size() & 1) pgl_idx.push_back(0);
+        if (pgl_idx.empty()) pgl_idx.assign(2, 0);
+        if (ptri_gid.empty()) { ptri_gid.push_back(0); ptri_idx.push_back(make_uint2(0, 0)); }
+        if (phin_gid.empty()) { phin_gid.push_back(0); phin_idx.push_back(make_uint2(0, 0)); }
+        h.pinfo.swap(pinfo);
+        h.ptri_gid.swap(ptri_gid);
+        h.phin_gid.swap(phin_gid);
+        h.ptri_idx.swap(ptri_idx);
+        h.phin_idx.swap(phin_idx);
+        h.pgl_idx.swap(pgl_idx);
+    }
+
     FusedPlan& p = h.meta;
+    p.pair = pair_ok ? 1 : 0;
+    if (pair_ok) {
+        p.pstride = pstride;
+        p.plmax = plmax;
+        p.ptmax = std::max(ptmax, 1);
+        p.phmax = std::max(phmax, 1);
+        p.pgmax = std::max(pgmax, 2);
+        p.pcmax = pcmax;
+        // per launch: own x_k, x_{k-1} read and x_{k+1}, x_{k+2} written
+        // (24 B each), base, dy, record; per ring-1 entry x_k, x_{k-1}, base,
+        // dy, record, id and gather meta; per ring-2 entry x_k, base and id;
+        // both element lists; both gather lists; descriptors
+        size_t n2 = 0;
+        for (int t = 0; t < ntiles; ++t) n2 += (size_t)h.pinfo[(size_t)t * kPairInfo];
+        p.pbytes_per_launch = (double)Np * (4 * 24 + 2 * 12 + 16) + (double)halo_list.size() * 96 +
+                              (double)n2 * 40 +
+                              (double)ntiles * (kTileInfo + kPairInfo + hstride) * 4 +
+                              (double)(tri_gid.size() + h.ptri_gid.size()) * 28 +
+                              (double)(hin_gid.size() + h.phin_gid.size()) * 32 +
+                              (double)(gl_idx.size() + h.pgl_idx.size()) * 2;
+    }
     p.N = Np;
     p.num_verts = N;
     p.num_cloths = C;
@@ -1099,6 +1679,108 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
 // element's vertex, padding has no gather list, and each own vertex's gather
 // list equals its incident (element, corner) slots in ascending order (the
 // order of the unfused pipeline and the oracle) inside the tile's segment.
+// Two-iteration extension of tile t: ring 1 in the pair halo is the tile's
+// halo; ring 2 lies outside own and ring 1, without repeats, in the tile's
+// cloth; every ring-1-only element names its vertices through local ids of
+// the table, touches ring 1 and no own vertex; each ring-1 vertex's gather
+// list equals its incident (element, corner) slots in ascending order over
+// both element lists.
+template <typename ClothOf>
+int pair_tile_check(const HostPlan& h, int t, const std::vector<int>& tris,
+                    const std::vector<int>& quads, const std::vector<int>& vt_ptr,
+                    const std::vector<int>& vt_ids, const std::vector<int>& ve_ptr,
+                    const std::vector<int>& ve_ids, ClothOf cloth_of)
+{
+    const FusedPlan& p = h.meta;
+    const int V = p.tile_verts, vbeg = t * V, nown = V;
+    const int* ti = &h.info[(size_t)t * kTileInfo];
+    const int* pi = &h.pinfo[(size_t)t * kPairInfo];
+    const int nh1 = ti[0], tbeg = ti[1], ntri = ti[2], ebeg = ti[3], nhin = ti[4];
+    const int nh2 = pi[0], tbeg2 = pi[1], ntri2 = pi[2], ebeg2 = pi[3], nhin2 = pi[4];
+    const size_t pgbeg = (size_t)pi[5] * 2;
+    const int pglen = pi[6];
+    const int nl = nown + nh1, nl2 = nl + nh2;
+    if (nh2 < 0 || nh1 + nh2 > p.pstride || nl2 > p.plmax || ntri + ntri2 > p.ptmax ||
+        nhin + nhin2 > p.phmax || nl > p.pcmax || pglen < 0 || pglen > p.pgmax)
+        return fail(CLH_ERR_INVALID, "plan: pair maxima");
+    if (tbeg2 < 0 || (size_t)tbeg2 + ntri2 > h.ptri_idx.size() || ebeg2 < 0 ||
+        (size_t)ebeg2 + nhin2 > h.phin_idx.size() ||
+        pgbeg + (size_t)((pglen + 1) & ~1) > h.pgl_idx.size())
+        return fail(CLH_ERR_INVALID, "plan: pair ranges");
+    const int* ph = &h.phalo[(size_t)t * p.pstride];
+    auto pid = [&](int l) { return l < nown ? vbeg + l : ph[l - nown]; };
+    for (int l = nown; l < nl; ++l)
+        if (pid(l) != h.halo[(size_t)t * p.hstride + l - nown])
+            return fail(CLH_ERR_INVALID, "plan: pair ring 1 differs from the halo");
+    std::vector<int> ids;
+    for (int l = nown; l < nl2; ++l) {
+        const int g = pid(l);
+        if (g < 0 || g >= p.N || (g >= vbeg && g < vbeg + nown))
+            return fail(CLH_ERR_INVALID, "plan: pair halo id");
+        if (h.perm[(size_t)g] < 0 || cloth_of(h.perm[(size_t)g]) != h.tile_cloth[(size_t)t])
+            return fail(CLH_ERR_INVALID, "plan: pair halo vertex outside the tile's cloth");
+        ids.push_back(g);
+    }
+    std::sort(ids.begin(), ids.end());
+    if (std::adjacent_find(ids.begin(), ids.end()) != ids.end())
+        return fail(CLH_ERR_INVALID, "plan: pair halo repeats a vertex");
+    const int T = (int)(tris.size() / 3), E = (int)(quads.size() / 4);
+    for (int j = 0; j < ntri2; ++j) {
+        const uint2 id = h.ptri_idx[(size_t)tbeg2 + j];
+        const int e = h.ptri_gid[(size_t)tbeg2 + j];
+        const int l[3] = {(int)(id.x & 0xFFFF), (int)(id.x >> 16), (int)(id.y & 0xFFFF)};
+        if (e < 0 || e >= T) return fail(CLH_ERR_INVALID, "plan: pair triangle id");
+        bool ring1 = false;
+        for (int c = 0; c < 3; ++c) {
+            if (l[c] >= nl2 || h.perm[(size_t)pid(l[c])] != tris[(size_t)e * 3 + c])
+                return fail(CLH_ERR_INVALID, "plan: pair triangle local index");
+            if (l[c] < nown) return fail(CLH_ERR_INVALID, "plan: pair triangle touches an own vertex");
+            ring1 = ring1 || l[c] < nl;
+        }
+        if (!ring1) return fail(CLH_ERR_INVALID, "plan: pair triangle misses ring 1");
+    }
+    for (int j = 0; j < nhin2; ++j) {
+        const uint2 id = h.phin_idx[(size_t)ebeg2 + j];
+        const int e = h.phin_gid[(size_t)ebeg2 + j];
+        const int l[4] = {(int)(id.x & 0xFFFF), (int)(id.x >> 16), (int)(id.y & 0xFFFF), (int)(id.y >> 16)};
+        if (e < 0 || e >= E) return fail(CLH_ERR_INVALID, "plan: pair hinge id");
+        bool ring1 = false;
+        for (int c = 0; c < 4; ++c) {
+            if (l[c] >= nl2 || h.perm[(size_t)pid(l[c])] != quads[(size_t)e * 4 + c])
+                return fail(CLH_ERR_INVALID, "plan: pair hinge local index");
+            if (l[c] < nown) return fail(CLH_ERR_INVALID, "plan: pair hinge touches an own vertex");
+            ring1 = ring1 || l[c] < nl;
+        }
+        if (!ring1) return fail(CLH_ERR_INVALID, "plan: pair hinge misses ring 1");
+    }
+    for (int i = 0; i < nh1; ++i) {
+        const int o = h.perm[(size_t)pid(nown + i)];
+        const unsigned m = h.pmeta[(size_t)t * p.hstride + i];
+        const int j0 = (int)(m & 0xFFFFu), nt = (int)((m >> 16) & 0xFFu), na = (int)(m >> 24);
+        if (nt > na || j0 + na > pglen) return fail(CLH_ERR_INVALID, "plan: pair gather meta");
+        std::vector<int> got;
+        for (int j = j0; j < j0 + na; ++j) {
+            const int sl = h.pgl_idx[pgbeg + (size_t)j];
+            if (j < j0 + nt) {
+                const int k = sl / 3;
+                if (k >= ntri + ntri2) return fail(CLH_ERR_INVALID, "plan: pair gather triangle slot");
+                const int e = k < ntri ? h.tri_gid[(size_t)tbeg + k] : h.ptri_gid[(size_t)tbeg2 + k - ntri];
+                got.push_back(e * 3 + sl % 3);
+            } else {
+                const int k = sl / 4;
+                if (k >= nhin + nhin2) return fail(CLH_ERR_INVALID, "plan: pair gather hinge slot");
+                const int e = k < nhin ? h.hin_gid[(size_t)ebeg + k] : h.phin_gid[(size_t)ebeg2 + k - nhin];
+                got.push_back(-1 - (e * 4 + sl % 4));
+            }
+        }
+        std::vector<int> want;
+        for (int j = vt_ptr[(size_t)o]; j < vt_ptr[(size_t)o + 1]; ++j) want.push_back(vt_ids[(size_t)j]);
+        for (int j = ve_ptr[(size_t)o]; j < ve_ptr[(size_t)o + 1]; ++j) want.push_back(-1 - ve_ids[(size_t)j]);
+        if (got != want) return fail(CLH_ERR_INVALID, "plan: pair gather list differs from element order");
+    }
+    return CLH_OK;
+}
+
 int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::vector<int>& quads,
                     const std::vector<int>& seg_in)
 {
@@ -1202,17 +1884,18 @@ int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::
             for (int j = ve_ptr[(size_t)o]; j < ve_ptr[(size_t)o + 1]; ++j) want.push_back(-1 - ve_ids[(size_t)j]);
             if (got != want) return fail(CLH_ERR_INVALID, "plan: gather list differs from element order");
         }
+        if (p.pair) CLH_TRY(pair_tile_check(h, t, tris, quads, vt_ptr, vt_ids, ve_ptr, ve_ids, cloth_of));
     }
     return CLH_OK;
 }
 
 int fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
                      const std::vector<int>& quads, const std::vector<float>& pos, int tile_verts,
-                     const std::vector<int>& seg)
+                     const std::vector<int>& seg, bool pair)
 {
     fused_plan_release(p);
     HostPlan h;
-    CLH_TRY(host_plan_build(h, N, tris, quads, pos, tile_verts, seg));
+    CLH_TRY(host_plan_build(h, N, tris, quads, pos, tile_verts, seg, pair));
     const FusedPlan& m = h.meta;
     p.N = m.N; p.num_verts = m.num_verts; p.num_cloths = m.num_cloths; p.num_tiles = m.num_tiles; p.tile_verts = m.tile_verts; p.hstride = m.hstride;
     p.lmax = m.lmax; p.tmax = m.tmax; p.hmax = m.hmax; p.gmax = m.gmax; p.vmax = m.vmax;
@@ -1236,6 +1919,25 @@ int fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
     CLH_TRY(p.gl_meta.upload(h.gl_meta));
     CLH_TRY(p.tile_cloth.upload(h.tile_cloth));
     CLH_TRY(p.cloth_tiles.upload(h.cloth_tiles));
+    p.pair = m.pair;
+    if (p.pair) {
+        p.pstride = m.pstride; p.plmax = m.plmax; p.ptmax = m.ptmax; p.phmax = m.phmax;
+        p.pgmax = m.pgmax; p.pcmax = m.pcmax; p.pbytes_per_launch = m.pbytes_per_launch;
+        CLH_TRY(p.pinfo.upload(h.pinfo));
+        CLH_TRY(p.phalo.upload(h.phalo));
+        CLH_TRY(p.pmeta.upload(h.pmeta));
+        CLH_TRY(p.ptri_gid.upload(h.ptri_gid));
+        CLH_TRY(p.ptri_idx.upload(h.ptri_idx));
+        CLH_TRY(p.phin_gid.upload(h.phin_gid));
+        CLH_TRY(p.phin_idx.upload(h.phin_idx));
+        CLH_TRY(p.ptri_c0.resize(h.ptri_gid.size()));
+        CLH_TRY(p.ptri_c1.resize(h.ptri_gid.size()));
+        CLH_TRY(p.phin_w.resize(h.phin_gid.size()));
+        CLH_TRY(p.phin_c.resize(h.phin_gid.size()));
+        std::vector<unsigned> pw(h.pgl_idx.size() / 2);
+        std::memcpy(pw.data(), h.pgl_idx.data(), pw.size() * sizeof(unsigned));
+        CLH_TRY(p.pgl_words.upload(pw));
+    }
     std::memset(p.packed_src, 0, sizeof(p.packed_src));
     p.packed_h2 = -1.0f;
     return CLH_OK;
@@ -1255,6 +1957,17 @@ int fused_pack_constants(FusedPlan& p, const clh_step_args& a, float h2, hipStre
         hipLaunchKernelGGL(k_pack_hinges, dim3(blocks_for(nh)), dim3(256), 0, stream,
                            p.hin_gid.ptr, a.d_bend_w, a.d_bend_n, a.d_bend_k, p.hin_w.ptr,
                            p.hin_c.ptr, nh, h2);
+    if (p.pair) {
+        const int pt = (int)p.ptri_gid.count, ph = (int)p.phin_gid.count;
+        if (a.d_Dm_inv && a.d_rest_area)
+            hipLaunchKernelGGL(k_pack_tris, dim3(blocks_for(pt)), dim3(256), 0, stream,
+                               p.ptri_gid.ptr, a.d_Dm_inv, a.d_rest_area, a.d_stretch_k,
+                               p.ptri_c0.ptr, p.ptri_c1.ptr, pt, h2);
+        if (a.d_bend_w && a.d_bend_n && a.d_bend_k)
+            hipLaunchKernelGGL(k_pack_hinges, dim3(blocks_for(ph)), dim3(256), 0, stream,
+                               p.phin_gid.ptr, a.d_bend_w, a.d_bend_n, a.d_bend_k, p.phin_w.ptr,
+                               p.phin_c.ptr, ph, h2);
+    }
     CLH_HIP(hipGetLastError());
     std::memcpy(p.packed_src, src, sizeof(src));
     p.packed_h2 = h2;
@@ -1284,34 +1997,17 @@ bool fused_persist_fits(const FusedPlan& p)
     return (long long)per * cus >= p.num_tiles;
 }
 
-int fused_launch_persist(const FusedPlan& p, const FusedIterArgs& a, double* const it[3], int k0,
+int fused_launch_persist(const FusedPlan& p, const FusedIterArgs& a, double* const it[kIterBufs], int k0,
                          int k1, int K, int check_every, FrameCtl* ctl, double* partials,
                          double* hist, unsigned* bar, hipStream_t stream)
 {
     PersistArgs P;
     FusedKernelArgs& A = P.A;
-    A.it = a;
-    A.tile_info = p.tile_info.ptr;
-    A.halo = p.halo.ptr;
-    A.tri_idx = p.tri_idx.ptr;
-    A.tri_c0 = p.tri_c0.ptr;
-    A.tri_c1 = p.tri_c1.ptr;
-    A.hin_idx = p.hin_idx.ptr;
-    A.hin_w = p.hin_w.ptr;
-    A.hin_c = p.hin_c.ptr;
-    A.gl_words = p.gl_words.ptr;
-    A.N = p.N;
-    A.tile_verts = p.tile_verts;
-    A.hstride = p.hstride;
-    A.lmax = p.lmax;
-    A.tmax = p.tmax;
-    A.hmax = p.hmax;
-    A.num_tiles = p.num_tiles;
-    A.xcd_remap = p.xcd_remap;
+    fill_kernel_args(p, a, A);
     const PersistLayout L = persist_layout(p);
     A.off_g = (int)L.it.off_g;
     A.off_r = (int)L.it.off_r;
-    for (int i = 0; i < 3; ++i) P.it[i] = it[i];
+    for (int i = 0; i < kIterBufs; ++i) P.it[i] = it[i];
     P.k0 = k0;
     P.k1 = k1;
     P.K = K;
@@ -1329,28 +2025,55 @@ int fused_launch_persist(const FusedPlan& p, const FusedIterArgs& a, double* con
                       "persistent iteration launch");
 }
 
+size_t fused_pair_lds_bytes(const FusedPlan& p, bool check) { return pair_lds_layout(p, check).total; }
+
+bool fused_pair_usable(const FusedPlan& p) { return p.pair && p.pcmax <= p.threads; }
+
+int fused_launch_pair(const FusedPlan& p, const FusedIterArgs& a, void* out2, float omega1,
+                      int blend1, hipStream_t stream)
+{
+    if (!fused_pair_usable(p))
+        return fail(CLH_ERR_UNSUPPORTED, "two-iteration launch: no pair plan for this workgroup size");
+    PairKernelArgs P;
+    fill_kernel_args(p, a, P.A);
+    const bool check = a.partials != nullptr;
+    const PairLdsLayout L = pair_lds_layout(p, check);
+    P.A.off_g = (int)L.off_g;
+    P.A.off_r = (int)L.off_r;
+    P.off_g2 = (int)L.off_g2;
+    P.out2 = out2;
+    P.omega1 = omega1;
+    P.blend1 = blend1;
+    P.pinfo = p.pinfo.ptr;
+    P.phalo = p.phalo.ptr;
+    P.pmeta = p.pmeta.ptr;
+    P.ptri_idx = p.ptri_idx.ptr;
+    P.ptri_c0 = p.ptri_c0.ptr;
+    P.ptri_c1 = p.ptri_c1.ptr;
+    P.phin_idx = p.phin_idx.ptr;
+    P.phin_w = p.phin_w.ptr;
+    P.phin_c = p.phin_c.ptr;
+    P.pgl_words = p.pgl_words.ptr;
+    P.pstride = p.pstride;
+    P.plmax = p.plmax;
+    P.ptmax = p.ptmax;
+    P.phmax = p.phmax;
+    const dim3 grid(p.num_tiles);
+    if (p.threads == 512) {
+        if (check) hipLaunchKernelGGL((k_fused_pair<true, 512>), grid, dim3(512), L.total, stream, P);
+        else       hipLaunchKernelGGL((k_fused_pair<false, 512>), grid, dim3(512), L.total, stream, P);
+    } else {
+        if (check) hipLaunchKernelGGL((k_fused_pair<true, 256>), grid, dim3(256), L.total, stream, P);
+        else       hipLaunchKernelGGL((k_fused_pair<false, 256>), grid, dim3(256), L.total, stream, P);
+    }
+    return hip_status(hipGetLastError(), "two-iteration launch");
+}
+
 int fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
                            hipStream_t stream)
 {
     FusedKernelArgs A;
-    A.it = a;
-    A.tile_info = p.tile_info.ptr;
-    A.halo = p.halo.ptr;
-    A.tri_idx = p.tri_idx.ptr;
-    A.tri_c0 = p.tri_c0.ptr;
-    A.tri_c1 = p.tri_c1.ptr;
-    A.hin_idx = p.hin_idx.ptr;
-    A.hin_w = p.hin_w.ptr;
-    A.hin_c = p.hin_c.ptr;
-    A.gl_words = p.gl_words.ptr;
-    A.N = p.N;
-    A.tile_verts = p.tile_verts;
-    A.hstride = p.hstride;
-    A.lmax = p.lmax;
-    A.tmax = p.tmax;
-    A.hmax = p.hmax;
-    A.num_tiles = p.num_tiles;
-    A.xcd_remap = p.xcd_remap;
+    fill_kernel_args(p, a, A);
     const bool check = a.partials != nullptr;
     const LdsLayout L = lds_layout(p, mode, check);
     A.off_g = (int)L.off_g;
@@ -1390,7 +2113,11 @@ int fused_enable_lds(int& bytes)
                          reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 512>),
                          reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 512>),
                          reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false, 512>),
-                         reinterpret_cast<const void*>(&k_fused_persist<256>)};
+                         reinterpret_cast<const void*>(&k_fused_persist<256>),
+                         reinterpret_cast<const void*>(&k_fused_pair<true, 256>),
+                         reinterpret_cast<const void*>(&k_fused_pair<false, 256>),
+                         reinterpret_cast<const void*>(&k_fused_pair<true, 512>),
+                         reinterpret_cast<const void*>(&k_fused_pair<false, 512>)};
     for (const void* f : fns) {
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -1450,7 +2177,7 @@ extern "C" int clh_fused_plan_check(int N, int T, int E, const int* tris, const 
     if (num_cloths > 0) seg.assign(cloth_vert_offsets, cloth_vert_offsets + num_cloths + 1);
     std::vector<float> x(pos, pos + (size_t)N * 3);
     HostPlan h;
-    CLH_TRY(host_plan_build(h, N, t, q, x, tile_verts, seg));
+    CLH_TRY(host_plan_build(h, N, t, q, x, tile_verts, seg, mode == CLH_MODE_PD));
     CLH_TRY(host_plan_check(h, t, q, seg));
     if (out) {
         std::memset(out, 0, sizeof(*out));
@@ -1476,6 +2203,16 @@ void fused_plan_info(const FusedPlan& p, clh_plan_info* out)
     out->halo_bend_factor = p.bend_factor;
     out->bytes_per_iter = p.bytes_per_iter;
     out->block_threads = p.threads;
+    out->pair = p.pair;
+    out->iters_per_launch = 1;
+    if (p.pair) {
+        out->pair_halo_max = p.pstride;
+        out->pair_tri_max = p.ptmax;
+        out->pair_hinge_max = p.phmax;
+        out->pair_bytes_per_launch = p.pbytes_per_launch;
+        out->pair_compute_max = p.pcmax;
+        out->pair_lds_bytes = (double)pair_lds_layout(p, true).total;
+    }
 }
 
 void fused_plan_release(FusedPlan& p)
@@ -1496,6 +2233,19 @@ void fused_plan_release(FusedPlan& p)
     p.gl_meta.release();
     p.tile_cloth.release();
     p.cloth_tiles.release();
+    p.pinfo.release();
+    p.phalo.release();
+    p.pmeta.release();
+    p.ptri_gid.release();
+    p.phin_gid.release();
+    p.ptri_idx.release();
+    p.phin_idx.release();
+    p.ptri_c0.release();
+    p.phin_w.release();
+    p.ptri_c1.release();
+    p.phin_c.release();
+    p.pgl_words.release();
+    p.pair = 0;
     p.num_tiles = 0;
 }
 

@@ -372,15 +372,20 @@ __device__ __forceinline__ float ctl_omega(const FrameCtl& c, int ck, float prev
 // Writes the state of the iteration after the one in `s` into slot par ^ 1
 // (block 0 only: the one writer of that slot during a launch; readers use
 // `par`).
+__device__ __forceinline__ CtlSlot ctl_next(const FrameCtl& c, const CtlSlot& s)
+{
+    CtlSlot n;
+    n.ck = s.ck + 1;
+    n.omega = ctl_omega(c, n.ck, s.omega);
+    n.blend = (c.use_cheb && n.ck > 0 && n.omega != 1.0f) ? 1 : 0;
+    n.k = s.k + 1;
+    return n;
+}
+
 __device__ __forceinline__ void ctl_write_next(FrameCtl* c, int par, const CtlSlot& s)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    CtlSlot n;
-    n.ck = s.ck + 1;
-    n.omega = ctl_omega(*c, n.ck, s.omega);
-    n.blend = (c->use_cheb && n.ck > 0 && n.omega != 1.0f) ? 1 : 0;
-    n.k = s.k + 1;
-    c->slot[par ^ 1] = n;
+    c->slot[par ^ 1] = ctl_next(*c, s);
 }
 
 // Reads the state of iteration slot `par`; false when the frame has stopped.
@@ -394,19 +399,20 @@ __device__ __forceinline__ bool ctl_begin(FrameCtl* c, int par, float& omega, in
     return true;
 }
 
-// After the residual of a check iteration (slot `par`) has been summed:
-// history, stop on convergence (`conv`: every cloth meets the tolerance) or a
-// non-finite residual, and a Chebyshev restart when the residual grew since
-// the previous check. Every third restart in a frame halves 1 - rho. One
-// thread.
+// After the residual of a check iteration has been summed: history, stop on
+// convergence (`conv`: every cloth meets the tolerance) or a non-finite
+// residual, and a Chebyshev restart when the residual grew since the previous
+// check. Every third restart in a frame halves 1 - rho. The check iteration
+// is slot `par`'s iteration plus kofs (1 when a two-iteration launch checked
+// its second). One thread.
 __device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, double b2, double* hist,
-                                          bool conv)
+                                          bool conv, int kofs = 0)
 {
     const int slot = c->checks;
     hist[2 * slot + 0] = r2;
     hist[2 * slot + 1] = b2;
     c->checks = slot + 1;
-    const int k = c->slot[par].k;
+    const int k = c->slot[par].k + kofs;
     if (!(r2 == r2) || r2 > 1.7e308) {
         c->nonfinite = 1;
         c->stop = 1;

@@ -38,6 +38,12 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxIterations = 1 << 20;
+// Iterations per fused PD launch when the configuration leaves it at 0. One:
+// on MI355X the two-iteration kernel is slower per iteration at every size
+// measured (7.7 against 7.2 us at 256x256, 139 against 101 us at 1024x1024,
+// 735 against 599 us for 512 batched cloths), because its ring-1 elements
+// and extra element passes cost more than the launch and memory trip saved.
+constexpr int kDefaultItersPerLaunch = 1;
 
 inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
@@ -320,7 +326,7 @@ __global__ void __launch_bounds__(kBlock) k_vertex_pd(VertexArgs a)
 // Sums the per-block partials in a fixed order into hist[2*slot .. +1].
 __global__ void __launch_bounds__(kBlock)
 k_reduce_partials(const double* __restrict__ partials, int nblocks, double* __restrict__ hist,
-                  int slot, FrameCtl* __restrict__ ctl, int par)
+                  int slot, FrameCtl* __restrict__ ctl, int par, int kofs)
 {
     if (ctl && ctl->stop) return;
     double r = 0.0, b = 0.0;
@@ -341,7 +347,7 @@ k_reduce_partials(const double* __restrict__ partials, int nblocks, double* __re
     }
     if (threadIdx.x == 0) {
         if (ctl) {
-            ctl_check(ctl, par, sr[0], sb[0], hist, sr[0] <= ctl->tol2 * sb[0]);
+            ctl_check(ctl, par, sr[0], sb[0], hist, sr[0] <= ctl->tol2 * sb[0], kofs);
         } else {
             hist[2 * slot + 0] = sr[0];
             hist[2 * slot + 1] = sb[0];
@@ -358,7 +364,7 @@ constexpr int kReduceCloths = 1024;
 __global__ void __launch_bounds__(kReduceCloths)
 k_reduce_cloths(const double* __restrict__ partials, const int* __restrict__ cloth_tiles, int C,
                 double* __restrict__ cloth_res, double* __restrict__ hist, int slot,
-                FrameCtl* __restrict__ ctl, int par)
+                FrameCtl* __restrict__ ctl, int par, int kofs)
 {
     if (ctl && ctl->stop) return;
     constexpr int kWaves = kReduceCloths / 64;
@@ -402,7 +408,7 @@ k_reduce_cloths(const double* __restrict__ partials, const int* __restrict__ clo
             conv &= sc[i];
         }
         if (ctl) {
-            ctl_check(ctl, par, tr, tb, hist, conv != 0);
+            ctl_check(ctl, par, tr, tb, hist, conv != 0, kofs);
         } else {
             hist[2 * slot + 0] = tr;
             hist[2 * slot + 1] = tb;
@@ -523,6 +529,8 @@ struct clh_solver {
     int num_cloths = 1;
     unsigned long long shape_key = 0;  // autotuner cache key of the mesh shape
     bool persist = false;        // tolerance frames in persistent cooperative launches
+    int ipl = 1;                 // iterations per fused launch (2: two-iteration kernel)
+    bool pair_fits = false;      // the plan's two-iteration tables fit in LDS
     DevBuf<unsigned> bar;        // grid barrier words of the persistent kernel
     DevBuf<double> cloth_res;    // per-cloth (r^2, b^2) of the last residual evaluation
     int variant = CLH_KERNEL_UNFUSED;
@@ -532,7 +540,7 @@ struct clh_solver {
 
     // scratch
     DevBuf<float> y, xprev, fr, fd, bp, bd;
-    DevBuf<double> it[3];  // iterates: float triples (reference) or double (PD)
+    DevBuf<double> it[kIterBufs];  // iterates: float triples (reference) or double (PD)
     DevBuf<int> pin_map;
     DevBuf<double> partials, hist;
     int nblocks_v = 0;
@@ -549,12 +557,13 @@ struct clh_solver {
 
     // frames run to a tolerance: device control block, its pinned host
     // snapshots (one per chunk in flight), and graphs of one chunk of
-    // check_every iterations, by the chunk's first iteration modulo 6 (the
-    // iterate buffers rotate by 3, the control slots by 2)
+    // check_every iterations, by the chunk's first iteration modulo 4 (the
+    // iterate buffers rotate by 4; the control slots alternate per launch,
+    // every iteration or, with two-iteration launches, every second one)
     DevBuf<FrameCtl> ctl;
     FrameCtl* h_ctl = nullptr;
     hipEvent_t ev_chunk[2] = {nullptr, nullptr};
-    hipGraphExec_t chunk_exec[6] = {};
+    hipGraphExec_t chunk_exec[kIterBufs] = {};
     clh_step_args chunk_args = {};
     int chunk_len = 0;
 
@@ -580,12 +589,13 @@ void drop_chunk_graphs(clh_solver* s)
 }
 
 // FNV-1a over the counts that decide the fused kernel's cost: vertices,
-// triangles, hinges, cloths, mode, and the histogram of (triangle, hinge)
-// valence per vertex.
+// triangles, hinges, cloths, mode, whether two-iteration launches are
+// candidates, and the histogram of (triangle, hinge) valence per vertex.
 unsigned long long shape_key(const clh_solver& s, const std::vector<int>& tptr,
                              const std::vector<int>& eptr)
 {
-    std::vector<long long> words = {s.N, s.T, s.E, s.num_cloths, s.cfg.mode};
+    std::vector<long long> words = {s.N, s.T, s.E, s.num_cloths, s.cfg.mode,
+                                    s.cfg.iters_per_launch == 2 ? 1 : 0};
     std::vector<long long> hist(32 * 32, 0);
     for (int v = 0; v < s.N; ++v) {
         const int nt = std::min(31, tptr[(size_t)v + 1] - tptr[(size_t)v]);
@@ -632,6 +642,7 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
     s->shape_key = shape_key(*s, tptr, ptr);
 
     s->variant = CLH_KERNEL_UNFUSED;
+    s->pair_fits = false;
     if (s->cfg.kernel_variant != CLH_KERNEL_UNFUSED && N > 0 && a.d_pos) {
         std::vector<float> pos((size_t)N * 3);
         CLH_HIP(hipMemcpy(pos.data(), a.d_pos, pos.size() * sizeof(float), hipMemcpyDeviceToHost));
@@ -640,13 +651,17 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
         CLH_HIP(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         CLH_TRY(fused_enable_lds(lds_limit));
         int st = CLH_ERR_UNSUPPORTED;
+        const bool want_pair = s->cfg.mode == CLH_MODE_PD && s->cfg.iters_per_launch == 2;
         for (int tv = s->tile_verts; tv >= 32; tv /= 2) {
-            st = fused_plan_build(s->fused, N, tris, quads, pos, tv, s->seg);
+            st = fused_plan_build(s->fused, N, tris, quads, pos, tv, s->seg, want_pair);
             if (st != CLH_OK) break;
             s->fused.threads = s->block_threads;
             if (fused_lds_bytes(s->fused, s->cfg.mode, true) <= (size_t)lds_limit) break;
             st = fail(CLH_ERR_UNSUPPORTED, "fused plan: tile does not fit in LDS");
         }
+        // two-iteration launches only where their larger tables fit
+        s->pair_fits = st == CLH_OK && s->fused.pair &&
+                       fused_pair_lds_bytes(s->fused, true) <= (size_t)lds_limit;
         if (st == CLH_OK) s->variant = CLH_KERNEL_FUSED;
         else if (s->cfg.kernel_variant == CLH_KERNEL_FUSED || s->num_cloths > 1) return st;
     }
@@ -748,15 +763,33 @@ int enqueue_prologue(clh_solver* s, const clh_step_args& a)
     return hip_status(hipGetLastError(), "prologue launch");
 }
 
-// Enqueues iteration k (x_k in it[k%3] -> x_{k+1} in it[(k+1)%3]).
-int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega, bool blend_in,
-                      bool check, int check_slot, bool device_ctl = false)
+// Residual reduction after a fused check launch: per-cloth sums for a batch,
+// one sum otherwise; kofs = 1 when the check was a two-iteration launch's
+// second iteration.
+int enqueue_fused_reduce(clh_solver* s, int check_slot, FrameCtl* ctl, int par, int kofs)
 {
+    if (s->num_cloths > 1)
+        hipLaunchKernelGGL(k_reduce_cloths, dim3(1), dim3(kReduceCloths), 0, s->stream,
+                           s->partials.ptr, s->fused.cloth_tiles.ptr, s->num_cloths,
+                           s->cloth_res.ptr, s->hist.ptr, check_slot, ctl, par, kofs);
+    else
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->stream,
+                           s->partials.ptr, s->fused.num_tiles, s->hist.ptr, check_slot, ctl, par,
+                           kofs);
+    return hip_status(hipGetLastError(), "residual reduction launch");
+}
+
+// Enqueues iteration k (x_k in it[k%4], x_{k-1} in it[(k+3)%4] -> x_{k+1} in
+// it[(k+1)%4]); par is the control slot of the launch (default k & 1).
+int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega, bool blend_in,
+                      bool check, int check_slot, bool device_ctl = false, int par = -1)
+{
+    if (par < 0) par = k & 1;
     const clh_config& c = s->cfg;
     FrameCtl* ctl = device_ctl ? s->ctl.ptr : nullptr;
     const float h2 = c.dt * c.dt;
     const int pd = (c.mode == CLH_MODE_PD);
-    const void* xk = s->it[k % 3].ptr;
+    const void* xk = s->it[k % kIterBufs].ptr;
     const bool use_stretch = s->T > 0 && a.d_stretch_k != nullptr;
     const bool use_bend = s->E > 0 && a.d_bend_quads != nullptr;
     const bool blend = pd && c.use_chebyshev && blend_in && omega != 1.0f;
@@ -764,8 +797,8 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
     if (s->variant == CLH_KERNEL_FUSED) {
         FusedIterArgs f;
         f.x = xk;
-        f.xkm1 = s->it[(k + 2) % 3].ptr;
-        f.out = s->it[(k + 1) % 3].ptr;
+        f.xkm1 = s->it[(k + kIterBufs - 1) % kIterBufs].ptr;
+        f.out = s->it[(k + 1) % kIterBufs].ptr;
         f.base = s->xprev.ptr;
         f.y = s->y.ptr;
         f.vinfo = s->vinfo.ptr;
@@ -776,16 +809,9 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
         f.use_stretch = use_stretch ? 1 : 0;
         f.use_bend = use_bend ? 1 : 0;
         f.ctl = ctl;
-        f.par = k & 1;
+        f.par = par;
         CLH_TRY(fused_launch_iteration(s->fused, f, c.mode, s->stream));
-        if (check && s->num_cloths > 1)
-            hipLaunchKernelGGL(k_reduce_cloths, dim3(1), dim3(kReduceCloths), 0, s->stream,
-                               s->partials.ptr, s->fused.cloth_tiles.ptr, s->num_cloths,
-                               s->cloth_res.ptr, s->hist.ptr, check_slot, ctl, k & 1);
-        else if (check)
-            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->stream,
-                               s->partials.ptr, s->fused.num_tiles, s->hist.ptr, check_slot, ctl,
-                               k & 1);
+        if (check) CLH_TRY(enqueue_fused_reduce(s, check_slot, ctl, par, 0));
         return hip_status(hipGetLastError(), "fused iteration launch");
     }
 
@@ -812,8 +838,8 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
     VertexArgs va;
     va.base = s->xprev.ptr;
     va.x = xk;
-    va.xkm1 = s->it[(k + 2) % 3].ptr;
-    va.out = s->it[(k + 1) % 3].ptr;
+    va.xkm1 = s->it[(k + kIterBufs - 1) % kIterBufs].ptr;
+    va.out = s->it[(k + 1) % kIterBufs].ptr;
     va.y = s->y.ptr;
     va.mass = a.d_mass;
     va.diag = a.d_jacobi_diag;
@@ -846,8 +872,44 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
     }
     if (check && s->N > 0)
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->stream, s->partials.ptr,
-                           s->nblocks_v, s->hist.ptr, check_slot, ctl, k & 1);
+                           s->nblocks_v, s->hist.ptr, check_slot, ctl, par, 0);
     return hip_status(hipGetLastError(), "iteration launch");
+}
+
+bool pair_active(const clh_solver* s)
+{
+    return s->variant == CLH_KERNEL_FUSED && s->cfg.mode == CLH_MODE_PD && s->ipl == 2 &&
+           !s->persist && s->pair_fits && fused_pair_usable(s->fused);
+}
+
+// Enqueues iterations k and k+1 in one launch (x_k in it[k%4], x_{k-1} in
+// it[(k+3)%4] -> x_{k+1} in it[(k+1)%4], x_{k+2} in it[(k+2)%4]); a check
+// measures iteration k+1.
+int enqueue_pair(clh_solver* s, const clh_step_args& a, int k, float omega0, bool blend0_in,
+                 float omega1, bool blend1_in, bool check, int check_slot, bool device_ctl, int par)
+{
+    const clh_config& c = s->cfg;
+    FrameCtl* ctl = device_ctl ? s->ctl.ptr : nullptr;
+    FusedIterArgs f;
+    f.x = s->it[k % kIterBufs].ptr;
+    f.xkm1 = s->it[(k + kIterBufs - 1) % kIterBufs].ptr;
+    f.out = s->it[(k + 1) % kIterBufs].ptr;
+    f.base = s->xprev.ptr;
+    f.y = s->y.ptr;
+    f.vinfo = s->vinfo.ptr;
+    f.pin_target = a.d_pin_target;
+    f.partials = check ? s->partials.ptr : nullptr;
+    f.omega = omega0;
+    f.blend = (c.use_chebyshev && blend0_in && omega0 != 1.0f) ? 1 : 0;
+    f.use_stretch = (s->T > 0 && a.d_stretch_k != nullptr) ? 1 : 0;
+    f.use_bend = (s->E > 0 && a.d_bend_quads != nullptr) ? 1 : 0;
+    f.ctl = ctl;
+    f.par = par;
+    const int blend1 = (c.use_chebyshev && blend1_in && omega1 != 1.0f) ? 1 : 0;
+    CLH_TRY(fused_launch_pair(s->fused, f, s->it[(k + 2) % kIterBufs].ptr, omega1, blend1,
+                              s->stream));
+    if (check) CLH_TRY(enqueue_fused_reduce(s, check_slot, ctl, par, 1));
+    return hip_status(hipGetLastError(), "two-iteration launch");
 }
 
 int enqueue_epilogue(clh_solver* s, const clh_step_args& a, int iters)
@@ -855,11 +917,11 @@ int enqueue_epilogue(clh_solver* s, const clh_step_args& a, int iters)
     const clh_config& c = s->cfg;
     const float scale = (1.0f - c.damping) / c.dt;
     if (s->variant == CLH_KERNEL_FUSED)
-        return fused_epilogue(s->fused, a, s->it[iters % 3].ptr, s->xprev.ptr, s->vinfo.ptr,
+        return fused_epilogue(s->fused, a, s->it[iters % kIterBufs].ptr, s->xprev.ptr, s->vinfo.ptr,
                               scale, c.mode == CLH_MODE_PD ? 1 : 0, s->stream);
     if (s->N > 0)
         hipLaunchKernelGGL(k_epilogue, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream,
-                           s->it[iters % 3].ptr, s->xprev.ptr, s->pin_map.ptr, a.d_pin_target,
+                           s->it[iters % kIterBufs].ptr, s->xprev.ptr, s->pin_map.ptr, a.d_pin_target,
                            a.d_pos, a.d_vel, s->N, scale, c.mode == CLH_MODE_PD ? 1 : 0);
     return hip_status(hipGetLastError(), "epilogue launch");
 }
@@ -868,12 +930,12 @@ int enqueue_epilogue(clh_solver* s, const clh_step_args& a, int iters)
 int enqueue_trace(clh_solver* s, const clh_step_args& a, int k)
 {
     if (s->variant == CLH_KERNEL_FUSED)
-        return fused_trace(s->fused, s->it[(k + 1) % 3].ptr, s->xprev.ptr,
+        return fused_trace(s->fused, s->it[(k + 1) % kIterBufs].ptr, s->xprev.ptr,
                            a.d_trace + (size_t)k * s->N * 3, s->cfg.mode == CLH_MODE_PD ? 1 : 0,
                            s->stream);
     if (s->N > 0)
         hipLaunchKernelGGL(k_trace, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream,
-                           s->it[(k + 1) % 3].ptr, s->xprev.ptr,
+                           s->it[(k + 1) % kIterBufs].ptr, s->xprev.ptr,
                            a.d_trace + (size_t)k * s->N * 3, s->N,
                            s->cfg.mode == CLH_MODE_PD ? 1 : 0);
     return hip_status(hipGetLastError(), "trace launch");
@@ -883,15 +945,28 @@ int enqueue_trace(clh_solver* s, const clh_step_args& a, int k)
 int enqueue_fixed_frame(clh_solver* s, const clh_step_args& a)
 {
     const int K = s->cfg.max_iterations;
+    const bool pairs = pair_active(s) && !a.d_trace;
     CLH_TRY(enqueue_prologue(s, a));
     float omega = 1.0f;
     int checks = 0;
-    for (int k = 0; k < K; ++k) {
-        omega = chebyshev_omega(s->cfg, k, omega);
+    for (int k = 0; k < K;) {
+        const float w0 = chebyshev_omega(s->cfg, k, omega);
+        if (pairs && k + 1 < K && !is_check_iteration(s->cfg, k)) {
+            const float w1 = chebyshev_omega(s->cfg, k + 1, w0);
+            const bool check = is_check_iteration(s->cfg, k + 1);
+            CLH_TRY(enqueue_pair(s, a, k, w0, k > 0, w1, true, check, check ? checks : 0, false,
+                                 (k >> 1) & 1));
+            if (check) ++checks;
+            omega = w1;
+            k += 2;
+            continue;
+        }
+        omega = w0;
         const bool check = is_check_iteration(s->cfg, k);
         CLH_TRY(enqueue_iteration(s, a, k, omega, k > 0, check, check ? checks : 0));
         if (check) ++checks;
         if (a.d_trace) CLH_TRY(enqueue_trace(s, a, k));
+        ++k;
     }
     CLH_TRY(enqueue_epilogue(s, a, K));
     s->last_iters = K;
@@ -977,10 +1052,23 @@ int step_impl(clh_solver* s, const clh_step_args& a)
                        c.use_chebyshev ? 1 : 0, c.cheb_delay,
                        (double)c.tolerance * (double)c.tolerance);
     CLH_HIP(hipGetLastError());
+    // two-iteration launches when every check falls on a pair's second
+    // iteration: chunks of even length start at even k, so pairs run at even
+    // k, and a last odd iteration (K odd) runs alone; the control slot then
+    // alternates per launch, (k >> 1) & 1
+    const bool pairs = pair_active(s) && !a.d_trace && P % 2 == 0;
     auto enqueue_chunk = [&](int k0, int k1) -> int {
-        for (int k = k0; k < k1; ++k) {
-            CLH_TRY(enqueue_iteration(s, a, k, 1.0f, false, is_check_iteration(c, k), 0, true));
+        for (int k = k0; k < k1;) {
+            if (pairs && k + 1 < k1 && !is_check_iteration(c, k)) {
+                CLH_TRY(enqueue_pair(s, a, k, 1.0f, false, 1.0f, false, is_check_iteration(c, k + 1),
+                                     0, true, (k >> 1) & 1));
+                k += 2;
+                continue;
+            }
+            CLH_TRY(enqueue_iteration(s, a, k, 1.0f, false, is_check_iteration(c, k), 0, true,
+                                      pairs ? (k >> 1) & 1 : k & 1));
             if (a.d_trace) CLH_TRY(enqueue_trace(s, a, k));
+            ++k;
         }
         return CLH_OK;
     };
@@ -994,11 +1082,11 @@ int step_impl(clh_solver* s, const clh_step_args& a)
             f.pin_target = a.d_pin_target;
             f.use_stretch = (s->T > 0 && a.d_stretch_k != nullptr) ? 1 : 0;
             f.use_bend = (s->E > 0 && a.d_bend_quads != nullptr) ? 1 : 0;
-            double* const its[3] = {s->it[0].ptr, s->it[1].ptr, s->it[2].ptr};
+            double* const its[kIterBufs] = {s->it[0].ptr, s->it[1].ptr, s->it[2].ptr, s->it[3].ptr};
             CLH_TRY(fused_launch_persist(s->fused, f, its, k0, k1, K, P, s->ctl.ptr,
                                          s->partials.ptr, s->hist.ptr, s->bar.ptr, s->stream));
         } else if (graphs && k1 - k0 == P) {
-            hipGraphExec_t& ex = s->chunk_exec[k0 % 6];
+            hipGraphExec_t& ex = s->chunk_exec[k0 % kIterBufs];
             if (!ex) {
                 CLH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
                 const int st = enqueue_chunk(k0, k1);
@@ -1098,6 +1186,11 @@ extern "C" int clh_solver_create(const clh_config* cfg, const clh_mesh_desc* mes
         return fail(CLH_ERR_INVALID, "clh_solver_create: block_threads must be 0, 256 or 512");
     }
     if (cfg->block_threads > 0) s->block_threads = cfg->block_threads;
+    if (cfg->iters_per_launch < 0 || cfg->iters_per_launch > 2) {
+        clh_solver_destroy(s);
+        return fail(CLH_ERR_INVALID, "clh_solver_create: iters_per_launch must be 0, 1 or 2");
+    }
+    s->ipl = cfg->iters_per_launch == 0 ? kDefaultItersPerLaunch : cfg->iters_per_launch;
     int st = hip_status(hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking),
                         "hipStreamCreate");
     if (st == CLH_OK) st = hip_status(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming), "hipEventCreate");
@@ -1225,6 +1318,35 @@ extern "C" int clh_solver_run_frames(clh_solver* s, const clh_step_args* a, int 
     return CLH_OK;
 }
 
+namespace {
+// Average duration (ms) of one launch of the iteration kernel in use, in the
+// state of the frame loop's steady phase: a fresh prologue leaves x_0 in
+// it[0], iteration 0 writes x_1 to it[1], and the timed launches re-run
+// iteration 1 (and 2, with two-iteration launches) with the Chebyshev blend
+// on (in PD mode with Chebyshev), so they read x_k and x_{k-1} as the loop
+// does. Only solver scratch is written.
+int time_iterations(clh_solver* s, const clh_step_args& a, int num_launches, float* avg_ms)
+{
+    const bool pairs = pair_active(s);
+    auto launch = [&]() -> int {
+        if (pairs) return enqueue_pair(s, a, 1, 1.5f, true, 1.5f, true, false, 0, false, 0);
+        return enqueue_iteration(s, a, 1, 1.5f, true, false, 0);
+    };
+    CLH_TRY(enqueue_prologue(s, a));
+    CLH_TRY(enqueue_iteration(s, a, 0, 1.0f, false, false, 0));
+    CLH_TRY(launch());  // warm-up
+    CLH_HIP(hipStreamSynchronize(s->stream));
+    CLH_HIP(hipEventRecord(s->ev_t0, s->stream));
+    for (int i = 0; i < num_launches; ++i) CLH_TRY(launch());
+    CLH_HIP(hipEventRecord(s->ev_t1, s->stream));
+    CLH_HIP(hipEventSynchronize(s->ev_t1));
+    float ms = 0.0f;
+    CLH_HIP(hipEventElapsedTime(&ms, s->ev_t0, s->ev_t1));
+    *avg_ms = ms / (float)num_launches;
+    return CLH_OK;
+}
+}  // namespace
+
 extern "C" int clh_solver_time_iteration_kernel(clh_solver* s, const clh_step_args* a,
                                                 int num_launches, float* avg_ms)
 {
@@ -1233,19 +1355,7 @@ extern "C" int clh_solver_time_iteration_kernel(clh_solver* s, const clh_step_ar
     CLH_TRY(validate_args(s, a));
     CLH_TRY(ensure_plan(s, *a));
     CLH_TRY(ensure_scratch(s));
-    // a fresh prologue leaves a consistent iterate in it[0]; the launches
-    // below re-run iteration 0 -> 1 without touching the mesh state
-    CLH_TRY(enqueue_prologue(s, *a));
-    CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, false, 0));  // warm-up
-    CLH_HIP(hipStreamSynchronize(s->stream));
-    CLH_HIP(hipEventRecord(s->ev_t0, s->stream));
-    for (int i = 0; i < num_launches; ++i)
-        CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, false, 0));
-    CLH_HIP(hipEventRecord(s->ev_t1, s->stream));
-    CLH_HIP(hipEventSynchronize(s->ev_t1));
-    float ms = 0.0f;
-    CLH_HIP(hipEventElapsedTime(&ms, s->ev_t0, s->ev_t1));
-    *avg_ms = ms / (float)num_launches;
+    CLH_TRY(time_iterations(s, *a, num_launches, avg_ms));
     return CLH_OK;
 }
 
@@ -1254,8 +1364,10 @@ extern "C" int clh_solver_plan_info(clh_solver* s, clh_plan_info* out)
     if (!s || !out) return fail(CLH_ERR_INVALID, "clh_solver_plan_info: null pointer");
     std::memset(out, 0, sizeof(*out));
     out->variant = s->plan_ready ? s->variant : CLH_KERNEL_AUTO;
+    out->iters_per_launch = 1;
     if (s->plan_ready && s->variant == CLH_KERNEL_FUSED) fused_plan_info(s->fused, out);
     out->persistent = s->plan_ready && s->persist ? 1 : 0;
+    if (s->plan_ready && pair_active(s)) out->iters_per_launch = 2;
     return CLH_OK;
 }
 
@@ -1307,7 +1419,10 @@ extern "C" int clh_batch_cloth_residuals(clh_solver* s, double* out, int max_cou
 
 namespace {
 std::mutex g_tune_mu;
-std::map<unsigned long long, std::pair<int, int>> g_tune_cache;  // key -> (tile_verts, threads)
+struct TuneChoice {
+    int tile_verts, threads, ipl;
+};
+std::map<unsigned long long, TuneChoice> g_tune_cache;
 
 // Rebuilds the plan with the given launch shape; returns the tile size the
 // plan actually got (smaller when the tile did not fit in LDS).
@@ -1321,6 +1436,17 @@ int replan(clh_solver* s, const clh_step_args& a, int tile_verts, int threads, i
     s->fused.threads = threads;
     got = s->variant == CLH_KERNEL_FUSED ? s->fused.tile_verts : 0;
     return CLH_OK;
+}
+
+// Sets the launch shape of an already built plan (workgroup size, iterations
+// per launch); captured frames of the previous shape are dropped.
+void set_shape(clh_solver* s, int threads, int ipl)
+{
+    s->block_threads = threads;
+    s->fused.threads = threads;
+    s->ipl = ipl;
+    s->graph_valid = false;
+    drop_chunk_graphs(s);
 }
 }  // namespace
 
@@ -1341,42 +1467,43 @@ extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int nu
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(s->shape_key);
         if (it != g_tune_cache.end()) {
+            const TuneChoice c = it->second;
             int got = 0;
-            if (s->fused.tile_verts != it->second.first || s->fused.threads != it->second.second)
-                CLH_TRY(replan(s, *a, it->second.first, it->second.second, got));
+            if (s->fused.tile_verts != c.tile_verts) CLH_TRY(replan(s, *a, c.tile_verts, c.threads, got));
+            set_shape(s, c.threads, c.ipl);
             out->num_candidates = 1;
-            out->tile_verts[0] = it->second.first;
-            out->block_threads[0] = it->second.second;
+            out->tile_verts[0] = c.tile_verts;
+            out->block_threads[0] = c.threads;
+            out->iters_per_launch[0] = c.ipl;
             out->best = 0;
             out->from_cache = 1;
             return CLH_OK;
         }
     }
     const int tiles[4] = {32, 64, 128, 256}, threads[2] = {256, 512};
+    const bool try_pairs = s->cfg.mode == CLH_MODE_PD && s->cfg.iters_per_launch == 2;
     int n = 0, best = -1;
     for (int tv : tiles) {
         int got = 0;
         CLH_TRY(replan(s, *a, tv, 256, got));
         for (int th : threads) {
-            out->tile_verts[n] = tv;
-            out->block_threads[n] = th;
-            out->us_per_launch[n] = 0.0f;
-            if (got == tv) {
-                s->fused.threads = th;
-                CLH_TRY(enqueue_prologue(s, *a));
-                CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, false, 0));  // warm-up
-                CLH_HIP(hipStreamSynchronize(s->stream));
-                CLH_HIP(hipEventRecord(s->ev_t0, s->stream));
-                for (int i = 0; i < num_launches; ++i)
-                    CLH_TRY(enqueue_iteration(s, *a, 0, 1.0f, false, false, 0));
-                CLH_HIP(hipEventRecord(s->ev_t1, s->stream));
-                CLH_HIP(hipEventSynchronize(s->ev_t1));
-                float ms = 0.0f;
-                CLH_HIP(hipEventElapsedTime(&ms, s->ev_t0, s->ev_t1));
-                out->us_per_launch[n] = 1e3f * ms / (float)num_launches;
-                if (best < 0 || out->us_per_launch[n] < out->us_per_launch[best]) best = n;
+            for (int ipl = 1; ipl <= (try_pairs ? 2 : 1); ++ipl) {
+                out->tile_verts[n] = tv;
+                out->block_threads[n] = th;
+                out->iters_per_launch[n] = ipl;
+                out->us_per_iteration[n] = 0.0f;
+                if (got == tv) {
+                    set_shape(s, th, ipl);
+                    if (ipl == 1 || pair_active(s)) {
+                        float ms = 0.0f;
+                        CLH_TRY(time_iterations(s, *a, num_launches, &ms));
+                        out->us_per_iteration[n] = 1e3f * ms / (float)ipl;
+                        if (best < 0 || out->us_per_iteration[n] < out->us_per_iteration[best])
+                            best = n;
+                    }
+                }
+                ++n;
             }
-            ++n;
         }
     }
     out->num_candidates = n;
@@ -1384,8 +1511,10 @@ extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int nu
     out->best = best;
     int got = 0;
     CLH_TRY(replan(s, *a, out->tile_verts[best], out->block_threads[best], got));
+    set_shape(s, out->block_threads[best], out->iters_per_launch[best]);
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[s->shape_key] = {out->tile_verts[best], out->block_threads[best]};
+    g_tune_cache[s->shape_key] = {out->tile_verts[best], out->block_threads[best],
+                                  out->iters_per_launch[best]};
     return CLH_OK;
 }
 

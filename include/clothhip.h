@@ -116,6 +116,11 @@ typedef struct clh_config {
                                launch per chunk (grid barrier between iterations) when
                                every tile fits resident; off by default: on MI355X the
                                barrier costs more than a launch */
+    int   iters_per_launch; /* fused variant, CLH_MODE_PD: 2 runs two iterations per
+                               launch (each tile also updates its ring-1 halo in the
+                               first; same bits as one per launch) and lets the
+                               autotuner choose between 1 and 2; 1 or 0 (default) one
+                               per launch: on MI355X two are slower per iteration */
 } clh_config;
 
 enum { CLH_PERSIST_OFF = 0, CLH_PERSIST_ON = 1 };
@@ -208,10 +213,12 @@ int clh_solver_set_stream(clh_solver* s, void* stream);
 int clh_solver_run_frames(clh_solver* s, const clh_step_args* a,
                           int num_frames, float* elapsed_ms);
 
-/* Launches the dominant per-iteration kernel num_launches times back to back
- * on the solver's stream on the state left by the last frame, bracketed by
- * events, and returns the average time per launch. The mesh state is restored
- * afterwards. */
+/* Launches the dominant iteration kernel num_launches times back to back on
+ * the solver's stream, bracketed by events, and returns the average time per
+ * launch. The launches repeat one steady-phase launch (x_k and x_{k-1} read,
+ * the Chebyshev blend on in PD mode; two iterations per launch when
+ * clh_plan_info.iters_per_launch is 2) on iterates computed from the mesh
+ * state; only solver scratch is written, the mesh state is left as it was. */
 int clh_solver_time_iteration_kernel(clh_solver* s, const clh_step_args* a,
                                      int num_launches, float* avg_ms);
 
@@ -233,6 +240,14 @@ typedef struct clh_plan_info {
     int    tri_max;          /* largest triangle count of a tile (halo included) */
     int    hinge_max;        /* largest hinge count of a tile */
     int    halo_max;         /* largest halo of a tile */
+    int    pair;             /* the plan has the two-iteration extension (PD mode) */
+    int    iters_per_launch; /* iterations per launch of the fused kernel in use: 1 or 2 */
+    int    pair_halo_max;    /* largest ring-1 + ring-2 halo of a tile */
+    int    pair_tri_max;     /* largest triangle count of a two-iteration tile */
+    int    pair_hinge_max;   /* largest hinge count of a two-iteration tile */
+    double pair_bytes_per_launch; /* bytes a two-iteration launch streams (model) */
+    int    pair_compute_max; /* largest own + ring-1 count: the workgroup size it needs */
+    double pair_lds_bytes;   /* LDS per workgroup of the two-iteration kernel (check variant) */
 } clh_plan_info;
 int clh_solver_plan_info(clh_solver* s, clh_plan_info* out);
 
@@ -269,16 +284,20 @@ int clh_batch_create(const clh_config* cfg, const clh_mesh_desc* mesh, int num_c
 int clh_batch_cloth_residuals(clh_solver* s, double* out, int max_count, int* count);
 
 /* Autotuner of the fused kernel's launch shape: times every (tile_verts in
- * {32, 64, 128, 256}, block_threads in {256, 512}) candidate on this solver's mesh (num_launches iteration
- * launches each, after one warm-up) and keeps the fastest. Results are
- * cached per process under a key of the vertex, triangle, hinge and cloth
- * counts, the mode and the histogram of vertex valences, so another solver
- * on an equally shaped mesh reuses the choice without timing. */
+ * {32, 64, 128, 256}, block_threads in {256, 512}, and in CLH_MODE_PD with
+ * iters_per_launch = 2, iterations per launch in {1, 2}) candidate on this
+ * solver's mesh (num_launches steady-phase launches each, as timed by
+ * clh_solver_time_iteration_kernel) and keeps the one with the shortest time
+ * per iteration. Results are cached per process under a key of the vertex,
+ * triangle, hinge and cloth counts, the mode, whether two-iteration launches
+ * may be used and the histogram of vertex valences, so another solver on an
+ * equally shaped mesh reuses the choice without timing. */
 typedef struct clh_tune_report {
     int   num_candidates;
-    int   tile_verts[8];
-    int   block_threads[8];
-    float us_per_launch[8];   /* 0 when the candidate did not fit */
+    int   tile_verts[16];
+    int   block_threads[16];
+    int   iters_per_launch[16];
+    float us_per_iteration[16]; /* launch time / iterations; 0 when the candidate did not fit */
     int   best;               /* index of the chosen candidate */
     int   from_cache;         /* 1 when the choice came from the cache */
     unsigned long long key;   /* cache key of the mesh shape */

@@ -138,19 +138,24 @@ def test_autotune_caches_and_keeps_results():
     b.set_state(pos0, vel0)
     sol = ch.Solver(c, b.N, b.T, b.E, cloth_offsets=b.vert_off)
     rep = sol.autotune(b.step_args(), 10)
+    # 4 tile sizes x 2 workgroup sizes
     assert rep.num_candidates == 8 and rep.from_cache == 0
-    times = [rep.us_per_launch[i] for i in range(8)]
-    assert rep.us_per_launch[rep.best] == min(t for t in times if t > 0)
+    cands = rep.candidates()
+    times = [cnd["us_per_iteration"] for cnd in cands]
+    assert rep.us_per_iteration[rep.best] == min(t for t in times if t > 0)
+    assert all(cnd["iters_per_launch"] == 1 for cnd in cands)
     info = sol.plan_info()
-    assert (info.tile_verts, info.block_threads) == (rep.tile_verts[rep.best],
-                                                     rep.block_threads[rep.best])
+    best = cands[rep.best]
+    assert (info.tile_verts, info.block_threads, info.iters_per_launch) == (
+        best["tile_verts"], best["block_threads"], best["iters_per_launch"])
+    # whatever shape wins, the frames are those of the default solver, bit for bit
     got, _, _ = run_batch(b, c, 2, ACCEL, solver=sol)
     np.testing.assert_array_equal(got, ref)
     sol2 = ch.Solver(c, b.N, b.T, b.E, cloth_offsets=b.vert_off)
     rep2 = sol2.autotune(b.step_args(), 10)
     assert rep2.from_cache == 1 and rep2.key == rep.key
-    assert (rep2.tile_verts[0], rep2.block_threads[0]) == (rep.tile_verts[rep.best],
-                                                           rep.block_threads[rep.best])
+    assert (rep2.tile_verts[0], rep2.block_threads[0], rep2.iters_per_launch[0]) == (
+        best["tile_verts"], best["block_threads"], best["iters_per_launch"])
 
 
 def test_batch_rejects_unfused_and_bad_offsets():

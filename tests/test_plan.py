@@ -28,6 +28,18 @@ def test_plan_is_consistent(case, tile):
                                s.get("pos0"), tile)
     assert info.num_tiles == -(-s.N // info.tile_verts)
     assert 1.0 <= info.halo_tri_factor < 3.0
+    # PD plans carry the two-iteration extension (ring 2, ring-1 gather lists),
+    # checked by the same call
+    assert info.pair == 1
+    assert info.pair_halo_max >= info.halo_max and info.pair_tri_max >= info.tri_max
+    assert info.pair_hinge_max >= info.hinge_max
+
+
+def test_reference_mode_plan_has_no_pair_extension():
+    s = ch.Scene(**CFG1)
+    info = ch.fused_plan_check(s.N, s.get("tris"), s.get("bend_quads"), s.get("pos0"), 128,
+                               mode=ch.MODE_REF_CUDA)
+    assert info.pair == 0
 
 
 def test_plan_quality_on_grid():
