@@ -402,7 +402,9 @@ __device__ __forceinline__ bool ctl_begin(FrameCtl* c, int par, float& omega, in
 // After the residual of a check iteration has been summed: history, stop on
 // convergence (`conv`: every cloth meets the tolerance) or a non-finite
 // residual, and a Chebyshev restart when the residual grew since the previous
-// check. Every third restart in a frame halves 1 - rho. The check iteration
+// check. Every third restart in a frame doubles 1 - rho (a less aggressive
+// sequence, against the oscillation the nonlinear local step can feed; a rho
+// below the Jacobi spectral radius is not repaired by it). The check iteration
 // is slot `par`'s iteration plus kofs (1 when a two-iteration launch checked
 // its second). One thread.
 __device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, double b2, double* hist,
