@@ -61,8 +61,10 @@ def algorithmic_bytes(N, T, E, iters_per_launch=1):
 
 
 def tune_line(cands, digits):
-    return ", ".join(f"{c['tile_verts']}/{c['block_threads']}x{c['iters_per_launch']}: "
-                     f"{c['us_per_iteration']:.{digits}f}" for c in cands) + " us/iteration"
+    """tile/threads, x iterations per launch, r element regions: us per iteration"""
+    return ", ".join(f"{c['tile_verts']}/{c['block_threads']}x{c['iters_per_launch']}"
+                     f"r{c['element_regions']}: {c['us_per_iteration']:.{digits}f}"
+                     for c in cands) + " us/iteration"
 
 
 def profiled_traffic(path, plan):
@@ -429,6 +431,7 @@ def main():
                    "frames_per_s_fixed_20_iterations": fps20,
                    "fp64_oracle_error_20_iterations": err64,
                    "kernel_variant": int(plan.variant), "iters_per_launch": int(ipl),
+                   "element_regions": int(plan.element_regions),
                    "tile_verts": int(plan.tile_verts),
                    "block_threads": int(plan.block_threads), "autotune_256": tune256},
         "roofline": {"bound": "hbm", "achieved": bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -43,7 +43,7 @@ class Config(C.Structure):
                 ("cheb_delay", C.c_int), ("ext_accel", C.c_float * 3),
                 ("kernel_variant", C.c_int), ("use_graph", C.c_int), ("tile_verts", C.c_int),
                 ("block_threads", C.c_int), ("persistent", C.c_int),
-                ("iters_per_launch", C.c_int)]
+                ("iters_per_launch", C.c_int), ("element_regions", C.c_int)]
 
 
 class MeshDesc(C.Structure):
@@ -76,19 +76,22 @@ class PlanInfo(C.Structure):
                 ("halo_max", C.c_int), ("pair", C.c_int), ("iters_per_launch", C.c_int),
                 ("pair_halo_max", C.c_int), ("pair_tri_max", C.c_int),
                 ("pair_hinge_max", C.c_int), ("pair_bytes_per_launch", C.c_double),
-                ("pair_compute_max", C.c_int), ("pair_lds_bytes", C.c_double)]
+                ("pair_compute_max", C.c_int), ("pair_lds_bytes", C.c_double),
+                ("element_regions", C.c_int), ("split_lds_bytes", C.c_double)]
 
 
 class TuneReport(C.Structure):
-    _fields_ = [("num_candidates", C.c_int), ("tile_verts", C.c_int * 16),
-                ("block_threads", C.c_int * 16), ("iters_per_launch", C.c_int * 16),
-                ("us_per_iteration", C.c_float * 16),
+    _fields_ = [("num_candidates", C.c_int), ("tile_verts", C.c_int * 32),
+                ("block_threads", C.c_int * 32), ("iters_per_launch", C.c_int * 32),
+                ("element_regions", C.c_int * 32), ("us_per_iteration", C.c_float * 32),
                 ("best", C.c_int), ("from_cache", C.c_int), ("key", C.c_ulonglong)]
 
     def candidates(self):
-        """[{tile_verts, block_threads, iters_per_launch, us_per_iteration}]"""
+        """[{tile_verts, block_threads, iters_per_launch, element_regions,
+        us_per_iteration}]"""
         return [{"tile_verts": self.tile_verts[i], "block_threads": self.block_threads[i],
                  "iters_per_launch": self.iters_per_launch[i],
+                 "element_regions": self.element_regions[i],
                  "us_per_iteration": self.us_per_iteration[i]}
                 for i in range(self.num_candidates)]
 

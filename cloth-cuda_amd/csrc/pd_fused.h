@@ -53,6 +53,10 @@ struct FusedPlan {
     int lmax = 0, tmax = 0, hmax = 0, gmax = 0, vmax = 0;  // per-tile maxima (LDS sizing)
     double tri_factor = 0.0, bend_factor = 0.0, bytes_per_iter = 0.0;
     int xcd_remap = 1;
+    // launch choices (not part of the plan's data): separate triangle and
+    // hinge LDS regions in PD launches whose layout fits lds_limit
+    int split = 0;
+    int lds_limit = 65536;
 
     DevBuf<int>   perm, inv_perm;             // tile order <-> original
     DevBuf<int>   tile_info;                  // [num_tiles * kTileInfo]

@@ -166,21 +166,47 @@ __device__ __forceinline__ void stsX(double* s, int n, int i, d3 v)
     s[2 * n + i] = v.z;
 }
 
+// Per-vertex arithmetic shared by every kernel that updates a vertex, so that
+// a vertex updated by two workgroups (as own vertex and as another tile's
+// ring-1 vertex), or by two kernels, gets the same bits. Multiply-adds are
+// explicit fused operations and the rest does not contract, so the rounding
+// does not depend on what the compiler does with the surrounding code.
+
 // PD update of a free vertex: residual r = f + m (dy - d) of the global
 // system at the displacement d, the Jacobi step d + r / D, then the Chebyshev
-// blend with d_{k-1} when `blend`. Every kernel that updates a vertex calls
-// this one function, so a vertex updated by two workgroups (as own vertex and
-// as another tile's ring-1 vertex) gets the same bits.
+// blend with d_{k-1} when `blend`.
 __device__ __forceinline__ d3 pd_update(f3 ar, float om, float odg, f3 oy, d3 od, d3 oxp,
                                         float omega, int blend, d3& r)
 {
+#pragma clang fp contract(off)
     const double md = om;
-    r = d3{(double)ar.x + md * ((double)oy.x - od.x), (double)ar.y + md * ((double)oy.y - od.y),
-           (double)ar.z + md * ((double)oy.z - od.z)};
+    r = d3{fma(md, (double)oy.x - od.x, (double)ar.x), fma(md, (double)oy.y - od.y, (double)ar.y),
+           fma(md, (double)oy.z - od.z, (double)ar.z)};
     const double inv = (odg > 1e-10f) ? 1.0 / (double)odg : 0.0;
-    d3 xn = od + r * inv;
-    if (blend) xn = (xn - oxp) * (double)omega + oxp;
+    d3 xn = d3{fma(r.x, inv, od.x), fma(r.y, inv, od.y), fma(r.z, inv, od.z)};
+    if (blend) {
+        const double w = omega;
+        xn = d3{fma(xn.x - oxp.x, w, oxp.x), fma(xn.y - oxp.y, w, oxp.y), fma(xn.z - oxp.z, w, oxp.z)};
+    }
     return xn;
+}
+
+// acc + kw * v, component-wise, fused.
+__device__ __forceinline__ f3 fmadd3(f3 acc, float kw, f3 v)
+{
+    return f3{fmaf(kw, v.x, acc.x), fmaf(kw, v.y, acc.y), fmaf(kw, v.z, acc.z)};
+}
+
+// |r|^2 and |b|^2 of one vertex for the FP64 relative residual, with
+// b = ab + m (x_n + dy) the right-hand side of the global system.
+__device__ __forceinline__ void pd_residual_terms(d3 r, f3 ab, float om, f3 ob, f3 oy, double& rr,
+                                                  double& bb)
+{
+#pragma clang fp contract(off)
+    const f3 ya = ob + oy;
+    const f3 b = fmadd3(ab, om, ya);
+    rr = fma(r.z, r.z, fma(r.y, r.y, r.x * r.x));
+    bb = fma((double)b.z, (double)b.z, fma((double)b.y, (double)b.y, (double)b.x * (double)b.x));
 }
 
 template <int kT>
@@ -222,8 +248,11 @@ __device__ void fused_block_sum2(double r, double b, double* out, int slot)
 // LDS: position table (PD: FP64 x_n + d as three component arrays;
 // reference: FP32 triples), the tile's gather list, and one element region
 // that holds the triangle corner forces and, once they are summed, the hinge
-// weights (four component arrays) and vectors.
-template <int kMode, bool kCheck, int kT>
+// weights (four component arrays) and vectors. With kSplit the hinges have a
+// region of their own: their first records then come with trip 2, triangles
+// and hinges are evaluated in one phase, and the own vertices sum both after
+// one barrier (two barriers fewer; same sums in the same order).
+template <int kMode, bool kCheck, int kT, bool kSplit>
 __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
 {
     constexpr bool PD = (kMode == CLH_MODE_PD);
@@ -257,7 +286,8 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
     const unsigned short* sg = reinterpret_cast<const unsigned short*>(sgw);
     float* st = reinterpret_cast<float*>(lds + A.off_r);     // tri corner forces [tmax*9]
     float* st2 = st + A.tmax * 9;                            // tri R part [tmax*9] (PD, check)
-    float* skw = st;                                         // hinge k h^2 w_i [4][hmax]
+    float* skw = kSplit ? st + A.tmax * 9 * ((PD && kCheck) ? 2 : 1)
+                        : st;                                // hinge k h^2 w_i [4][hmax]
     float* sq = skw + A.hmax * 4;                            // hinge p-e (PD) or p [hmax*3]
     float* sp = sq + A.hmax * 3;                             // hinge p [hmax*3] (PD, check)
 
@@ -313,6 +343,14 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
         rG = A.tri_c0[tbeg + tid];
         rwA = A.tri_c1[tbeg + tid];
     }
+    uint2 rh = make_uint2(0, 0);
+    float4 rw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float2 rc = make_float2(0.0f, 0.0f);
+    if (kSplit && a.use_bend && tid < nhin) {
+        rh = A.hin_idx[ebeg + tid];
+        rw = A.hin_w[ebeg + tid];
+        rc = A.hin_c[ebeg + tid];
+    }
     for (int j = tid; j < (glen + 1) >> 1; j += kT) sgw[j] = A.gl_words[gbeg + j];
     if (own) {
         if (PD) stsX(sX, L, tid, to_d(ob) + od);
@@ -355,16 +393,15 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
             sts3(o, 2, c[2]);
         }
     }
-    // first hinge record (its latency overlaps the triangle sums)
-    uint2 rh = make_uint2(0, 0);
-    float4 rw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float2 rc = make_float2(0.0f, 0.0f);
-    if (a.use_bend && tid < nhin) {
-        rh = A.hin_idx[ebeg + tid];
-        rw = A.hin_w[ebeg + tid];
-        rc = A.hin_c[ebeg + tid];
+    if (!kSplit) {
+        // first hinge record (its latency overlaps the triangle sums)
+        if (a.use_bend && tid < nhin) {
+            rh = A.hin_idx[ebeg + tid];
+            rw = A.hin_w[ebeg + tid];
+            rc = A.hin_c[ebeg + tid];
+        }
+        __syncthreads();
     }
-    __syncthreads();
 
     // ---- own vertices: triangle corners (ascending element id) ---------------
     const int opin = vi.z;
@@ -373,15 +410,22 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
               j1 = j0 + (int)(meta >> 24);
     const bool gather = own && opin < 0;
     f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
-    if (gather && a.use_stretch) {
-        for (int j = j0; j < jt; ++j) {
-            const int s = sg[j];
-            ar = ar + lds3(st, s);
-            if (PD && kCheck) ab = ab + lds3(st2, s);
+    auto sum_triangles = [&]() {
+        if (gather && a.use_stretch) {
+            for (int j = j0; j < jt; ++j) {
+                const int s = sg[j];
+                ar = ar + lds3(st, s);
+                if (PD && kCheck) ab = ab + lds3(st2, s);
+            }
         }
+    };
+    if (!kSplit) sum_triangles();
+    if (kSplit && !a.use_bend) {
+        __syncthreads();
+        sum_triangles();
     }
     if (a.use_bend) {
-        __syncthreads();  // the hinge vectors reuse the triangle region
+        if (!kSplit) __syncthreads();  // the hinge vectors reuse the triangle region
         // ---- hinges -> bend vectors (src/pd_solver.cu:304-358) ----------------
         for (int j = tid; j < nhin; j += kT) {
             const uint2 id = rh;
@@ -413,16 +457,17 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
             if (PD && kCheck) sts3(sp, j, p);
         }
         __syncthreads();
+        if (kSplit) sum_triangles();
         if (gather) {
             for (int j = jt; j < j1; ++j) {
                 const int s = sg[j];
                 const float kw = skw[(s & 3) * A.hmax + (s >> 2)];
                 const f3 d = lds3(sq, s >> 2);
-                if (PD) ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+                if (PD) ar = fmadd3(ar, kw, d);
                 else    ar = madd_nc(ar, kw, d);
                 if (PD && kCheck) {
                     const f3 q = lds3(sp, s >> 2);
-                    ab = ab + mk(kw * q.x, kw * q.y, kw * q.z);
+                    ab = fmadd3(ab, kw, q);
                 }
             }
         }
@@ -444,10 +489,7 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
             const d3 xn = pd_update(ar, om, odg, oy, od, oxp, a.omega, a.blend, r);
             st3d(static_cast<double*>(a.out), g, xn);
             if (kCheck) {
-                const f3 ya = ob + oy;
-                const f3 b = mk(ab.x + om * ya.x, ab.y + om * ya.y, ab.z + om * ya.z);
-                rr = r.x * r.x + r.y * r.y + r.z * r.z;
-                bb = (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
+                pd_residual_terms(r, ab, om, ob, oy, rr, bb);
             }
         }
     }
@@ -703,7 +745,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
                 const int s = gl[j];
                 const float kw = skw[(s & 3) * P.phmax + (s >> 2)];
                 const f3 d = lds3(sq, s >> 2);
-                ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+                ar = fmadd3(ar, kw, d);
             }
         }
     }
@@ -795,10 +837,10 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
                 const int s = sg[j];
                 const float kw = skw[(s & 3) * P.phmax + (s >> 2)];
                 const f3 d = lds3(sq, s >> 2);
-                ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+                ar = fmadd3(ar, kw, d);
                 if (kCheck) {
                     const f3 q = lds3(sp, s >> 2);
-                    ab = ab + mk(kw * q.x, kw * q.y, kw * q.z);
+                    ab = fmadd3(ab, kw, q);
                 }
             }
         }
@@ -814,10 +856,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             d3 r;
             x2 = pd_update(ar, om, odg, oy, x1, od, c1.omega, c1.blend, r);
             if (kCheck) {
-                const f3 ya = ob + oy;
-                const f3 b = mk(ab.x + om * ya.x, ab.y + om * ya.y, ab.z + om * ya.z);
-                rr = r.x * r.x + r.y * r.y + r.z * r.z;
-                bb = (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
+                pd_residual_terms(r, ab, om, ob, oy, rr, bb);
             }
         }
         st3d(static_cast<double*>(P.out2), g, x2);
@@ -1052,10 +1091,10 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
                     const int s = sg[j];
                     const float kw = skw[(s & 3) * A.hmax + (s >> 2)];
                     const f3 d = lds3(sq, s >> 2);
-                    ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
+                    ar = fmadd3(ar, kw, d);
                     if (check) {
                         const f3 q = lds3(sp, s >> 2);
-                        ab = ab + mk(kw * q.x, kw * q.y, kw * q.z);
+                        ab = fmadd3(ab, kw, q);
                     }
                 }
             }
@@ -1074,10 +1113,7 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
                 d3 r;
                 xn = pd_update(ar, om, odg, oy, od, oxp, cs.omega, cs.blend, r);
                 if (check) {
-                    const f3 ya = ob + oy;
-                    const f3 b = mk(ab.x + om * ya.x, ab.y + om * ya.y, ab.z + om * ya.z);
-                    rr = r.x * r.x + r.y * r.y + r.z * r.z;
-                    bb = (double)b.x * b.x + (double)b.y * b.y + (double)b.z * b.z;
+                    pd_residual_terms(r, ab, om, ob, oy, rr, bb);
                 }
             }
             st3d(out, g, xn);
@@ -1242,7 +1278,7 @@ struct LdsLayout {
     size_t off_g, off_r, total;
 };
 
-LdsLayout lds_layout(const FusedPlan& p, int mode, bool check)
+LdsLayout lds_layout(const FusedPlan& p, int mode, bool check, bool split = false)
 {
     const bool pd = (mode == CLH_MODE_PD), two = pd && check;
     LdsLayout L;
@@ -1250,11 +1286,21 @@ LdsLayout lds_layout(const FusedPlan& p, int mode, bool check)
     L.off_r = L.off_g + align16((size_t)(p.gmax + 1) / 2 * 4);
     const size_t ra = (size_t)p.tmax * 9 * (two ? 2 : 1) * sizeof(float);
     const size_t rb = (size_t)p.hmax * (4 + 3 + (two ? 3 : 0)) * sizeof(float);
-    L.total = L.off_r + align16(std::max(ra, rb));
+    // split: the hinge region follows the triangle region (ra is a multiple
+    // of 4 bytes, as the hinge arrays need)
+    L.total = L.off_r + (split ? align16(ra + rb) : align16(std::max(ra, rb)));
     return L;
 }
 
 size_t lds_bytes(const FusedPlan& p, int mode, bool check) { return lds_layout(p, mode, check).total; }
+
+// Separate triangle and hinge regions for this launch: PD mode, asked for by
+// the plan, and within the workgroup's LDS.
+bool use_split(const FusedPlan& p, int mode, bool check)
+{
+    return p.split && mode == CLH_MODE_PD &&
+           lds_layout(p, mode, check, true).total <= (size_t)p.lds_limit;
+}
 
 // LDS byte offsets of the two-iteration kernel: position table (own, ring 1,
 // ring 2), own gather list, ring-1 gather list, element region (triangles of
@@ -2075,26 +2121,33 @@ int fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
     FusedKernelArgs A;
     fill_kernel_args(p, a, A);
     const bool check = a.partials != nullptr;
-    const LdsLayout L = lds_layout(p, mode, check);
+    const bool split = use_split(p, mode, check);
+    const LdsLayout L = lds_layout(p, mode, check, split);
     A.off_g = (int)L.off_g;
     A.off_r = (int)L.off_r;
     const size_t lds = L.total;
     const dim3 grid(p.num_tiles);
     if (p.threads == 512) {
         const dim3 block(512);
-        if (mode == CLH_MODE_PD) {
-            if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true, 512>), grid, block, lds, stream, A);
-            else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false, 512>), grid, block, lds, stream, A);
+        if (mode == CLH_MODE_PD && split) {
+            if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true, 512, true>), grid, block, lds, stream, A);
+            else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false, 512, true>), grid, block, lds, stream, A);
+        } else if (mode == CLH_MODE_PD) {
+            if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true, 512, false>), grid, block, lds, stream, A);
+            else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false, 512, false>), grid, block, lds, stream, A);
         } else {
-            hipLaunchKernelGGL((k_fused_iter<CLH_MODE_REF_CUDA, false, 512>), grid, block, lds, stream, A);
+            hipLaunchKernelGGL((k_fused_iter<CLH_MODE_REF_CUDA, false, 512, false>), grid, block, lds, stream, A);
         }
     } else {
         const dim3 block(256);
-        if (mode == CLH_MODE_PD) {
-            if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true, 256>), grid, block, lds, stream, A);
-            else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false, 256>), grid, block, lds, stream, A);
+        if (mode == CLH_MODE_PD && split) {
+            if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true, 256, true>), grid, block, lds, stream, A);
+            else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false, 256, true>), grid, block, lds, stream, A);
+        } else if (mode == CLH_MODE_PD) {
+            if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true, 256, false>), grid, block, lds, stream, A);
+            else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false, 256, false>), grid, block, lds, stream, A);
         } else {
-            hipLaunchKernelGGL((k_fused_iter<CLH_MODE_REF_CUDA, false, 256>), grid, block, lds, stream, A);
+            hipLaunchKernelGGL((k_fused_iter<CLH_MODE_REF_CUDA, false, 256, false>), grid, block, lds, stream, A);
         }
     }
     return hip_status(hipGetLastError(), "fused iteration launch");
@@ -2107,12 +2160,16 @@ int fused_enable_lds(int& bytes)
     // above 64 KB the kernels must opt in; when the runtime refuses, stay at
     // 64 KB
     if (bytes <= 64 * 1024) return CLH_OK;
-    const void* fns[] = {reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 256>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 256>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false, 256>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 512>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 512>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false, 512>),
+    const void* fns[] = {reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 256, false>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 256, false>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false, 256, false>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 512, false>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 512, false>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false, 512, false>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 256, true>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 256, true>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 512, true>),
+                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 512, true>),
                          reinterpret_cast<const void*>(&k_fused_persist<256>),
                          reinterpret_cast<const void*>(&k_fused_pair<true, 256>),
                          reinterpret_cast<const void*>(&k_fused_pair<false, 256>),
@@ -2213,6 +2270,8 @@ void fused_plan_info(const FusedPlan& p, clh_plan_info* out)
         out->pair_compute_max = p.pcmax;
         out->pair_lds_bytes = (double)pair_lds_layout(p, true).total;
     }
+    out->split_lds_bytes = (double)lds_layout(p, CLH_MODE_PD, true, true).total;
+    out->element_regions = use_split(p, CLH_MODE_PD, false) ? 2 : 1;
 }
 
 void fused_plan_release(FusedPlan& p)

@@ -44,6 +44,8 @@ constexpr int kMaxIterations = 1 << 20;
 // 735 against 599 us for 512 batched cloths), because its ring-1 elements
 // and extra element passes cost more than the launch and memory trip saved.
 constexpr int kDefaultItersPerLaunch = 1;
+// LDS regions of the element results when the configuration leaves it at 0
+constexpr int kDefaultElementRegions = 1;
 
 inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
@@ -530,6 +532,7 @@ struct clh_solver {
     unsigned long long shape_key = 0;  // autotuner cache key of the mesh shape
     bool persist = false;        // tolerance frames in persistent cooperative launches
     int ipl = 1;                 // iterations per fused launch (2: two-iteration kernel)
+    int regions = 1;             // element regions in LDS (2: separate triangle, hinge)
     bool pair_fits = false;      // the plan's two-iteration tables fit in LDS
     DevBuf<unsigned> bar;        // grid barrier words of the persistent kernel
     DevBuf<double> cloth_res;    // per-cloth (r^2, b^2) of the last residual evaluation
@@ -662,6 +665,8 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
         // two-iteration launches only where their larger tables fit
         s->pair_fits = st == CLH_OK && s->fused.pair &&
                        fused_pair_lds_bytes(s->fused, true) <= (size_t)lds_limit;
+        s->fused.lds_limit = lds_limit;
+        s->fused.split = s->regions == 2 ? 1 : 0;
         if (st == CLH_OK) s->variant = CLH_KERNEL_FUSED;
         else if (s->cfg.kernel_variant == CLH_KERNEL_FUSED || s->num_cloths > 1) return st;
     }
@@ -1191,6 +1196,11 @@ extern "C" int clh_solver_create(const clh_config* cfg, const clh_mesh_desc* mes
         return fail(CLH_ERR_INVALID, "clh_solver_create: iters_per_launch must be 0, 1 or 2");
     }
     s->ipl = cfg->iters_per_launch == 0 ? kDefaultItersPerLaunch : cfg->iters_per_launch;
+    if (cfg->element_regions < 0 || cfg->element_regions > 2) {
+        clh_solver_destroy(s);
+        return fail(CLH_ERR_INVALID, "clh_solver_create: element_regions must be 0, 1 or 2");
+    }
+    s->regions = cfg->element_regions == 0 ? kDefaultElementRegions : cfg->element_regions;
     int st = hip_status(hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking),
                         "hipStreamCreate");
     if (st == CLH_OK) st = hip_status(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming), "hipEventCreate");
@@ -1368,6 +1378,8 @@ extern "C" int clh_solver_plan_info(clh_solver* s, clh_plan_info* out)
     if (s->plan_ready && s->variant == CLH_KERNEL_FUSED) fused_plan_info(s->fused, out);
     out->persistent = s->plan_ready && s->persist ? 1 : 0;
     if (s->plan_ready && pair_active(s)) out->iters_per_launch = 2;
+    if (!(s->plan_ready && s->variant == CLH_KERNEL_FUSED && s->cfg.mode == CLH_MODE_PD))
+        out->element_regions = 1;
     return CLH_OK;
 }
 
@@ -1420,7 +1432,7 @@ extern "C" int clh_batch_cloth_residuals(clh_solver* s, double* out, int max_cou
 namespace {
 std::mutex g_tune_mu;
 struct TuneChoice {
-    int tile_verts, threads, ipl;
+    int tile_verts, threads, ipl, regions;
 };
 std::map<unsigned long long, TuneChoice> g_tune_cache;
 
@@ -1439,12 +1451,15 @@ int replan(clh_solver* s, const clh_step_args& a, int tile_verts, int threads, i
 }
 
 // Sets the launch shape of an already built plan (workgroup size, iterations
-// per launch); captured frames of the previous shape are dropped.
-void set_shape(clh_solver* s, int threads, int ipl)
+// per launch, element regions); captured frames of the previous shape are
+// dropped.
+void set_shape(clh_solver* s, int threads, int ipl, int regions)
 {
     s->block_threads = threads;
     s->fused.threads = threads;
     s->ipl = ipl;
+    s->regions = regions;
+    s->fused.split = regions == 2 ? 1 : 0;
     s->graph_valid = false;
     drop_chunk_graphs(s);
 }
@@ -1470,39 +1485,47 @@ extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int nu
             const TuneChoice c = it->second;
             int got = 0;
             if (s->fused.tile_verts != c.tile_verts) CLH_TRY(replan(s, *a, c.tile_verts, c.threads, got));
-            set_shape(s, c.threads, c.ipl);
+            set_shape(s, c.threads, c.ipl, c.regions);
             out->num_candidates = 1;
             out->tile_verts[0] = c.tile_verts;
             out->block_threads[0] = c.threads;
             out->iters_per_launch[0] = c.ipl;
+            out->element_regions[0] = c.regions;
             out->best = 0;
             out->from_cache = 1;
             return CLH_OK;
         }
     }
     const int tiles[4] = {32, 64, 128, 256}, threads[2] = {256, 512};
-    const bool try_pairs = s->cfg.mode == CLH_MODE_PD && s->cfg.iters_per_launch == 2;
+    const bool pd = s->cfg.mode == CLH_MODE_PD;
+    const bool try_pairs = pd && s->cfg.iters_per_launch == 2;
     int n = 0, best = -1;
     for (int tv : tiles) {
         int got = 0;
         CLH_TRY(replan(s, *a, tv, 256, got));
         for (int th : threads) {
             for (int ipl = 1; ipl <= (try_pairs ? 2 : 1); ++ipl) {
-                out->tile_verts[n] = tv;
-                out->block_threads[n] = th;
-                out->iters_per_launch[n] = ipl;
-                out->us_per_iteration[n] = 0.0f;
-                if (got == tv) {
-                    set_shape(s, th, ipl);
-                    if (ipl == 1 || pair_active(s)) {
-                        float ms = 0.0f;
-                        CLH_TRY(time_iterations(s, *a, num_launches, &ms));
-                        out->us_per_iteration[n] = 1e3f * ms / (float)ipl;
-                        if (best < 0 || out->us_per_iteration[n] < out->us_per_iteration[best])
-                            best = n;
+                for (int reg = 1; reg <= (pd && ipl == 1 ? 2 : 1); ++reg) {
+                    out->tile_verts[n] = tv;
+                    out->block_threads[n] = th;
+                    out->iters_per_launch[n] = ipl;
+                    out->element_regions[n] = reg;
+                    out->us_per_iteration[n] = 0.0f;
+                    if (got == tv) {
+                        set_shape(s, th, ipl, reg);
+                        clh_plan_info pi;
+                        fused_plan_info(s->fused, &pi);
+                        const bool runs = ipl == 2 ? pair_active(s) : (reg == 1 || pi.element_regions == 2);
+                        if (runs) {
+                            float ms = 0.0f;
+                            CLH_TRY(time_iterations(s, *a, num_launches, &ms));
+                            out->us_per_iteration[n] = 1e3f * ms / (float)ipl;
+                            if (best < 0 || out->us_per_iteration[n] < out->us_per_iteration[best])
+                                best = n;
+                        }
                     }
+                    ++n;
                 }
-                ++n;
             }
         }
     }
@@ -1511,10 +1534,10 @@ extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int nu
     out->best = best;
     int got = 0;
     CLH_TRY(replan(s, *a, out->tile_verts[best], out->block_threads[best], got));
-    set_shape(s, out->block_threads[best], out->iters_per_launch[best]);
+    set_shape(s, out->block_threads[best], out->iters_per_launch[best], out->element_regions[best]);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[s->shape_key] = {out->tile_verts[best], out->block_threads[best],
-                                  out->iters_per_launch[best]};
+                                  out->iters_per_launch[best], out->element_regions[best]};
     return CLH_OK;
 }
 

@@ -121,6 +121,11 @@ typedef struct clh_config {
                                first; same bits as one per launch) and lets the
                                autotuner choose between 1 and 2; 1 or 0 (default) one
                                per launch: on MI355X two are slower per iteration */
+    int   element_regions;  /* fused variant, CLH_MODE_PD: LDS regions of a tile's element
+                               results: 2 = separate triangle and hinge regions
+                               (evaluated in one phase, two barriers fewer) where they
+                               fit, 1 = one shared region, 0 = the default; the
+                               autotuner times both. Same bits either way. */
 } clh_config;
 
 enum { CLH_PERSIST_OFF = 0, CLH_PERSIST_ON = 1 };
@@ -248,6 +253,9 @@ typedef struct clh_plan_info {
     double pair_bytes_per_launch; /* bytes a two-iteration launch streams (model) */
     int    pair_compute_max; /* largest own + ring-1 count: the workgroup size it needs */
     double pair_lds_bytes;   /* LDS per workgroup of the two-iteration kernel (check variant) */
+    int    element_regions;  /* 2 when iteration launches use separate triangle and hinge
+                                regions, else 1 */
+    double split_lds_bytes;  /* LDS per workgroup with separate regions (check variant) */
 } clh_plan_info;
 int clh_solver_plan_info(clh_solver* s, clh_plan_info* out);
 
@@ -284,8 +292,9 @@ int clh_batch_create(const clh_config* cfg, const clh_mesh_desc* mesh, int num_c
 int clh_batch_cloth_residuals(clh_solver* s, double* out, int max_count, int* count);
 
 /* Autotuner of the fused kernel's launch shape: times every (tile_verts in
- * {32, 64, 128, 256}, block_threads in {256, 512}, and in CLH_MODE_PD with
- * iters_per_launch = 2, iterations per launch in {1, 2}) candidate on this
+ * {32, 64, 128, 256}, block_threads in {256, 512}; in CLH_MODE_PD element
+ * regions in {1, 2} where separate regions fit, and with iters_per_launch = 2
+ * also two iterations per launch) candidate on this
  * solver's mesh (num_launches steady-phase launches each, as timed by
  * clh_solver_time_iteration_kernel) and keeps the one with the shortest time
  * per iteration. Results are cached per process under a key of the vertex,
@@ -294,10 +303,11 @@ int clh_batch_cloth_residuals(clh_solver* s, double* out, int max_count, int* co
  * equally shaped mesh reuses the choice without timing. */
 typedef struct clh_tune_report {
     int   num_candidates;
-    int   tile_verts[16];
-    int   block_threads[16];
-    int   iters_per_launch[16];
-    float us_per_iteration[16]; /* launch time / iterations; 0 when the candidate did not fit */
+    int   tile_verts[32];
+    int   block_threads[32];
+    int   iters_per_launch[32];
+    int   element_regions[32];
+    float us_per_iteration[32]; /* launch time / iterations; 0 when the candidate did not fit */
     int   best;               /* index of the chosen candidate */
     int   from_cache;         /* 1 when the choice came from the cache */
     unsigned long long key;   /* cache key of the mesh shape */

@@ -138,24 +138,27 @@ def test_autotune_caches_and_keeps_results():
     b.set_state(pos0, vel0)
     sol = ch.Solver(c, b.N, b.T, b.E, cloth_offsets=b.vert_off)
     rep = sol.autotune(b.step_args(), 10)
-    # 4 tile sizes x 2 workgroup sizes
-    assert rep.num_candidates == 8 and rep.from_cache == 0
+    # 4 tile sizes x 2 workgroup sizes x 1 or 2 element regions (PD)
+    assert rep.num_candidates == 16 and rep.from_cache == 0
     cands = rep.candidates()
     times = [cnd["us_per_iteration"] for cnd in cands]
     assert rep.us_per_iteration[rep.best] == min(t for t in times if t > 0)
     assert all(cnd["iters_per_launch"] == 1 for cnd in cands)
+    assert sorted({cnd["element_regions"] for cnd in cands}) == [1, 2]
     info = sol.plan_info()
     best = cands[rep.best]
-    assert (info.tile_verts, info.block_threads, info.iters_per_launch) == (
-        best["tile_verts"], best["block_threads"], best["iters_per_launch"])
+    assert (info.tile_verts, info.block_threads, info.iters_per_launch, info.element_regions) == (
+        best["tile_verts"], best["block_threads"], best["iters_per_launch"],
+        best["element_regions"])
     # whatever shape wins, the frames are those of the default solver, bit for bit
     got, _, _ = run_batch(b, c, 2, ACCEL, solver=sol)
     np.testing.assert_array_equal(got, ref)
     sol2 = ch.Solver(c, b.N, b.T, b.E, cloth_offsets=b.vert_off)
     rep2 = sol2.autotune(b.step_args(), 10)
     assert rep2.from_cache == 1 and rep2.key == rep.key
-    assert (rep2.tile_verts[0], rep2.block_threads[0], rep2.iters_per_launch[0]) == (
-        best["tile_verts"], best["block_threads"], best["iters_per_launch"])
+    assert (rep2.tile_verts[0], rep2.block_threads[0], rep2.iters_per_launch[0],
+            rep2.element_regions[0]) == (best["tile_verts"], best["block_threads"],
+                                         best["iters_per_launch"], best["element_regions"])
 
 
 def test_batch_rejects_unfused_and_bad_offsets():
