@@ -292,6 +292,17 @@ def main():
     copy_ms = ch.measure_copy(256 << 20, 20)
     copy_gbs = 2 * (256 << 20) / (copy_ms * 1e-3) / 1e9
     log(f"copy probe: {copy_gbs:.0f} GB/s")
+    # the same copy kernel at small sizes: the cost of any launch (16 KiB, 4
+    # workgroups), of one with the iteration kernel's 512 workgroups (2 MiB),
+    # and of moving the iteration kernel's profiled traffic (read + write)
+    tr256 = profiled_traffic(TRAFFIC_FILE, plan)
+    small = [16 << 10, 2 << 20] + ([int(tr256 / 2) & ~15] if tr256 else [])
+    copy_small = []
+    for nb in small:
+        us = 1e3 * min(ch.measure_copy(nb, 500) for _ in range(3))
+        copy_small.append({"bytes_read": nb, "bytes_moved": 2 * nb, "us_per_launch": us})
+    log("copy probe, small: " + ", ".join(f"{c['bytes_moved'] / 1e6:.2f} MB moved in "
+                                           f"{c['us_per_launch']:.2f} us" for c in copy_small))
     err64 = fp64_error(ch, s, pos0, vel0, common, 20)
     log(f"FP64 oracle error after one 20-iteration frame: {err64:.2e} of extent")
 
@@ -439,7 +450,8 @@ def main():
                      "kernel": kernel_name(plan, "256x256"),
                      "us_per_launch": t_iter * 1e3, "iters_per_launch": ipl,
                      "us_per_iteration": t_iter * 1e3 / ipl, "bytes_per_launch": b_iter,
-                     "copy_probe_GBs": copy_gbs, "mesh_1024": roof_1024,
+                     "copy_probe_GBs": copy_gbs, "copy_probe_small": copy_small,
+                     "mesh_1024": roof_1024,
                      "batched_512": batch},
         "cpu_baseline": cpu,
     }
