@@ -649,9 +649,14 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
     if (s->cfg.kernel_variant != CLH_KERNEL_UNFUSED && N > 0 && a.d_pos) {
         std::vector<float> pos((size_t)N * 3);
         CLH_HIP(hipMemcpy(pos.data(), a.d_pos, pos.size() * sizeof(float), hipMemcpyDeviceToHost));
-        int dev = 0, lds_limit = 0;
+        int dev = 0, lds_limit = 0, optin = 0;
         CLH_HIP(hipGetDevice(&dev));
         CLH_HIP(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        // larger workgroup allocations by opt-in (gfx950: 160 KB of LDS per CU)
+        if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess)
+            lds_limit = std::max(lds_limit, optin);
+        else
+            (void)hipGetLastError();
         CLH_TRY(fused_enable_lds(lds_limit));
         int st = CLH_ERR_UNSUPPORTED;
         const bool want_pair = s->cfg.mode == CLH_MODE_PD && s->cfg.iters_per_launch == 2;
@@ -1375,7 +1380,10 @@ extern "C" int clh_solver_plan_info(clh_solver* s, clh_plan_info* out)
     std::memset(out, 0, sizeof(*out));
     out->variant = s->plan_ready ? s->variant : CLH_KERNEL_AUTO;
     out->iters_per_launch = 1;
-    if (s->plan_ready && s->variant == CLH_KERNEL_FUSED) fused_plan_info(s->fused, out);
+    if (s->plan_ready && s->variant == CLH_KERNEL_FUSED) {
+        fused_plan_info(s->fused, out);
+        out->lds_limit = s->fused.lds_limit;
+    }
     out->persistent = s->plan_ready && s->persist ? 1 : 0;
     if (s->plan_ready && pair_active(s)) out->iters_per_launch = 2;
     if (!(s->plan_ready && s->variant == CLH_KERNEL_FUSED && s->cfg.mode == CLH_MODE_PD))

@@ -256,6 +256,7 @@ typedef struct clh_plan_info {
     int    element_regions;  /* 2 when iteration launches use separate triangle and hinge
                                 regions, else 1 */
     double split_lds_bytes;  /* LDS per workgroup with separate regions (check variant) */
+    int    lds_limit;        /* LDS a workgroup of the fused kernels may use on this device */
 } clh_plan_info;
 int clh_solver_plan_info(clh_solver* s, clh_plan_info* out);
 
