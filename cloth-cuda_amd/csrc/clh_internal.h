@@ -53,6 +53,10 @@ struct FrameCtl {
     int    barrier_error;   // a grid barrier of the persistent kernel timed out
     double last_r2;         // residual^2 at the previous check (-1: none)
     double tol2;            // tolerance^2
+    // host-visible copies of this block (pinned, coherent): after check c
+    // the reduction writes the block to mirror[c & 1], so the host reads the
+    // control state of a chunk without a copy on the stream (null: no copies)
+    FrameCtl* mirror;
 };
 
 // Device buffer owned by the solver.

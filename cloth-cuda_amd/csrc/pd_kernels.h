@@ -436,6 +436,7 @@ __device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, doubl
         }
         c->last_r2 = r2;
     }
+    if (c->mirror) c->mirror[slot & 1] = *c;
 }
 
 }  // namespace clh

@@ -420,7 +420,7 @@ k_reduce_cloths(const double* __restrict__ partials, const int* __restrict__ clo
 
 // Starts the control state of a frame run to a tolerance.
 __global__ void k_ctl_frame_init(FrameCtl* __restrict__ c, float rho, int use_cheb, int delay,
-                                 double tol2)
+                                 double tol2, FrameCtl* mirror)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     FrameCtl f;
@@ -441,6 +441,7 @@ __global__ void k_ctl_frame_init(FrameCtl* __restrict__ c, float rho, int use_ch
     f.barrier_error = 0;
     f.last_r2 = -1.0;
     f.tol2 = tol2;
+    f.mirror = mirror;
     *c = f;
 }
 
@@ -564,7 +565,8 @@ struct clh_solver {
     // iterate buffers rotate by 4; the control slots alternate per launch,
     // every iteration or, with two-iteration launches, every second one)
     DevBuf<FrameCtl> ctl;
-    FrameCtl* h_ctl = nullptr;
+    FrameCtl* h_ctl = nullptr;       // [2], pinned and coherent
+    FrameCtl* d_ctl_mirror = nullptr;  // device address of h_ctl
     hipEvent_t ev_chunk[2] = {nullptr, nullptr};
     hipGraphExec_t chunk_exec[kIterBufs] = {};
     clh_step_args chunk_args = {};
@@ -715,7 +717,11 @@ int ensure_scratch(clh_solver* s)
     s->nblocks_v = (int)grid_for(N);
     CLH_TRY(s->partials.resize((size_t)std::max(s->nblocks_v, fused_max_blocks(s->fused)) * 2));
     CLH_TRY(s->ctl.resize(1));
-    if (!s->h_ctl) CLH_HIP(hipHostMalloc((void**)&s->h_ctl, 2 * sizeof(FrameCtl)));
+    if (!s->h_ctl) {
+        CLH_HIP(hipHostMalloc((void**)&s->h_ctl, 2 * sizeof(FrameCtl),
+                              hipHostMallocMapped | hipHostMallocCoherent));
+        CLH_HIP(hipHostGetDevicePointer((void**)&s->d_ctl_mirror, s->h_ctl, 0));
+    }
     for (auto& e : s->ev_chunk)
         if (!e) CLH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const int max_checks = s->cfg.max_iterations + 1;
@@ -1058,9 +1064,14 @@ int step_impl(clh_solver* s, const clh_step_args& a)
         s->chunk_len = P;
     }
     CLH_TRY(enqueue_prologue(s, a));
+    // the persistent kernel's grid barrier reports its time-outs in the block
+    // itself, so that path copies the block back; the others read the copies
+    // the reductions write
+    const bool mirrored = !s->persist || a.d_trace;
     hipLaunchKernelGGL(k_ctl_frame_init, dim3(1), dim3(64), 0, s->stream, s->ctl.ptr, c.rho,
                        c.use_chebyshev ? 1 : 0, c.cheb_delay,
-                       (double)c.tolerance * (double)c.tolerance);
+                       (double)c.tolerance * (double)c.tolerance,
+                       mirrored ? s->d_ctl_mirror : nullptr);
     CLH_HIP(hipGetLastError());
     // two-iteration launches when every check falls on a pair's second
     // iteration: chunks of even length start at even k, so pairs run at even
@@ -1115,8 +1126,9 @@ int step_impl(clh_solver* s, const clh_step_args& a)
         } else {
             CLH_TRY(enqueue_chunk(k0, k1));
         }
-        CLH_HIP(hipMemcpyAsync(&s->h_ctl[ci & 1], s->ctl.ptr, sizeof(FrameCtl),
-                               hipMemcpyDeviceToHost, s->stream));
+        if (!mirrored)
+            CLH_HIP(hipMemcpyAsync(&s->h_ctl[ci & 1], s->ctl.ptr, sizeof(FrameCtl),
+                                   hipMemcpyDeviceToHost, s->stream));
         CLH_HIP(hipEventRecord(s->ev_chunk[ci & 1], s->stream));
         return CLH_OK;
     };
