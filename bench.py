@@ -13,8 +13,9 @@ with device events).
 config.warm_start compares the plain start with the second-order
 extrapolation and the learned warm start (trained here, on the GPU, by the
 library's own kernels, on frames the batched solver generates for 64 cloths
-with random wind) on the same workload, at 256x256 and at 64x64, each frame
-to the same tolerance with the plain start as arbiter and fallback.
+with random wind; the gain on its correction chosen on 8 held-out cloths) on
+the same workload, at 256x256, 64x64 and 32x32, each frame to the same
+tolerance with the plain start as arbiter and fallback.
 
 The roofline object describes the kernel that dominates the run, the fused
 iteration kernel at 256x256 in the launch shape the autotuner chose (one or
@@ -120,7 +121,7 @@ def warm_start_study(ch, n, rho, args, pos0=None, vel0=None, scene=None, gen_tol
     and wind), each from the same state with the same warm-up."""
     import numpy as np
     from helpers import CFG2, CFG2_WIND, cfg5_specs
-    from warmstart import FrameLoop
+    from warmstart import FrameLoop, select_gain
     dt = CFG2["dt"]
     t0 = time.time()
     specs, acc = cfg5_specs(args.ws_cloths, n=n)
@@ -151,6 +152,24 @@ def warm_start_study(ch, n, rho, args, pos0=None, vel0=None, scene=None, gen_tol
     log(f"warm start {n}x{n}: {args.ws_steps} Adam steps in {train_s:.1f} s, loss "
         f"{float(np.mean(losses[:20])):.4f} -> {float(np.mean(losses[-100:])):.4f}")
     del wsm, b
+    # the gain on the network's correction, chosen on 8 held-out cloths (other
+    # jitter seeds, wind directions and strengths, up to the workload's 0.5
+    # N/m^2) by the iterations per frame to the tolerance
+    t2 = time.time()
+    vspecs, vacc = cfg5_specs(8, seed=9, n=n, cloth_seed=9000, wind_max=0.5)
+    vb = build_scene(dict(specs=vspecs))
+    vsol = ch.Solver(ch.config(max_iterations=3 * args.ws_gen_cap, tolerance=args.tol,
+                               check_every=args.check_every, mode=ch.MODE_PD, gravity=-9.81,
+                               dt=dt, rho=rho, cheb_delay=1), vb.N, vb.T, vb.E,
+                     cloth_offsets=vb.vert_off)
+    vws = ch.WarmStart.for_scene(vb, dt, -9.81)
+    vws.set_params(params, scale)
+    gain, gain_table = select_gain(vb, vsol, vws, cloth_accel=vacc)
+    scale *= gain
+    log(f"warm start {n}x{n}: gain {gain} chosen on held-out cloths in {time.time() - t2:.1f} s "
+        f"(iterations per frame: " + ", ".join(f"{g}: {it:.0f}" for g, it in gain_table.items())
+        + ")")
+    del vb, vsol, vws
     if scene is None:
         scene = build_scene(dict(CFG2, nrows=n, ncols=n, size=1 / (n - 1)))
         pos0, vel0 = scene.get_state()
@@ -189,6 +208,7 @@ def warm_start_study(ch, n, rho, args, pos0=None, vel0=None, scene=None, gen_tol
            "loss_last": float(np.mean(losses[-100:])), "generate_seconds": gen_s,
            "train_seconds": train_s, "total_seconds": time.time() - t0,
            "warm_cap_iterations": cap, "modes": res,
+           "gain": gain, "gain_held_out_iterations": {str(g): it for g, it in gain_table.items()},
            "speedup_learned_vs_plain": res["learned"]["frames_per_s"] / res["plain"]["frames_per_s"],
            "speedup_learned_vs_extrapolation":
                res["learned"]["frames_per_s"] / res["extrap"]["frames_per_s"]}

@@ -587,6 +587,9 @@ int launch_features(clh_ws* w, const float* xm2, const float* xm1, const float* 
                     const float* xnext, const float* cloth_accel, unsigned short* F, float4* T,
                     float* extrap, const float* cloth_weight = nullptr)
 {
+    // training targets are stored divided by the output scale
+    if (T && !(w->so > 0.0f))
+        return fail(CLH_ERR_INVALID, "warm start: training frames need an output scale > 0");
     FeatArgs a;
     a.cloth_weight = cloth_weight;
     a.xm2 = xm2;
@@ -781,7 +784,9 @@ extern "C" int clh_ws_set_params(clh_ws* w, const float* in, int count, float ou
 {
     if (!w || (count > 0 && !in)) return fail(CLH_ERR_INVALID, "clh_ws_set_params: null pointer");
     if (count != kParams && count != 0) return fail(CLH_ERR_INVALID, "clh_ws_set_params: count must equal clh_ws_param_count()");
-    if (!(out_scale > 0.0f)) return fail(CLH_ERR_INVALID, "clh_ws_set_params: output scale must be > 0");
+    // 0 turns the correction off (the prediction is then the extrapolation)
+    if (!(out_scale >= 0.0f) || !std::isfinite(out_scale))
+        return fail(CLH_ERR_INVALID, "clh_ws_set_params: output scale must be finite and >= 0");
     CLH_HIP(hipStreamSynchronize(w->stream));
     if (count) {
         std::vector<float> p(in, in + kParams);
@@ -928,6 +933,7 @@ extern "C" int clh_ws_fit_output_scale(clh_ws* w, float* out_scale)
 {
     // root mean square of the weighted targets, measured at scale 1
     if (!w || w->size == 0) return fail(CLH_ERR_INVALID, "clh_ws_fit_output_scale: empty dataset");
+    if (!(w->so > 0.0f)) return fail(CLH_ERR_INVALID, "clh_ws_fit_output_scale: output scale is 0");
     CLH_HIP(hipStreamSynchronize(w->stream));
     std::vector<float4> t((size_t)w->size);
     CLH_HIP(hipMemcpy(t.data(), w->T.ptr, t.size() * sizeof(float4), hipMemcpyDeviceToHost));

@@ -123,3 +123,29 @@ class FrameLoop:
             its.append(self.step(mode, dataset))
         ch.check(ch.hip().clh_synchronize(), "sync")
         return its, time.perf_counter() - t0
+
+
+def select_gain(scene, solver, ws, gains=(0.0, 0.25, 0.5, 1.0), warmup=3, frames=6,
+                cloth_accel=None):
+    """Chooses the factor on the network's correction by the iterations it
+    saves on held-out frames: from the same state, each gain runs `warmup`
+    plain frames (to fill the position history) and then `frames` learned
+    frames on `scene` with `solver`, which should iterate to the tolerance
+    the warm start is used with. Gain 0 is the extrapolation alone, so on
+    these frames the chosen start is never slower than it. Sets the chosen
+    gain on `ws` (its output scale times the gain), restores the scene's
+    state and returns (gain, {gain: mean iterations per learned frame})."""
+    params, scale = ws.params()
+    pos0, vel0 = scene.get_state()
+    res = {}
+    for g in gains:
+        scene.set_state(pos0, vel0)
+        ws.set_params(params, scale * g)
+        loop = FrameLoop(scene, solver, ws, cloth_accel)
+        loop.run(warmup, mode="plain")
+        its, _ = loop.run(frames, mode="learned")
+        res[g] = float(np.mean(its))
+    best = min(gains, key=lambda g: res[g])
+    ws.set_params(params, scale * best)
+    scene.set_state(pos0, vel0)
+    return best, res

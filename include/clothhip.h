@@ -364,6 +364,9 @@ int clh_ws_set_stream(clh_ws* w, void* stream);
  * W3 [16x64] (rows 3..15 zero), b3 [16] (entries 3..15 zero), row-major. */
 int clh_ws_param_count(void);
 int clh_ws_get_params(clh_ws* w, float* out, int count, float* out_scale);
+/* out_scale >= 0 multiplies the network's correction; 0 turns it off (the
+ * prediction is the extrapolation), and training frames then need a positive
+ * scale set first. */
 int clh_ws_set_params(clh_ws* w, const float* in, int count, float out_scale);
 /* d_guess [N*3] from the positions of the three previous frames (device
  * [N*3] each) and the optional per-cloth acceleration [C*3];

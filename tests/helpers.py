@@ -62,16 +62,17 @@ def extent(pos):
     return float(np.max(pos.max(0) - pos.min(0)))
 
 
-def cfg5_specs(count=64, seed=5, n=256):
+def cfg5_specs(count=64, seed=5, n=256, cloth_seed=5000, wind_max=2.0):
     """The learned-warm-start workload: count n x n cloths with the 256x256
     cloth's material and pins (stretch 1e3, bend 1e-2, density 0.1, top edge
-    pinned, +-1e-3 m jitter), each with a wind of U(0, 2) N/m^2 in a uniformly
-    random direction, i.e. an acceleration of up to 20 m/s^2. Returns the
-    specs and the per-cloth accelerations [count x 3]."""
+    pinned, +-1e-3 m jitter, cloth i jittered with seed cloth_seed + i), each
+    with a wind of U(0, wind_max) N/m^2 (default 2, an acceleration of up to
+    20 m/s^2) in a uniformly random direction. Returns the specs and the
+    per-cloth accelerations [count x 3]."""
     rng = np.random.default_rng(seed)
     specs, accel = [], []
     for i in range(count):
-        specs.append(dict(CFG2, nrows=n, ncols=n, size=1 / (n - 1), seed=5000 + i))
+        specs.append(dict(CFG2, nrows=n, ncols=n, size=1 / (n - 1), seed=cloth_seed + i))
         d = rng.normal(size=3)
-        accel.append(d / np.linalg.norm(d) * rng.uniform(0.0, 2.0) / CFG2["density"])
+        accel.append(d / np.linalg.norm(d) * rng.uniform(0.0, wind_max) / CFG2["density"])
     return specs, np.asarray(accel, np.float32)

@@ -9,7 +9,7 @@ import pytest
 import clothhip as ch
 import ws_oracle as wo
 from helpers import CFG1, cfg5_specs
-from warmstart import FrameLoop
+from warmstart import FrameLoop, select_gain
 
 pytestmark = pytest.mark.gpu
 
@@ -205,6 +205,48 @@ def test_training_saves_iterations():
             res[mode].append(float(np.mean(its[2:])))
     print("mean iterations per frame:", res)
     assert np.sum(res["plain"]) / np.sum(res["learned"]) >= 1.5, res
+
+
+def test_select_gain_on_held_out_frames():
+    """The gain on the network's correction is chosen by the iterations on
+    held-out frames: the choice is the argmin of the measured table, the warm
+    start keeps its output scale times the gain, the scene's state is
+    restored, and gain 0 runs the extrapolation's frames exactly."""
+    specs, accel = cfg5_specs(8, seed=101, n=16)
+    b = ch.Scene(specs=specs)
+    b.upload()
+    sol = ch.Solver(solver_cfg(cap=4000), b.N, b.T, b.E, cloth_offsets=b.vert_off)
+    ws = ch.WarmStart.for_scene(b, DT, G, seed=3)
+    FrameLoop(b, sol, ws, accel).run(12, mode="extrap", dataset=True)
+    ws.fit_output_scale()
+    ws.train(300, 4096, 2e-3)
+    params, scale = ws.params()
+
+    vspecs, vaccel = cfg5_specs(4, seed=901, n=16, cloth_seed=7000)
+    v = ch.Scene(specs=vspecs)
+    v.upload()
+    vsol = ch.Solver(solver_cfg(cap=4000), v.N, v.T, v.E, cloth_offsets=v.vert_off)
+    vws = ch.WarmStart.for_scene(v, DT, G)
+    vws.set_params(params, scale)
+    pos0, vel0 = v.get_state()
+    gain, table = select_gain(v, vsol, vws, gains=(0.0, 1.0), warmup=2, frames=4,
+                              cloth_accel=vaccel)
+    assert set(table) == {0.0, 1.0}
+    assert table[gain] == min(table.values())
+    np.testing.assert_array_equal(vws.params()[0], params)
+    assert vws.params()[1] == np.float32(scale * gain)
+    np.testing.assert_array_equal(v.get_state()[0], pos0)
+    # gain 0: the learned frames are the extrapolation's
+    vws.set_params(params, 0.0)
+    its_g0 = FrameLoop(v, vsol, vws, vaccel)
+    its_g0.run(2, mode="plain")
+    a, _ = its_g0.run(4, mode="learned")
+    v.set_state(pos0, vel0)
+    ex = FrameLoop(v, vsol, vws, vaccel)
+    ex.run(2, mode="plain")
+    e, _ = ex.run(4, mode="extrap")
+    assert a == e
+    assert table[0.0] == float(np.mean(e))
 
 
 def test_fallback_frames_equal_plain_frames():
