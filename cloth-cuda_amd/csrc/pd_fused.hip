@@ -209,6 +209,79 @@ __device__ __forceinline__ void pd_residual_terms(d3 r, f3 ab, float om, f3 ob, 
     bb = fma((double)b.z, (double)b.z, fma((double)b.y, (double)b.y, (double)b.x * (double)b.x));
 }
 
+// Gather sums of one vertex over its list gl, in list order (ascending
+// element id). Every entry is two dependent LDS reads (the slot, then its
+// values); four entries' reads are issued before their sums, which keep the
+// list order, so the result is that of the entry-by-entry loop.
+//
+// Triangle corners j0 .. jt: corner forces st into ar and, with kCheck, the
+// R parts st2 into ab.
+template <bool kCheck>
+__device__ __forceinline__ void gather_corners(const unsigned short* gl, int j0, int jt,
+                                               const float* st, const float* st2, f3& ar, f3& ab)
+{
+    int j = j0;
+    for (; j + 4 <= jt; j += 4) {
+        int s[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s[u] = gl[j + u];
+        f3 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = lds3(st, s[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ar = ar + v[u];
+        if (kCheck) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = lds3(st2, s[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) ab = ab + v[u];
+        }
+    }
+    for (; j < jt; ++j) {
+        const int s = gl[j];
+        ar = ar + lds3(st, s);
+        if (kCheck) ab = ab + lds3(st2, s);
+    }
+}
+
+// Hinge slots jt .. j1: k h^2 w_i (arrays of stride hmax in skw) times the
+// hinge vectors sq into ar (fused in PD mode, unfused in reference mode) and,
+// with kCheck, times the projections sp into ab.
+template <bool kPD, bool kCheck>
+__device__ __forceinline__ void gather_hinges(const unsigned short* gl, int jt, int j1,
+                                              const float* skw, int hmax, const float* sq,
+                                              const float* sp, f3& ar, f3& ab)
+{
+    int j = jt;
+    for (; j + 4 <= j1; j += 4) {
+        int s[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s[u] = gl[j + u];
+        float kw[4];
+        f3 d[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            kw[u] = skw[(s[u] & 3) * hmax + (s[u] >> 2)];
+            d[u] = lds3(sq, s[u] >> 2);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ar = kPD ? fmadd3(ar, kw[u], d[u]) : madd_nc(ar, kw[u], d[u]);
+        if (kPD && kCheck) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) d[u] = lds3(sp, s[u] >> 2);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) ab = fmadd3(ab, kw[u], d[u]);
+        }
+    }
+    for (; j < j1; ++j) {
+        const int s = gl[j];
+        const float kw = skw[(s & 3) * hmax + (s >> 2)];
+        const f3 d = lds3(sq, s >> 2);
+        ar = kPD ? fmadd3(ar, kw, d) : madd_nc(ar, kw, d);
+        if (kPD && kCheck) ab = fmadd3(ab, kw, lds3(sp, s >> 2));
+    }
+}
+
 template <int kT>
 __device__ void fused_block_sum2(double r, double b, double* out, int slot)
 {
@@ -411,13 +484,7 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
     const bool gather = own && opin < 0;
     f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
     auto sum_triangles = [&]() {
-        if (gather && a.use_stretch) {
-            for (int j = j0; j < jt; ++j) {
-                const int s = sg[j];
-                ar = ar + lds3(st, s);
-                if (PD && kCheck) ab = ab + lds3(st2, s);
-            }
-        }
+        if (gather && a.use_stretch) gather_corners<PD && kCheck>(sg, j0, jt, st, st2, ar, ab);
     };
     if (!kSplit) sum_triangles();
     if (kSplit && !a.use_bend) {
@@ -458,19 +525,7 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
         }
         __syncthreads();
         if (kSplit) sum_triangles();
-        if (gather) {
-            for (int j = jt; j < j1; ++j) {
-                const int s = sg[j];
-                const float kw = skw[(s & 3) * A.hmax + (s >> 2)];
-                const f3 d = lds3(sq, s >> 2);
-                if (PD) ar = fmadd3(ar, kw, d);
-                else    ar = madd_nc(ar, kw, d);
-                if (PD && kCheck) {
-                    const f3 q = lds3(sp, s >> 2);
-                    ab = fmadd3(ab, kw, q);
-                }
-            }
-        }
+        if (gather) gather_hinges<PD, PD && kCheck>(sg, jt, j1, skw, A.hmax, sq, sp, ar, ab);
     }
 
     // ---- own vertices: update, pins, residual ----------------------------------
@@ -704,9 +759,8 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
     const unsigned short* gl = own ? sg : sg2;
     const int j0 = (int)(meta & 0xFFFFu), jt = j0 + (int)((meta >> 16) & 0xFFu),
               j1 = j0 + (int)(meta >> 24);
-    f3 ar = mk(0.0f, 0.0f, 0.0f);
-    if (gather && a.use_stretch)
-        for (int j = j0; j < jt; ++j) ar = ar + lds3(st, gl[j]);
+    f3 ar = mk(0.0f, 0.0f, 0.0f), ab0 = mk(0.0f, 0.0f, 0.0f);
+    if (gather && a.use_stretch) gather_corners<false>(gl, j0, jt, st, nullptr, ar, ab0);
     if (a.use_bend) {
         __syncthreads();
         uint2 rh = rh0;
@@ -740,14 +794,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             sts3(sq, j, pe);
         }
         __syncthreads();
-        if (gather) {
-            for (int j = jt; j < j1; ++j) {
-                const int s = gl[j];
-                const float kw = skw[(s & 3) * P.phmax + (s >> 2)];
-                const f3 d = lds3(sq, s >> 2);
-                ar = fmadd3(ar, kw, d);
-            }
-        }
+        if (gather) gather_hinges<true, false>(gl, jt, j1, skw, P.phmax, sq, nullptr, ar, ab0);
     }
     // x_{k+1} of own and ring-1 vertices; the position table is free (its
     // last reads were in the element loops, before the barrier above)
@@ -799,13 +846,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
     __syncthreads();
     const int k0 = (int)((unsigned)vi.w & 0xFFFFu), kt = k0 + (int)(((unsigned)vi.w >> 16) & 0xFFu),
               k1 = k0 + (int)((unsigned)vi.w >> 24);
-    if (gather2 && a.use_stretch) {
-        for (int j = k0; j < kt; ++j) {
-            const int s = sg[j];
-            ar = ar + lds3(st, s);
-            if (kCheck) ab = ab + lds3(st2, s);
-        }
-    }
+    if (gather2 && a.use_stretch) gather_corners<kCheck>(sg, k0, kt, st, st2, ar, ab);
     if (a.use_bend) {
         __syncthreads();
         uint2 rh = rh0;
@@ -832,18 +873,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             if (kCheck) sts3(sp, j, p);
         }
         __syncthreads();
-        if (gather2) {
-            for (int j = kt; j < k1; ++j) {
-                const int s = sg[j];
-                const float kw = skw[(s & 3) * P.phmax + (s >> 2)];
-                const f3 d = lds3(sq, s >> 2);
-                ar = fmadd3(ar, kw, d);
-                if (kCheck) {
-                    const f3 q = lds3(sp, s >> 2);
-                    ab = fmadd3(ab, kw, q);
-                }
-            }
-        }
+        if (gather2) gather_hinges<true, kCheck>(sg, kt, k1, skw, P.phmax, sq, sp, ar, ab);
     }
 
     // ---- own vertices: x_{k+2}, pins, residual ---------------------------------
@@ -1062,11 +1092,8 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
         const bool gather = own && vi.z < 0;
         f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
         if (gather && a.use_stretch) {
-            for (int j = j0; j < jt; ++j) {
-                const int s = sg[j];
-                ar = ar + lds3(st, s);
-                if (check) ab = ab + lds3(st2, s);
-            }
+            if (check) gather_corners<true>(sg, j0, jt, st, st2, ar, ab);
+            else       gather_corners<false>(sg, j0, jt, st, st2, ar, ab);
         }
         if (a.use_bend) {
             __syncthreads();
@@ -1087,16 +1114,8 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
             }
             __syncthreads();
             if (gather) {
-                for (int j = jt; j < j1; ++j) {
-                    const int s = sg[j];
-                    const float kw = skw[(s & 3) * A.hmax + (s >> 2)];
-                    const f3 d = lds3(sq, s >> 2);
-                    ar = fmadd3(ar, kw, d);
-                    if (check) {
-                        const f3 q = lds3(sp, s >> 2);
-                        ab = fmadd3(ab, kw, q);
-                    }
-                }
+                if (check) gather_hinges<true, true>(sg, jt, j1, skw, A.hmax, sq, sp, ar, ab);
+                else       gather_hinges<true, false>(sg, jt, j1, skw, A.hmax, sq, sp, ar, ab);
             }
         }
 
