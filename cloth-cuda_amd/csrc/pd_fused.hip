@@ -5,6 +5,7 @@
 #include "pd_kernels.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <numeric>
@@ -164,6 +165,17 @@ __device__ __forceinline__ void stsX(double* s, int n, int i, d3 v)
     s[i] = v.x;
     s[n + i] = v.y;
     s[2 * n + i] = v.z;
+}
+
+// Tile of workgroup b among n: workgroup b runs on XCD b % 8, and the
+// workgroups of one XCD take a contiguous range of tiles (a spatial region,
+// whose shared halo vertices then meet in that XCD's L2). Residue class x
+// has n / 8 members, plus one for x < n % 8, so the ranges start at
+// x (n / 8) + min(x, n % 8) and the map is a bijection for every n.
+__device__ __forceinline__ int xcd_tile(int b, int n)
+{
+    const int x = b & 7;
+    return x * (n >> 3) + min(x, n & 7) + (b >> 3);
 }
 
 // Per-vertex arithmetic shared by every kernel that updates a vertex, so that
@@ -342,13 +354,7 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
     }
 
     const int L = A.lmax;
-    int tile = blockIdx.x;
-    if (A.xcd_remap) {
-        // consecutive tiles (a spatial region) on one XCD: workgroup b runs
-        // on XCD b % 8
-        const int per = A.num_tiles >> 3;
-        tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    }
+    const int tile = A.xcd_remap ? xcd_tile(blockIdx.x, A.num_tiles) : (int)blockIdx.x;
     const int vbeg = tile * A.tile_verts;
     const int nown = min(A.tile_verts, A.N - vbeg);
 
@@ -607,11 +613,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
     }
 
     const int L = P.plmax;
-    int tile = blockIdx.x;
-    if (A.xcd_remap) {
-        const int per = A.num_tiles >> 3;
-        tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    }
+    const int tile = A.xcd_remap ? xcd_tile(blockIdx.x, A.num_tiles) : (int)blockIdx.x;
     const int vbeg = tile * A.tile_verts;
     const int nown = min(A.tile_verts, A.N - vbeg);
 
@@ -975,11 +977,7 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
     const FusedIterArgs& a = A.it;
     const int tid = threadIdx.x;
     const int L = A.lmax;
-    int tile = blockIdx.x;
-    if (A.xcd_remap) {
-        const int per = A.num_tiles >> 3;
-        tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    }
+    const int tile = A.xcd_remap ? xcd_tile(blockIdx.x, A.num_tiles) : (int)blockIdx.x;
     const int vbeg = tile * A.tile_verts;
     const int nown = min(A.tile_verts, A.N - vbeg);
 
@@ -1393,6 +1391,8 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
                     const std::vector<int>& seg_in, bool want_pair)
 {
     if (N <= 0) return fail(CLH_ERR_UNSUPPORTED, "fused plan: empty mesh");
+    for (float v : pos)
+        if (!std::isfinite(v)) return fail(CLH_ERR_NUMERIC, "fused plan: non-finite vertex positions");
     const int T = (int)(tris.size() / 3), E = (int)(quads.size() / 4);
     const int V = std::max(32, std::min(tile_verts, kFB));  // one own vertex per thread
 
@@ -1709,7 +1709,7 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
     p.vmax = V;
     p.tri_factor = T > 0 ? (double)tri_gid.size() / T : 0.0;
     p.bend_factor = E > 0 ? (double)hin_gid.size() / E : 0.0;
-    p.xcd_remap = (ntiles % 8 == 0) ? 1 : 0;
+    p.xcd_remap = 1;
     // bytes the PD iteration streams: per vertex the double iterates x_k,
     // x_{k-1}, x_{k+1} (24 B each), base and dy (12 B each) and the 16-byte
     // vertex record; per tile its descriptor and padded halo id block; per

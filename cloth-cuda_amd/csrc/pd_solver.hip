@@ -651,6 +651,11 @@ int ensure_plan(clh_solver* s, const clh_step_args& a)
     if (s->cfg.kernel_variant != CLH_KERNEL_UNFUSED && N > 0 && a.d_pos) {
         std::vector<float> pos((size_t)N * 3);
         CLH_HIP(hipMemcpy(pos.data(), a.d_pos, pos.size() * sizeof(float), hipMemcpyDeviceToHost));
+        // the tile order comes from these positions: a non-finite one would
+        // give tiles of unrelated vertices
+        for (float v : pos)
+            if (!std::isfinite(v))
+                return fail(CLH_ERR_NUMERIC, "clh_solver_step: non-finite positions at the first step");
         int dev = 0, lds_limit = 0, optin = 0;
         CLH_HIP(hipGetDevice(&dev));
         CLH_HIP(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));

@@ -55,3 +55,12 @@ def test_plan_rejects_bad_indices():
     tris[3, 2] = s.N
     with pytest.raises(ch.ClothHipError, match="out of range"):
         ch.fused_plan_check(s.N, tris, s.get("bend_quads"), s.get("pos0"), 128)
+
+
+def test_plan_rejects_non_finite_positions():
+    # the tile order is built from the positions; NaN would order at random
+    s = ch.Scene(**CFG1)
+    pos = s.get("pos0").copy()
+    pos[7, 1] = np.nan
+    with pytest.raises(ch.ClothHipError, match="non-finite"):
+        ch.fused_plan_check(s.N, s.get("tris"), s.get("bend_quads"), pos, 128)
