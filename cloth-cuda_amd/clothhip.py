@@ -78,7 +78,7 @@ class PlanInfo(C.Structure):
                 ("pair_hinge_max", C.c_int), ("pair_bytes_per_launch", C.c_double),
                 ("pair_compute_max", C.c_int), ("pair_lds_bytes", C.c_double),
                 ("element_regions", C.c_int), ("split_lds_bytes", C.c_double),
-                ("lds_limit", C.c_int)]
+                ("lds_limit", C.c_int), ("gather_lanes", C.c_int)]
 
 
 class TuneReport(C.Structure):

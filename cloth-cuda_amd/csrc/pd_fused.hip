@@ -221,76 +221,113 @@ __device__ __forceinline__ void pd_residual_terms(d3 r, f3 ab, float om, f3 ob, 
     bb = fma((double)b.z, (double)b.z, fma((double)b.y, (double)b.y, (double)b.x * (double)b.x));
 }
 
-// Gather sums of one vertex over its list gl, in list order (ascending
-// element id). Every entry is two dependent LDS reads (the slot, then its
-// values); four entries' reads are issued before their sums, which keep the
-// list order, so the result is that of the entry-by-entry loop.
-//
-// Triangle corners j0 .. jt: corner forces st into ar and, with kCheck, the
-// R parts st2 into ab.
-template <bool kCheck>
-__device__ __forceinline__ void gather_corners(const unsigned short* gl, int j0, int jt,
-                                               const float* st, const float* st2, f3& ar, f3& ab)
+// Gather sums of one vertex, in four interleaved partial sums. Partial q
+// takes the entries at offsets q, q + 4, q + 8, ... of each part of the
+// vertex's list (triangle corners from j0, hinge slots from jt), in list
+// order, and the vertex's sum is (P0 + P1) + (P2 + P3). kG adjacent lanes
+// (1, 2 or 4) share a vertex: lane p of the group holds the partials
+// q = p + kG m, m < 4 / kG, and the combination below uses only the
+// commutativity of IEEE addition, so the sum has the same bits for every kG.
+// Every entry is two dependent LDS reads (the slot, then its values); a lane
+// issues U entries' reads before their sums (U = 4, or 2 with four lanes per
+// vertex, whose lists then take one or two such steps per lane); reads past
+// the end repeat the last entry and are not summed.
+template <int kG>
+struct GatherAcc {
+    static constexpr int M = 4 / kG;
+    static constexpr int U = kG == 4 ? 2 : 4;
+    f3 r[M];   // force sums
+    f3 b[M];   // right-hand-side parts (residual check)
+    __device__ GatherAcc()
+    {
+#pragma unroll
+        for (int m = 0; m < M; ++m) r[m] = b[m] = mk(0.0f, 0.0f, 0.0f);
+    }
+};
+
+// Triangle corners j0 .. jt: corner forces st into acc.r and, with kCheck,
+// the R parts st2 into acc.b. p is the lane's index in its group.
+template <bool kCheck, int kG>
+__device__ __forceinline__ void gather_corners(const unsigned short* gl, int j0, int jt, int p,
+                                               const float* st, const float* st2,
+                                               GatherAcc<kG>& acc)
 {
-    int j = j0;
-    for (; j + 4 <= jt; j += 4) {
-        int s[4];
+    constexpr int M = GatherAcc<kG>::M;
+    constexpr int U = GatherAcc<kG>::U;
+    for (int j = j0 + p; j < jt; j += U * kG) {
+        int s[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) s[u] = gl[j + u];
-        f3 v[4];
+        for (int u = 0; u < U; ++u) s[u] = gl[min(j + kG * u, jt - 1)];
+        f3 v[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = lds3(st, s[u]);
+        for (int u = 0; u < U; ++u) v[u] = lds3(st, s[u]);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) ar = ar + v[u];
+        for (int u = 0; u < U; ++u)
+            if (j + kG * u < jt) acc.r[u % M] = acc.r[u % M] + v[u];
         if (kCheck) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = lds3(st2, s[u]);
+            for (int u = 0; u < U; ++u) v[u] = lds3(st2, s[u]);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) ab = ab + v[u];
+            for (int u = 0; u < U; ++u)
+                if (j + kG * u < jt) acc.b[u % M] = acc.b[u % M] + v[u];
         }
-    }
-    for (; j < jt; ++j) {
-        const int s = gl[j];
-        ar = ar + lds3(st, s);
-        if (kCheck) ab = ab + lds3(st2, s);
     }
 }
 
 // Hinge slots jt .. j1: k h^2 w_i (arrays of stride hmax in skw) times the
-// hinge vectors sq into ar (fused in PD mode, unfused in reference mode) and,
-// with kCheck, times the projections sp into ab.
-template <bool kPD, bool kCheck>
-__device__ __forceinline__ void gather_hinges(const unsigned short* gl, int jt, int j1,
+// hinge vectors sq into acc.r (fused in PD mode, unfused in reference mode)
+// and, with kCheck, times the projections sp into acc.b.
+template <bool kPD, bool kCheck, int kG>
+__device__ __forceinline__ void gather_hinges(const unsigned short* gl, int jt, int j1, int p,
                                               const float* skw, int hmax, const float* sq,
-                                              const float* sp, f3& ar, f3& ab)
+                                              const float* sp, GatherAcc<kG>& acc)
 {
-    int j = jt;
-    for (; j + 4 <= j1; j += 4) {
-        int s[4];
+    constexpr int M = GatherAcc<kG>::M;
+    constexpr int U = GatherAcc<kG>::U;
+    for (int j = jt + p; j < j1; j += U * kG) {
+        int s[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) s[u] = gl[j + u];
-        float kw[4];
-        f3 d[4];
+        for (int u = 0; u < U; ++u) s[u] = gl[min(j + kG * u, j1 - 1)];
+        float kw[U];
+        f3 d[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
             kw[u] = skw[(s[u] & 3) * hmax + (s[u] >> 2)];
             d[u] = lds3(sq, s[u] >> 2);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) ar = kPD ? fmadd3(ar, kw[u], d[u]) : madd_nc(ar, kw[u], d[u]);
+        for (int u = 0; u < U; ++u)
+            if (j + kG * u < j1)
+                acc.r[u % M] = kPD ? fmadd3(acc.r[u % M], kw[u], d[u]) : madd_nc(acc.r[u % M], kw[u], d[u]);
         if (kPD && kCheck) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) d[u] = lds3(sp, s[u] >> 2);
+            for (int u = 0; u < U; ++u) d[u] = lds3(sp, s[u] >> 2);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) ab = fmadd3(ab, kw[u], d[u]);
+            for (int u = 0; u < U; ++u)
+                if (j + kG * u < j1) acc.b[u % M] = fmadd3(acc.b[u % M], kw[u], d[u]);
         }
     }
-    for (; j < j1; ++j) {
-        const int s = gl[j];
-        const float kw = skw[(s & 3) * hmax + (s >> 2)];
-        const f3 d = lds3(sq, s >> 2);
-        ar = kPD ? fmadd3(ar, kw, d) : madd_nc(ar, kw, d);
-        if (kPD && kCheck) ab = fmadd3(ab, kw, lds3(sp, s >> 2));
+}
+
+__device__ __forceinline__ f3 shfl_xor3(f3 v, int mask)
+{
+    return f3{__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64), __shfl_xor(v.z, mask, 64)};
+}
+
+// (P0 + P1) + (P2 + P3) from the partials of the lane's group. With kG > 1
+// every lane of the wave must execute it (cross-lane reads).
+template <int kG>
+__device__ __forceinline__ f3 gather_total(const f3* a)
+{
+    if constexpr (kG == 1) {
+        return (a[0] + a[1]) + (a[2] + a[3]);
+    } else if constexpr (kG == 2) {
+        const f3 s0 = a[0] + shfl_xor3(a[0], 1);
+        const f3 s1 = a[1] + shfl_xor3(a[1], 1);
+        return s0 + s1;
+    } else {
+        const f3 s = a[0] + shfl_xor3(a[0], 1);
+        return s + shfl_xor3(s, 2);
     }
 }
 
@@ -337,7 +374,11 @@ __device__ void fused_block_sum2(double r, double b, double* out, int slot)
 // region of their own: their first records then come with trip 2, triangles
 // and hinges are evaluated in one phase, and the own vertices sum both after
 // one barrier (two barriers fewer; same sums in the same order).
-template <int kMode, bool kCheck, int kT, bool kSplit>
+//
+// kG adjacent lanes share an own vertex in the gathers (four partial sums,
+// same bits for every kG); the group's first lane loads the vertex's data and
+// updates it. The host picks kG = min(4, kT / tile_verts).
+template <int kMode, bool kCheck, int kT, bool kSplit, int kG>
 __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
 {
     constexpr bool PD = (kMode == CLH_MODE_PD);
@@ -375,13 +416,15 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
 
     // ---- trip 1: descriptor, own vertices, halo ids --------------------------
     const int* ti = A.tile_info + (size_t)tile * kTileInfo;
-    const bool own = tid < nown;
-    const size_t g = (size_t)vbeg + (own ? tid : 0);
+    const int vl = tid / kG, lane_p = tid % kG;  // own vertex of the lane's group
+    const bool own = vl < nown;
+    const bool lead = own && lane_p == 0;        // loads and updates the vertex
+    const size_t g = (size_t)vbeg + (own ? vl : 0);
     int4 vi = make_int4(0, 0, -1, 0);
     d3 od = d3{0.0, 0.0, 0.0}, oxp = d3{0.0, 0.0, 0.0};
     f3 ob = mk(0.0f, 0.0f, 0.0f), oy = mk(0.0f, 0.0f, 0.0f);
-    if (own) {
-        vi = a.vinfo[g];
+    if (own) vi = a.vinfo[g];
+    if (lead) {
         oy = ld3(a.y, g);
         if (PD) {
             od = ld3d(xd, g);
@@ -431,9 +474,9 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
         rc = A.hin_c[ebeg + tid];
     }
     for (int j = tid; j < (glen + 1) >> 1; j += kT) sgw[j] = A.gl_words[gbeg + j];
-    if (own) {
-        if (PD) stsX(sX, L, tid, to_d(ob) + od);
-        else    sts3(sx, tid, ob);
+    if (lead) {
+        if (PD) stsX(sX, L, vl, to_d(ob) + od);
+        else    sts3(sx, vl, ob);
     }
     __syncthreads();
 
@@ -488,9 +531,9 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
     const int j0 = (int)(meta & 0xFFFFu), jt = j0 + (int)((meta >> 16) & 0xFFu),
               j1 = j0 + (int)(meta >> 24);
     const bool gather = own && opin < 0;
-    f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
+    GatherAcc<kG> acc;
     auto sum_triangles = [&]() {
-        if (gather && a.use_stretch) gather_corners<PD && kCheck>(sg, j0, jt, st, st2, ar, ab);
+        if (gather && a.use_stretch) gather_corners<PD && kCheck, kG>(sg, j0, jt, lane_p, st, st2, acc);
     };
     if (!kSplit) sum_triangles();
     if (kSplit && !a.use_bend) {
@@ -531,12 +574,14 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
         }
         __syncthreads();
         if (kSplit) sum_triangles();
-        if (gather) gather_hinges<PD, PD && kCheck>(sg, jt, j1, skw, A.hmax, sq, sp, ar, ab);
+        if (gather) gather_hinges<PD, PD && kCheck, kG>(sg, jt, j1, lane_p, skw, A.hmax, sq, sp, acc);
     }
+    const f3 ar = gather_total<kG>(acc.r);
+    const f3 ab = (PD && kCheck) ? gather_total<kG>(acc.b) : mk(0.0f, 0.0f, 0.0f);
 
     // ---- own vertices: update, pins, residual ----------------------------------
     double rr = 0.0, bb = 0.0;
-    if (own) {
+    if (lead) {
         const float om = __int_as_float(vi.x), odg = __int_as_float(vi.y);
         if (opin >= 0) {
             const f3 tgt = ld3(a.pin_target, opin);
@@ -554,7 +599,21 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
             }
         }
     }
-    if (kCheck) fused_block_sum2<kT>(rr, bb, a.partials, tile);
+    if (kCheck) {
+        if (kG > 1) {
+            // the block sum takes vertex i's terms in thread i, as with one
+            // lane per vertex; the position table is free (its last reads
+            // came before the barrier ahead of the gathers)
+            if (lead) {
+                sX[vl] = rr;
+                sX[L + vl] = bb;
+            }
+            __syncthreads();
+            rr = tid < nown ? sX[tid] : 0.0;
+            bb = tid < nown ? sX[L + tid] : 0.0;
+        }
+        fused_block_sum2<kT>(rr, bb, a.partials, tile);
+    }
 }
 
 // ---- two PD iterations per launch ---------------------------------------------
@@ -761,8 +820,8 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
     const unsigned short* gl = own ? sg : sg2;
     const int j0 = (int)(meta & 0xFFFFu), jt = j0 + (int)((meta >> 16) & 0xFFu),
               j1 = j0 + (int)(meta >> 24);
-    f3 ar = mk(0.0f, 0.0f, 0.0f), ab0 = mk(0.0f, 0.0f, 0.0f);
-    if (gather && a.use_stretch) gather_corners<false>(gl, j0, jt, st, nullptr, ar, ab0);
+    GatherAcc<1> acc1;
+    if (gather && a.use_stretch) gather_corners<false, 1>(gl, j0, jt, 0, st, nullptr, acc1);
     if (a.use_bend) {
         __syncthreads();
         uint2 rh = rh0;
@@ -796,8 +855,9 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             sts3(sq, j, pe);
         }
         __syncthreads();
-        if (gather) gather_hinges<true, false>(gl, jt, j1, skw, P.phmax, sq, nullptr, ar, ab0);
+        if (gather) gather_hinges<true, false, 1>(gl, jt, j1, 0, skw, P.phmax, sq, nullptr, acc1);
     }
+    const f3 ar1 = gather_total<1>(acc1.r);
     // x_{k+1} of own and ring-1 vertices; the position table is free (its
     // last reads were in the element loops, before the barrier above)
     d3 x1 = d3{0.0, 0.0, 0.0};
@@ -806,7 +866,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             x1 = to_d(ld3(a.pin_target, opin)) - to_d(ob);
         } else {
             d3 r;
-            x1 = pd_update(ar, om, odg, oy, od, oxp, c0.omega, c0.blend, r);
+            x1 = pd_update(ar1, om, odg, oy, od, oxp, c0.omega, c0.blend, r);
         }
         if (own) st3d(static_cast<double*>(a.out), g, x1);
         stsX(sX, L, tid, to_d(ob) + x1);
@@ -814,8 +874,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
     __syncthreads();
 
     // ---- iteration k+1: elements of own vertices -------------------------------
-    ar = mk(0.0f, 0.0f, 0.0f);
-    f3 ab = mk(0.0f, 0.0f, 0.0f);
+    GatherAcc<1> acc2;
     const bool gather2 = own && opin < 0;
     if (a.use_stretch) {
         uint2 rt = rt0;
@@ -848,7 +907,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
     __syncthreads();
     const int k0 = (int)((unsigned)vi.w & 0xFFFFu), kt = k0 + (int)(((unsigned)vi.w >> 16) & 0xFFu),
               k1 = k0 + (int)((unsigned)vi.w >> 24);
-    if (gather2 && a.use_stretch) gather_corners<kCheck>(sg, k0, kt, st, st2, ar, ab);
+    if (gather2 && a.use_stretch) gather_corners<kCheck, 1>(sg, k0, kt, 0, st, st2, acc2);
     if (a.use_bend) {
         __syncthreads();
         uint2 rh = rh0;
@@ -875,8 +934,10 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             if (kCheck) sts3(sp, j, p);
         }
         __syncthreads();
-        if (gather2) gather_hinges<true, kCheck>(sg, kt, k1, skw, P.phmax, sq, sp, ar, ab);
+        if (gather2) gather_hinges<true, kCheck, 1>(sg, kt, k1, 0, skw, P.phmax, sq, sp, acc2);
     }
+    const f3 ar = gather_total<1>(acc2.r);
+    const f3 ab = kCheck ? gather_total<1>(acc2.b) : mk(0.0f, 0.0f, 0.0f);
 
     // ---- own vertices: x_{k+2}, pins, residual ---------------------------------
     double rr = 0.0, bb = 0.0;
@@ -1088,10 +1149,10 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
         const int j0 = (int)(meta & 0xFFFFu), jt = j0 + (int)((meta >> 16) & 0xFFu),
                   j1 = j0 + (int)(meta >> 24);
         const bool gather = own && vi.z < 0;
-        f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
+        GatherAcc<1> acc;
         if (gather && a.use_stretch) {
-            if (check) gather_corners<true>(sg, j0, jt, st, st2, ar, ab);
-            else       gather_corners<false>(sg, j0, jt, st, st2, ar, ab);
+            if (check) gather_corners<true, 1>(sg, j0, jt, 0, st, st2, acc);
+            else       gather_corners<false, 1>(sg, j0, jt, 0, st, st2, acc);
         }
         if (a.use_bend) {
             __syncthreads();
@@ -1112,10 +1173,12 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
             }
             __syncthreads();
             if (gather) {
-                if (check) gather_hinges<true, true>(sg, jt, j1, skw, A.hmax, sq, sp, ar, ab);
-                else       gather_hinges<true, false>(sg, jt, j1, skw, A.hmax, sq, sp, ar, ab);
+                if (check) gather_hinges<true, true, 1>(sg, jt, j1, 0, skw, A.hmax, sq, sp, acc);
+                else       gather_hinges<true, false, 1>(sg, jt, j1, 0, skw, A.hmax, sq, sp, acc);
             }
         }
+        const f3 ar = gather_total<1>(acc.r);
+        const f3 ab = gather_total<1>(acc.b);
 
         // ---- own vertices: update, pins, residual
         double rr = 0.0, bb = 0.0;
@@ -2134,6 +2197,31 @@ int fused_launch_pair(const FusedPlan& p, const FusedIterArgs& a, void* out2, fl
     return hip_status(hipGetLastError(), "two-iteration launch");
 }
 
+// Lanes per own vertex in the gathers of k_fused_iter: as many as the
+// workgroup has threads per tile vertex, at most 4 (a vertex's four partial
+// sums).
+int gather_lanes(const FusedPlan& p)
+{
+    const int r = p.threads / p.tile_verts;
+    return r >= 4 ? 4 : (r >= 2 ? 2 : 1);
+}
+
+template <int kMode, bool kCheck, int kT, bool kSplit>
+void launch_iter(int G, dim3 grid, size_t lds, hipStream_t stream, const FusedKernelArgs& A)
+{
+    if (G == 4)      hipLaunchKernelGGL((k_fused_iter<kMode, kCheck, kT, kSplit, 4>), grid, dim3(kT), lds, stream, A);
+    else if (G == 2) hipLaunchKernelGGL((k_fused_iter<kMode, kCheck, kT, kSplit, 2>), grid, dim3(kT), lds, stream, A);
+    else             hipLaunchKernelGGL((k_fused_iter<kMode, kCheck, kT, kSplit, 1>), grid, dim3(kT), lds, stream, A);
+}
+
+template <int kMode, bool kCheck, int kT, bool kSplit>
+void push_iter_kernels(std::vector<const void*>& v)
+{
+    v.push_back(reinterpret_cast<const void*>(&k_fused_iter<kMode, kCheck, kT, kSplit, 1>));
+    v.push_back(reinterpret_cast<const void*>(&k_fused_iter<kMode, kCheck, kT, kSplit, 2>));
+    v.push_back(reinterpret_cast<const void*>(&k_fused_iter<kMode, kCheck, kT, kSplit, 4>));
+}
+
 int fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
                            hipStream_t stream)
 {
@@ -2146,27 +2234,26 @@ int fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
     A.off_r = (int)L.off_r;
     const size_t lds = L.total;
     const dim3 grid(p.num_tiles);
+    const int G = gather_lanes(p);
     if (p.threads == 512) {
-        const dim3 block(512);
         if (mode == CLH_MODE_PD && split) {
-            if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true, 512, true>), grid, block, lds, stream, A);
-            else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false, 512, true>), grid, block, lds, stream, A);
+            if (check) launch_iter<CLH_MODE_PD, true, 512, true>(G, grid, lds, stream, A);
+            else       launch_iter<CLH_MODE_PD, false, 512, true>(G, grid, lds, stream, A);
         } else if (mode == CLH_MODE_PD) {
-            if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true, 512, false>), grid, block, lds, stream, A);
-            else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false, 512, false>), grid, block, lds, stream, A);
+            if (check) launch_iter<CLH_MODE_PD, true, 512, false>(G, grid, lds, stream, A);
+            else       launch_iter<CLH_MODE_PD, false, 512, false>(G, grid, lds, stream, A);
         } else {
-            hipLaunchKernelGGL((k_fused_iter<CLH_MODE_REF_CUDA, false, 512, false>), grid, block, lds, stream, A);
+            launch_iter<CLH_MODE_REF_CUDA, false, 512, false>(G, grid, lds, stream, A);
         }
     } else {
-        const dim3 block(256);
         if (mode == CLH_MODE_PD && split) {
-            if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true, 256, true>), grid, block, lds, stream, A);
-            else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false, 256, true>), grid, block, lds, stream, A);
+            if (check) launch_iter<CLH_MODE_PD, true, 256, true>(G, grid, lds, stream, A);
+            else       launch_iter<CLH_MODE_PD, false, 256, true>(G, grid, lds, stream, A);
         } else if (mode == CLH_MODE_PD) {
-            if (check) hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, true, 256, false>), grid, block, lds, stream, A);
-            else       hipLaunchKernelGGL((k_fused_iter<CLH_MODE_PD, false, 256, false>), grid, block, lds, stream, A);
+            if (check) launch_iter<CLH_MODE_PD, true, 256, false>(G, grid, lds, stream, A);
+            else       launch_iter<CLH_MODE_PD, false, 256, false>(G, grid, lds, stream, A);
         } else {
-            hipLaunchKernelGGL((k_fused_iter<CLH_MODE_REF_CUDA, false, 256, false>), grid, block, lds, stream, A);
+            launch_iter<CLH_MODE_REF_CUDA, false, 256, false>(G, grid, lds, stream, A);
         }
     }
     return hip_status(hipGetLastError(), "fused iteration launch");
@@ -2179,21 +2266,22 @@ int fused_enable_lds(int& bytes)
     // above 64 KB the kernels must opt in; when the runtime refuses, stay at
     // 64 KB
     if (bytes <= 64 * 1024) return CLH_OK;
-    const void* fns[] = {reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 256, false>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 256, false>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false, 256, false>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 512, false>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 512, false>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_REF_CUDA, false, 512, false>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 256, true>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 256, true>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, true, 512, true>),
-                         reinterpret_cast<const void*>(&k_fused_iter<CLH_MODE_PD, false, 512, true>),
-                         reinterpret_cast<const void*>(&k_fused_persist<256>),
-                         reinterpret_cast<const void*>(&k_fused_pair<true, 256>),
-                         reinterpret_cast<const void*>(&k_fused_pair<false, 256>),
-                         reinterpret_cast<const void*>(&k_fused_pair<true, 512>),
-                         reinterpret_cast<const void*>(&k_fused_pair<false, 512>)};
+    std::vector<const void*> fns;
+    push_iter_kernels<CLH_MODE_PD, true, 256, false>(fns);
+    push_iter_kernels<CLH_MODE_PD, false, 256, false>(fns);
+    push_iter_kernels<CLH_MODE_REF_CUDA, false, 256, false>(fns);
+    push_iter_kernels<CLH_MODE_PD, true, 512, false>(fns);
+    push_iter_kernels<CLH_MODE_PD, false, 512, false>(fns);
+    push_iter_kernels<CLH_MODE_REF_CUDA, false, 512, false>(fns);
+    push_iter_kernels<CLH_MODE_PD, true, 256, true>(fns);
+    push_iter_kernels<CLH_MODE_PD, false, 256, true>(fns);
+    push_iter_kernels<CLH_MODE_PD, true, 512, true>(fns);
+    push_iter_kernels<CLH_MODE_PD, false, 512, true>(fns);
+    fns.push_back(reinterpret_cast<const void*>(&k_fused_persist<256>));
+    fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<true, 256>));
+    fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<false, 256>));
+    fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<true, 512>));
+    fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<false, 512>));
     for (const void* f : fns) {
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -2291,6 +2379,7 @@ void fused_plan_info(const FusedPlan& p, clh_plan_info* out)
     }
     out->split_lds_bytes = (double)lds_layout(p, CLH_MODE_PD, true, true).total;
     out->element_regions = use_split(p, CLH_MODE_PD, false) ? 2 : 1;
+    out->gather_lanes = gather_lanes(p);
 }
 
 void fused_plan_release(FusedPlan& p)

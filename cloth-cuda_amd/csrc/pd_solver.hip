@@ -227,15 +227,25 @@ __global__ void __launch_bounds__(kBlock) k_vertex_ref(VertexArgs a)
         st3(out, v, ld3(a.pin_target, slot));
         return;
     }
-    f3 acc = mk(0.0f, 0.0f, 0.0f);
+    // the fused kernels' order: four interleaved partial sums (entry offset
+    // mod 4 in the triangle list and in the hinge list), (P0 + P1) + (P2 + P3)
+    f3 P[4] = {mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f),
+               mk(0.0f, 0.0f, 0.0f)};
     if (a.use_stretch)
-        for (int j = a.vt_ptr[v]; j < a.vt_ptr[v + 1]; ++j) acc = acc + ld3(a.fr, a.vt_idx[j]);
+        for (int j = a.vt_ptr[v], e = a.vt_ptr[v + 1]; j < e; j += 4)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (j + u < e) P[u] = P[u] + ld3(a.fr, a.vt_idx[j + u]);
     if (a.use_bend)
-        for (int j = a.ve_ptr[v]; j < a.ve_ptr[v + 1]; ++j) {
-            const int s = a.ve_idx[j];
-            const float kw = (a.bk[s >> 2] * a.h2) * a.bw[s];
-            acc = madd_nc(acc, kw, ld3(a.bp, s >> 2));
-        }
+        for (int j = a.ve_ptr[v], e = a.ve_ptr[v + 1]; j < e; j += 4)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (j + u < e) {
+                    const int s = a.ve_idx[j + u];
+                    const float kw = (a.bk[s >> 2] * a.h2) * a.bw[s];
+                    P[u] = madd_nc(P[u], kw, ld3(a.bp, s >> 2));
+                }
+    const f3 acc = (P[0] + P[1]) + (P[2] + P[3]);
     const float m = a.mass[v];
     const f3 rhs = madd_nc(acc, m, ld3(a.y, v));
     const float d = a.diag[v];
@@ -284,21 +294,32 @@ __global__ void __launch_bounds__(kBlock) k_vertex_pd(VertexArgs a)
         if (slot >= 0) {
             st3d(out, v, to_d(ld3(a.pin_target, slot)) - to_d(bv));
         } else {
-            f3 ar = mk(0.0f, 0.0f, 0.0f), ab = mk(0.0f, 0.0f, 0.0f);
+            // the fused kernels' order of the gather sums (four partials)
+            f3 PR[4], PB[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) PR[u] = PB[u] = mk(0.0f, 0.0f, 0.0f);
             if (a.use_stretch)
-                for (int j = a.vt_ptr[v]; j < a.vt_ptr[v + 1]; ++j) {
-                    const int s = a.vt_idx[j];
-                    ar = ar + ld3(a.fd, s);
-                    ab = ab + ld3(a.fr, s);
-                }
+                for (int j = a.vt_ptr[v], e = a.vt_ptr[v + 1]; j < e; j += 4)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (j + u < e) {
+                            const int s = a.vt_idx[j + u];
+                            PR[u] = PR[u] + ld3(a.fd, s);
+                            PB[u] = PB[u] + ld3(a.fr, s);
+                        }
             if (a.use_bend)
-                for (int j = a.ve_ptr[v]; j < a.ve_ptr[v + 1]; ++j) {
-                    const int s = a.ve_idx[j];
-                    const float kw = (a.bk[s >> 2] * a.h2) * a.bw[s];
-                    const f3 d = ld3(a.bd, s >> 2), p = ld3(a.bp, s >> 2);
-                    ar = ar + mk(kw * d.x, kw * d.y, kw * d.z);
-                    ab = ab + mk(kw * p.x, kw * p.y, kw * p.z);
-                }
+                for (int j = a.ve_ptr[v], e = a.ve_ptr[v + 1]; j < e; j += 4)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (j + u < e) {
+                            const int s = a.ve_idx[j + u];
+                            const float kw = (a.bk[s >> 2] * a.h2) * a.bw[s];
+                            const f3 d = ld3(a.bd, s >> 2), p = ld3(a.bp, s >> 2);
+                            PR[u] = PR[u] + mk(kw * d.x, kw * d.y, kw * d.z);
+                            PB[u] = PB[u] + mk(kw * p.x, kw * p.y, kw * p.z);
+                        }
+            const f3 ar = (PR[0] + PR[1]) + (PR[2] + PR[3]);
+            const f3 ab = (PB[0] + PB[1]) + (PB[2] + PB[3]);
             const float m = a.mass[v];
             const f3 yv = ld3(a.y, v);
             const d3 xv = ld3d(x, v);
@@ -1525,9 +1546,21 @@ extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int nu
     const bool pd = s->cfg.mode == CLH_MODE_PD;
     const bool try_pairs = pd && s->cfg.iters_per_launch == 2;
     int n = 0, best = -1;
+    // The GPU's clocks ramp up over the first tens of milliseconds of work
+    // that follows the host-side replan: each tile size starts with about
+    // 20 ms of launches that are not kept, and each candidate takes the lower
+    // of two timings.
+    auto warm_up = [&]() -> int {
+        float ms = 0.0f;
+        CLH_TRY(time_iterations(s, *a, num_launches, &ms));
+        const double n = std::ceil(20.0 / std::max((double)ms, 1e-4));
+        CLH_TRY(time_iterations(s, *a, (int)std::min(n, 100000.0), &ms));
+        return CLH_OK;
+    };
     for (int tv : tiles) {
         int got = 0;
         CLH_TRY(replan(s, *a, tv, 256, got));
+        if (got == tv) CLH_TRY(warm_up());
         for (int th : threads) {
             for (int ipl = 1; ipl <= (try_pairs ? 2 : 1); ++ipl) {
                 for (int reg = 1; reg <= (pd && ipl == 1 ? 2 : 1); ++reg) {
@@ -1542,9 +1575,10 @@ extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int nu
                         fused_plan_info(s->fused, &pi);
                         const bool runs = ipl == 2 ? pair_active(s) : (reg == 1 || pi.element_regions == 2);
                         if (runs) {
-                            float ms = 0.0f;
-                            CLH_TRY(time_iterations(s, *a, num_launches, &ms));
-                            out->us_per_iteration[n] = 1e3f * ms / (float)ipl;
+                            float ms0 = 0.0f, ms1 = 0.0f;
+                            CLH_TRY(time_iterations(s, *a, num_launches, &ms0));
+                            CLH_TRY(time_iterations(s, *a, num_launches, &ms1));
+                            out->us_per_iteration[n] = 1e3f * std::min(ms0, ms1) / (float)ipl;
                             if (best < 0 || out->us_per_iteration[n] < out->us_per_iteration[best])
                                 best = n;
                         }

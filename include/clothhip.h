@@ -257,6 +257,8 @@ typedef struct clh_plan_info {
                                 regions, else 1 */
     double split_lds_bytes;  /* LDS per workgroup with separate regions (check variant) */
     int    lds_limit;        /* LDS a workgroup of the fused kernels may use on this device */
+    int    gather_lanes;     /* lanes per own vertex in the gathers of a one-iteration launch
+                                (1, 2 or 4; the sums have the same bits for each) */
 } clh_plan_info;
 int clh_solver_plan_info(clh_solver* s, clh_plan_info* out);
 
@@ -296,9 +298,10 @@ int clh_batch_cloth_residuals(clh_solver* s, double* out, int max_count, int* co
  * {32, 64, 128, 256}, block_threads in {256, 512}; in CLH_MODE_PD element
  * regions in {1, 2} where separate regions fit, and with iters_per_launch = 2
  * also two iterations per launch) candidate on this
- * solver's mesh (num_launches steady-phase launches each, as timed by
- * clh_solver_time_iteration_kernel) and keeps the one with the shortest time
- * per iteration. Results are cached per process under a key of the vertex,
+ * solver's mesh (the lower of two timings of num_launches steady-phase
+ * launches each, as timed by clh_solver_time_iteration_kernel, after about
+ * 20 ms of warm-up launches per tile size) and keeps the one with the
+ * shortest time per iteration. Results are cached per process under a key of the vertex,
  * triangle, hinge and cloth counts, the mode, whether two-iteration launches
  * may be used and the histogram of vertex valences, so another solver on an
  * equally shaped mesh reuses the choice without timing. */
