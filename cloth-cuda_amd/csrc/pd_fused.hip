@@ -2155,7 +2155,11 @@ int fused_launch_persist(const FusedPlan& p, const FusedIterArgs& a, double* con
 
 size_t fused_pair_lds_bytes(const FusedPlan& p, bool check) { return pair_lds_layout(p, check).total; }
 
-bool fused_pair_usable(const FusedPlan& p) { return p.pair && p.pcmax <= p.threads; }
+// the two-iteration kernel is built for 256- and 512-thread workgroups
+bool fused_pair_usable(const FusedPlan& p)
+{
+    return p.pair && p.threads <= 512 && p.pcmax <= p.threads;
+}
 
 int fused_launch_pair(const FusedPlan& p, const FusedIterArgs& a, void* out2, float omega1,
                       int blend1, hipStream_t stream)
@@ -2235,7 +2239,17 @@ int fused_launch_iteration(const FusedPlan& p, const FusedIterArgs& a, int mode,
     const size_t lds = L.total;
     const dim3 grid(p.num_tiles);
     const int G = gather_lanes(p);
-    if (p.threads == 512) {
+    if (p.threads == 1024) {
+        if (mode == CLH_MODE_PD && split) {
+            if (check) launch_iter<CLH_MODE_PD, true, 1024, true>(G, grid, lds, stream, A);
+            else       launch_iter<CLH_MODE_PD, false, 1024, true>(G, grid, lds, stream, A);
+        } else if (mode == CLH_MODE_PD) {
+            if (check) launch_iter<CLH_MODE_PD, true, 1024, false>(G, grid, lds, stream, A);
+            else       launch_iter<CLH_MODE_PD, false, 1024, false>(G, grid, lds, stream, A);
+        } else {
+            launch_iter<CLH_MODE_REF_CUDA, false, 1024, false>(G, grid, lds, stream, A);
+        }
+    } else if (p.threads == 512) {
         if (mode == CLH_MODE_PD && split) {
             if (check) launch_iter<CLH_MODE_PD, true, 512, true>(G, grid, lds, stream, A);
             else       launch_iter<CLH_MODE_PD, false, 512, true>(G, grid, lds, stream, A);
@@ -2277,6 +2291,11 @@ int fused_enable_lds(int& bytes)
     push_iter_kernels<CLH_MODE_PD, false, 256, true>(fns);
     push_iter_kernels<CLH_MODE_PD, true, 512, true>(fns);
     push_iter_kernels<CLH_MODE_PD, false, 512, true>(fns);
+    push_iter_kernels<CLH_MODE_PD, true, 1024, false>(fns);
+    push_iter_kernels<CLH_MODE_PD, false, 1024, false>(fns);
+    push_iter_kernels<CLH_MODE_REF_CUDA, false, 1024, false>(fns);
+    push_iter_kernels<CLH_MODE_PD, true, 1024, true>(fns);
+    push_iter_kernels<CLH_MODE_PD, false, 1024, true>(fns);
     fns.push_back(reinterpret_cast<const void*>(&k_fused_persist<256>));
     fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<true, 256>));
     fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<false, 256>));

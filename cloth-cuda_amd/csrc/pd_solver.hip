@@ -1229,9 +1229,10 @@ extern "C" int clh_solver_create(const clh_config* cfg, const clh_mesh_desc* mes
     s->T = mesh->num_tris;
     s->E = mesh->num_bend;
     if (cfg->tile_verts > 0) s->tile_verts = std::max(32, std::min(cfg->tile_verts, 1024));
-    if (cfg->block_threads != 0 && cfg->block_threads != 256 && cfg->block_threads != 512) {
+    if (cfg->block_threads != 0 && cfg->block_threads != 256 && cfg->block_threads != 512 &&
+        cfg->block_threads != 1024) {
         clh_solver_destroy(s);
-        return fail(CLH_ERR_INVALID, "clh_solver_create: block_threads must be 0, 256 or 512");
+        return fail(CLH_ERR_INVALID, "clh_solver_create: block_threads must be 0, 256, 512 or 1024");
     }
     if (cfg->block_threads > 0) s->block_threads = cfg->block_threads;
     if (cfg->iters_per_launch < 0 || cfg->iters_per_launch > 2) {
@@ -1542,7 +1543,7 @@ extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int nu
             return CLH_OK;
         }
     }
-    const int tiles[4] = {32, 64, 128, 256}, threads[2] = {256, 512};
+    const int tiles[4] = {32, 64, 128, 256}, threads[3] = {256, 512, 1024};
     const bool pd = s->cfg.mode == CLH_MODE_PD;
     const bool try_pairs = pd && s->cfg.iters_per_launch == 2;
     int n = 0, best = -1;
@@ -1562,7 +1563,8 @@ extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int nu
         CLH_TRY(replan(s, *a, tv, 256, got));
         if (got == tv) CLH_TRY(warm_up());
         for (int th : threads) {
-            for (int ipl = 1; ipl <= (try_pairs ? 2 : 1); ++ipl) {
+            // two-iteration launches use 256- or 512-thread workgroups
+            for (int ipl = 1; ipl <= (try_pairs && th <= 512 ? 2 : 1); ++ipl) {
                 for (int reg = 1; reg <= (pd && ipl == 1 ? 2 : 1); ++reg) {
                     out->tile_verts[n] = tv;
                     out->block_threads[n] = th;

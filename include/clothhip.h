@@ -111,7 +111,8 @@ typedef struct clh_config {
     int   kernel_variant;   /* CLH_KERNEL_*                                     */
     int   use_graph;        /* capture fixed-length frames in a hipGraph        */
     int   tile_verts;       /* fused variant: vertices per tile (0 = default)   */
-    int   block_threads;    /* fused variant: workgroup size, 256 or 512 (0 = default) */
+    int   block_threads;    /* fused variant: workgroup size, 256, 512 or 1024 (0 = default;
+                               two-iteration launches need 256 or 512) */
     int   persistent;       /* CLH_PERSIST_ON: frames to a tolerance in one cooperative
                                launch per chunk (grid barrier between iterations) when
                                every tile fits resident; off by default: on MI355X the
@@ -295,9 +296,10 @@ int clh_batch_create(const clh_config* cfg, const clh_mesh_desc* mesh, int num_c
 int clh_batch_cloth_residuals(clh_solver* s, double* out, int max_count, int* count);
 
 /* Autotuner of the fused kernel's launch shape: times every (tile_verts in
- * {32, 64, 128, 256}, block_threads in {256, 512}; in CLH_MODE_PD element
- * regions in {1, 2} where separate regions fit, and with iters_per_launch = 2
- * also two iterations per launch) candidate on this
+ * {32, 64, 128, 256}, block_threads in {256, 512, 1024}; in CLH_MODE_PD
+ * element regions in {1, 2} where separate regions fit, and with
+ * iters_per_launch = 2 also two iterations per launch at 256 and 512
+ * threads; at most 32 candidates) candidate on this
  * solver's mesh (the lower of two timings of num_launches steady-phase
  * launches each, as timed by clh_solver_time_iteration_kernel, after about
  * 20 ms of warm-up launches per tile size) and keeps the one with the

@@ -138,8 +138,9 @@ def test_autotune_caches_and_keeps_results():
     b.set_state(pos0, vel0)
     sol = ch.Solver(c, b.N, b.T, b.E, cloth_offsets=b.vert_off)
     rep = sol.autotune(b.step_args(), 10)
-    # 4 tile sizes x 2 workgroup sizes x 1 or 2 element regions (PD)
-    assert rep.num_candidates == 16 and rep.from_cache == 0
+    # 4 tile sizes x 3 workgroup sizes x 1 or 2 element regions (PD)
+    assert rep.num_candidates == 24 and rep.from_cache == 0
+    assert sorted({cnd["block_threads"] for cnd in rep.candidates()}) == [256, 512, 1024]
     cands = rep.candidates()
     times = [cnd["us_per_iteration"] for cnd in cands]
     assert rep.us_per_iteration[rep.best] == min(t for t in times if t > 0)

@@ -1,5 +1,5 @@
 """Lanes per own vertex in the gathers of the one-iteration kernel (1, 2 or
-4, from the workgroup size over the tile size). A vertex's gather is four
+4, from the workgroup size of 256, 512 or 1024 threads over the tile size). A vertex's gather is four
 interleaved partial sums combined as (P0 + P1) + (P2 + P3) whatever the lane
 count, and the element and gather order follows the global element id, so
 fixed-length frames must have the same bits for every tile and workgroup
@@ -25,7 +25,7 @@ CASES = {
 
 # (tile_verts, block_threads) -> lanes per vertex
 SHAPES = {(256, 256): 1, (128, 256): 2, (256, 512): 2, (64, 256): 4, (128, 512): 4,
-          (32, 512): 4}
+          (32, 512): 4, (256, 1024): 4, (64, 1024): 4}
 
 
 def run(spec, shape, regions, frames=3, K=17, wind=None):
@@ -80,3 +80,33 @@ def test_reference_mode_lane_counts_agree():
     for pos, vel in outs[1:]:
         np.testing.assert_array_equal(pos, outs[0][0])
         np.testing.assert_array_equal(vel, outs[0][1])
+
+
+def test_two_iteration_setting_at_1024_threads():
+    """The two-iteration kernel is built for 256 and 512 threads: with 1024
+    the frames run one iteration per launch, with the same bits as the
+    two-iteration launches at 512."""
+    spec = CASES["irregular24_top"]
+    outs = []
+    for th in (1024, 512):
+        s = ch.Scene(**spec)
+        s.upload()
+        sol = ch.Solver(ch.config(mode=ch.MODE_PD, gravity=-9.81, dt=spec["dt"], rho=0.99,
+                                  cheb_delay=1, kernel_variant=ch.KERNEL_FUSED,
+                                  max_iterations=16, iters_per_launch=2, tile_verts=128,
+                                  block_threads=th), s.N, s.T, s.E)
+        args = s.step_args()
+        for _ in range(2):
+            sol.step(args)
+        info = sol.plan_info()
+        assert info.block_threads == th
+        assert info.iters_per_launch == (1 if th == 1024 else 2)
+        outs.append(s.get_state())
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+def test_block_threads_validated():
+    s = ch.Scene(**CFG1)
+    with pytest.raises(ch.ClothHipError, match="block_threads"):
+        ch.Solver(ch.config(mode=ch.MODE_PD, block_threads=768), s.N, s.T, s.E)

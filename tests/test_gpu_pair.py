@@ -199,10 +199,10 @@ def test_timing_uses_the_launch_in_use():
 
 def test_autotune_chooses_between_one_and_two():
     """With iters_per_launch = 2 the autotuner also times the two-iteration
-    kernel (per tile and workgroup size: one iteration with one or two element
-    regions, two iterations; 24 candidates), keeps the shortest time per
-    iteration, and the frames equal the single-iteration solver's whichever
-    wins."""
+    kernel (per tile size: one iteration with one or two element regions at
+    256, 512 and 1024 threads, two iterations at 256 and 512; 32 candidates),
+    keeps the shortest time per iteration, and the frames equal the
+    single-iteration solver's whichever wins."""
     spec = CASES["irregular24_top"]
     one = run(spec, 1, 2, max_iterations=20)
     s = ch.Scene(**spec)
@@ -212,7 +212,8 @@ def test_autotune_chooses_between_one_and_two():
     args = s.step_args()
     rep = sol.autotune(args, 5)
     cands = rep.candidates()
-    assert rep.num_candidates == 24
+    assert rep.num_candidates == 32
+    assert all(c["block_threads"] <= 512 for c in rep.candidates() if c["iters_per_launch"] == 2)
     assert sorted({c["iters_per_launch"] for c in cands}) == [1, 2]
     timed = [c["us_per_iteration"] for c in cands if c["us_per_iteration"] > 0]
     assert rep.us_per_iteration[rep.best] == min(timed)
