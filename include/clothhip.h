@@ -221,9 +221,10 @@ int clh_solver_run_frames(clh_solver* s, const clh_step_args* a,
 
 /* Launches the dominant iteration kernel num_launches times back to back on
  * the solver's stream, bracketed by events, and returns the average time per
- * launch. The launches repeat one steady-phase launch (x_k and x_{k-1} read,
- * the Chebyshev blend on in PD mode; two iterations per launch when
- * clh_plan_info.iters_per_launch is 2) on iterates computed from the mesh
+ * launch. The launches are consecutive steady-phase iterations (x_k and
+ * x_{k-1} read, each launch reading what the one before wrote, the Chebyshev
+ * blend on in PD mode at a fixed weight; two iterations per launch when
+ * clh_plan_info.iters_per_launch is 2) from iterates computed from the mesh
  * state; only solver scratch is written, the mesh state is left as it was. */
 int clh_solver_time_iteration_kernel(clh_solver* s, const clh_step_args* a,
                                      int num_launches, float* avg_ms);
