@@ -13,7 +13,9 @@ The plain method is both the arbiter and the fallback: with a fallback solver
 frame runs under the warm solver's own smaller cap, and a frame that misses
 the tolerance within it is redone from (x_n, v_n) by the fallback solver, so
 its result is the plain frame's. After `strikes` consecutive fallbacks the
-warm start pauses for `cooldown` frames.
+warm start pauses for `cooldown` frames. Before it is used, `select_gain`
+compares the warm starts with the plain start on held-out frames; a frame
+loop without a warm start (ws None) runs plain frames in every mode.
 """
 from __future__ import annotations
 
@@ -72,8 +74,8 @@ class FrameLoop:
         n = self.n
         xm2, xm1, x = self.hist[(n + 1) % 3], self.hist[(n + 2) % 3], self.hist[n % 3]
         self._copy(x.ptr, self.scene.device("pos"))
-        warm = mode != "plain" and self._pause == 0
-        if mode != "plain" and self._pause > 0:
+        warm = mode != "plain" and self.ws is not None and self._pause == 0
+        if mode != "plain" and self.ws is not None and self._pause > 0:
             self._pause -= 1
             self.paused_frames += 1
         used = self.solver
@@ -126,18 +128,26 @@ class FrameLoop:
 
 
 def select_gain(scene, solver, ws, gains=(0.0, 0.25, 0.5, 1.0), warmup=3, frames=6,
-                cloth_accel=None):
+                cloth_accel=None, plain=True):
     """Chooses the factor on the network's correction by the iterations it
     saves on held-out frames: from the same state, each gain runs `warmup`
     plain frames (to fill the position history) and then `frames` learned
     frames on `scene` with `solver`, which should iterate to the tolerance
     the warm start is used with. Gain 0 is the extrapolation alone, so on
-    these frames the chosen start is never slower than it. Sets the chosen
-    gain on `ws` (its output scale times the gain), restores the scene's
-    state and returns (gain, {gain: mean iterations per learned frame})."""
+    these frames the chosen start is never slower than it. With `plain`, the
+    plain start runs the same frames too and is the choice (returned as gain
+    None) when no gain needs fewer iterations: the warm start is then not to
+    be used. Sets the chosen gain on `ws` (its output scale times the gain;
+    unchanged for None), restores the scene's state and returns
+    (gain, {gain or "plain": mean iterations per frame})."""
     params, scale = ws.params()
     pos0, vel0 = scene.get_state()
     res = {}
+    if plain:
+        loop = FrameLoop(scene, solver, None, cloth_accel)
+        loop.run(warmup, mode="plain")
+        its, _ = loop.run(frames, mode="plain")
+        res["plain"] = float(np.mean(its))
     for g in gains:
         scene.set_state(pos0, vel0)
         ws.set_params(params, scale * g)
@@ -146,6 +156,8 @@ def select_gain(scene, solver, ws, gains=(0.0, 0.25, 0.5, 1.0), warmup=3, frames
         its, _ = loop.run(frames, mode="learned")
         res[g] = float(np.mean(its))
     best = min(gains, key=lambda g: res[g])
-    ws.set_params(params, scale * best)
+    if plain and res["plain"] <= res[best]:
+        best = None
+    ws.set_params(params, scale * (1.0 if best is None else best))
     scene.set_state(pos0, vel0)
     return best, res

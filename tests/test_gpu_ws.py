@@ -231,10 +231,10 @@ def test_select_gain_on_held_out_frames():
     pos0, vel0 = v.get_state()
     gain, table = select_gain(v, vsol, vws, gains=(0.0, 1.0), warmup=2, frames=4,
                               cloth_accel=vaccel)
-    assert set(table) == {0.0, 1.0}
-    assert table[gain] == min(table.values())
+    assert set(table) == {0.0, 1.0, "plain"}
+    assert table["plain" if gain is None else gain] == min(table.values())
     np.testing.assert_array_equal(vws.params()[0], params)
-    assert vws.params()[1] == np.float32(scale * gain)
+    assert vws.params()[1] == np.float32(scale * (1.0 if gain is None else gain))
     np.testing.assert_array_equal(v.get_state()[0], pos0)
     # gain 0: the learned frames are the extrapolation's
     vws.set_params(params, 0.0)
@@ -296,3 +296,40 @@ def test_start_residual_is_the_frames_first_check():
         h = sol.residual_history(8)
         assert h[0] == r, (h[0], r)
         s.set_state(pos0, vel0)
+
+
+def test_select_gain_keeps_the_plain_start_when_it_wins():
+    """A correction scaled far beyond the trained one (gain 40) cannot beat
+    the plain start on held-out frames: the choice is the plain start (gain
+    None), the warm start's parameters are left as they were, and a frame
+    loop without a warm start runs the plain frames exactly."""
+    specs, accel = cfg5_specs(8, seed=101, n=16)
+    b = ch.Scene(specs=specs)
+    b.upload()
+    sol = ch.Solver(solver_cfg(cap=4000), b.N, b.T, b.E, cloth_offsets=b.vert_off)
+    ws = ch.WarmStart.for_scene(b, DT, G, seed=3)
+    FrameLoop(b, sol, ws, accel).run(12, mode="extrap", dataset=True)
+    ws.fit_output_scale()
+    ws.train(300, 4096, 2e-3)
+    params, scale = ws.params()
+
+    vspecs, vaccel = cfg5_specs(4, seed=901, n=16, cloth_seed=7000)
+    v = ch.Scene(specs=vspecs)
+    v.upload()
+    vsol = ch.Solver(solver_cfg(cap=4000), v.N, v.T, v.E, cloth_offsets=v.vert_off)
+    vws = ch.WarmStart.for_scene(v, DT, G)
+    vws.set_params(params, scale)
+    pos0, vel0 = v.get_state()
+    gain, table = select_gain(v, vsol, vws, gains=(40.0,), warmup=2, frames=4,
+                              cloth_accel=vaccel)
+    assert table["plain"] < table[40.0]
+    assert gain is None
+    np.testing.assert_array_equal(vws.params()[0], params)
+    assert vws.params()[1] == np.float32(scale)
+    np.testing.assert_array_equal(v.get_state()[0], pos0)
+    # no warm start: every mode runs the plain frames
+    a, _ = FrameLoop(v, vsol, None, vaccel).run(4, mode="learned")
+    v.set_state(pos0, vel0)
+    p, _ = FrameLoop(v, vsol, None, vaccel).run(4, mode="plain")
+    assert a == p
+    assert table["plain"] > 0
