@@ -414,29 +414,33 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
     const float* xf = static_cast<const float*>(a.x);
     const double* xd = static_cast<const double*>(a.x);
 
+    // Each trip's loads are issued, without branches, before any of their
+    // values is used: lanes with nothing to load read an in-range element
+    // (index 0, or the tile's first own vertex), so no consumer placed inside a
+    // branch makes the lane wait for one load before issuing the next, and a
+    // trip is one round trip to memory.
+
     // ---- trip 1: descriptor, own vertices, halo ids --------------------------
     const int* ti = A.tile_info + (size_t)tile * kTileInfo;
     const int vl = tid / kG, lane_p = tid % kG;  // own vertex of the lane's group
     const bool own = vl < nown;
     const bool lead = own && lane_p == 0;        // loads and updates the vertex
     const size_t g = (size_t)vbeg + (own ? vl : 0);
-    int4 vi = make_int4(0, 0, -1, 0);
+    const int4 vld = a.vinfo[g];
+    const f3 oy = ld3(a.y, g);
     d3 od = d3{0.0, 0.0, 0.0}, oxp = d3{0.0, 0.0, 0.0};
-    f3 ob = mk(0.0f, 0.0f, 0.0f), oy = mk(0.0f, 0.0f, 0.0f);
-    if (own) vi = a.vinfo[g];
-    if (lead) {
-        oy = ld3(a.y, g);
-        if (PD) {
-            od = ld3d(xd, g);
-            ob = ld3(a.base, g);
-            // under device control the blend flag arrives with this trip
-            if (a.blend || a.ctl) oxp = ld3d(static_cast<const double*>(a.xkm1), g);
-        } else {
-            ob = ld3(xf, g);
-        }
+    f3 ob;
+    if (PD) {
+        od = ld3d(xd, g);
+        ob = ld3(a.base, g);
+        // under device control the blend flag arrives with this trip
+        if (a.blend || a.ctl) oxp = ld3d(static_cast<const double*>(a.xkm1), g);
+    } else {
+        ob = ld3(xf, g);
     }
     const int* hids = A.halo + (size_t)tile * A.hstride;
-    const int hid0 = tid < A.hstride ? hids[tid] : 0;
+    const int hid0 = hids[min(tid, A.hstride - 1)];
+    const int4 vi = own ? vld : make_int4(0, 0, -1, 0);
 
     const int nhalo = ti[0], tbeg = ti[1], ntri = ti[2], ebeg = ti[3], nhin = ti[4];
     const int gbeg = ti[5], glen = ti[6];
@@ -448,32 +452,42 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
     }
 
     // ---- trip 2: halo positions, first records, gather list -------------------
+    const int hsel = tid < nhalo ? hid0 : vbeg;
+    f3 hb = mk(0.0f, 0.0f, 0.0f), hf = mk(0.0f, 0.0f, 0.0f);
+    d3 hx = d3{0.0, 0.0, 0.0};
+    if (PD) {
+        hb = ld3(a.base, hsel);
+        hx = ld3d(xd, hsel);
+    } else {
+        hf = ld3(xf, hsel);
+    }
+    const size_t tj = tid < ntri ? (size_t)tbeg + tid : 0;
+    uint2 rt = A.tri_idx[tj];
+    float4 rG = A.tri_c0[tj];
+    float rwA = A.tri_c1[tj];
+    uint2 rh = make_uint2(0, 0);
+    float4 rw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float2 rc = make_float2(0.0f, 0.0f);
+    if (kSplit) {
+        const size_t hj = tid < nhin ? (size_t)ebeg + tid : 0;
+        rh = A.hin_idx[hj];
+        rw = A.hin_w[hj];
+        rc = A.hin_c[hj];
+    }
+    const int nwords = (glen + 1) >> 1;
+    const unsigned gw0 = A.gl_words[tid < nwords ? gbeg + tid : 0];
+    // consumers
     if (tid < nhalo) {
-        if (PD) stsX(sX, L, nown + tid, to_d(ld3(a.base, hid0)) + ld3d(xd, hid0));
-        else    sts3(sx, nown + tid, ld3(xf, hid0));
+        if (PD) stsX(sX, L, nown + tid, to_d(hb) + hx);
+        else    sts3(sx, nown + tid, hf);
     }
     for (int i = tid + kT; i < nhalo; i += kT) {
         const int hg = hids[i];
         if (PD) stsX(sX, L, nown + i, to_d(ld3(a.base, hg)) + ld3d(xd, hg));
         else    sts3(sx, nown + i, ld3(xf, hg));
     }
-    uint2 rt = make_uint2(0, 0);
-    float4 rG = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float rwA = 0.0f;
-    if (a.use_stretch && tid < ntri) {
-        rt = A.tri_idx[tbeg + tid];
-        rG = A.tri_c0[tbeg + tid];
-        rwA = A.tri_c1[tbeg + tid];
-    }
-    uint2 rh = make_uint2(0, 0);
-    float4 rw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float2 rc = make_float2(0.0f, 0.0f);
-    if (kSplit && a.use_bend && tid < nhin) {
-        rh = A.hin_idx[ebeg + tid];
-        rw = A.hin_w[ebeg + tid];
-        rc = A.hin_c[ebeg + tid];
-    }
-    for (int j = tid; j < (glen + 1) >> 1; j += kT) sgw[j] = A.gl_words[gbeg + j];
+    if (tid < nwords) sgw[tid] = gw0;
+    for (int j = tid + kT; j < nwords; j += kT) sgw[j] = A.gl_words[gbeg + j];
     if (lead) {
         if (PD) stsX(sX, L, vl, to_d(ob) + od);
         else    sts3(sx, vl, ob);
