@@ -231,6 +231,44 @@ def warm_start_study(ch, n, rho, args, pos0=None, vel0=None, scene=None, gen_tol
     return out
 
 
+def launch_timing(ch, args):
+    """The launch timing of the roofline alone: on the 256x256, 1024x1024
+    and 512-cloth meshes, the autotuner's candidates and then launch_ms of
+    the chosen shape (steady-phase consecutive iterations), with no frame
+    stepped, and the 256 MiB copy probe; prints one JSON line of the
+    per-launch times. Used to profile the iteration kernel (kernel trace,
+    counters)."""
+    from helpers import CFG2, CFG2_WIND, CFG3, cfg4_specs
+    out = {}
+    cases = [("256x256", dict(CFG2), dict(ext_accel=CFG2_WIND), 200, 500),
+             ("1024x1024", dict(CFG3), {}, 20, 50)]
+    if not args.no_batch:
+        cases.append(("512 cloths", dict(specs=cfg4_specs()), {}, 10, 20))
+    for name, spec, extra, n_tune, n_time in cases:
+        sc = build_scene(spec)
+        a = sc.step_args()
+        dt = spec.get("dt", 1 / 60)
+        kw = dict(mode=ch.MODE_PD, gravity=-9.81, dt=dt, rho=0.99, cheb_delay=1,
+                  max_iterations=20, **extra)
+        offs = dict(cloth_offsets=sc.vert_off) if "specs" in spec else {}
+        sol = ch.Solver(ch.config(**kw), sc.N, sc.T, sc.E, **offs)
+        sol.autotune(a, n_tune)
+        t = launch_ms(sol, a, n_time)
+        pi = sol.plan_info()
+        b = algorithmic_bytes(sc.N, sc.T, sc.E, pi.iters_per_launch)
+        out[name] = {"us_per_launch": t * 1e3, "tile_verts": pi.tile_verts,
+                     "block_threads": pi.block_threads, "element_regions": pi.element_regions,
+                     "achieved_GBs": b / (t * 1e-3) / 1e9}
+        log(f"{name}: {t * 1e3:.2f} us per launch at {pi.tile_verts} / {pi.block_threads}, "
+            f"{pi.element_regions} region(s)")
+        del sol, sc
+    # the copy probe the counter passes are calibrated on (256 MiB read and
+    # written per dispatch)
+    copy_ms = ch.measure_copy(256 << 20, 20)
+    out["copy_probe_GBs"] = 2 * (256 << 20) / (copy_ms * 1e-3) / 1e9
+    print(json.dumps({"launch_timing": out}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +284,9 @@ def main():
     ap.add_argument("--no-1024", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-ws", action="store_true", help="skip the learned warm start")
+    ap.add_argument("--launch-timing", action="store_true",
+                    help="only autotune and time the iteration kernel's launches on the three "
+                         "meshes (no frames): for kernel-trace profiling")
     ap.add_argument("--ws-cloths", type=int, default=64)
     ap.add_argument("--ws-frames", type=int, default=32, help="training frames per cloth")
     ap.add_argument("--ws-steps", type=int, default=4000)
@@ -269,6 +310,9 @@ def main():
 
     info = ch.device_info(0)
     log(f"device: {info.name.decode()} {info.arch.decode()} CUs={info.compute_units}")
+    if args.launch_timing:
+        launch_timing(ch, args)
+        return
 
     # ---- 256x256: frames/s to the residual tolerance ---------------------------
     s = build_scene(CFG2)
