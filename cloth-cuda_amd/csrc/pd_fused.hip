@@ -500,10 +500,11 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
             const uint2 id = rt;
             const float4 G = rG;
             const float wA = rwA;
-            if (j + kT < ntri) {  // next pass's record
-                rt = A.tri_idx[tbeg + j + kT];
-                rG = A.tri_c0[tbeg + j + kT];
-                rwA = A.tri_c1[tbeg + j + kT];
+            {  // next pass's record (index 0 after the last pass; no branch)
+                const size_t jn = j + kT < ntri ? (size_t)tbeg + j + kT : 0;
+                rt = A.tri_idx[jn];
+                rG = A.tri_c0[jn];
+                rwA = A.tri_c1[jn];
             }
             const int l0 = id.x & 0xFFFF, l1 = id.x >> 16, l2 = id.y & 0xFFFF;
             f3 c[3];
@@ -530,12 +531,11 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
         }
     }
     if (!kSplit) {
-        // first hinge record (its latency overlaps the triangle sums)
-        if (a.use_bend && tid < nhin) {
-            rh = A.hin_idx[ebeg + tid];
-            rw = A.hin_w[ebeg + tid];
-            rc = A.hin_c[ebeg + tid];
-        }
+        // first hinge record (its latency overlaps the triangle sums; no branch)
+        const size_t hj = tid < nhin ? (size_t)ebeg + tid : 0;
+        rh = A.hin_idx[hj];
+        rw = A.hin_w[hj];
+        rc = A.hin_c[hj];
         __syncthreads();
     }
 
@@ -561,10 +561,11 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
             const uint2 id = rh;
             const float4 w = rw;
             const float2 c = rc;  // (n_rest, k h^2)
-            if (j + kT < nhin) {
-                rh = A.hin_idx[ebeg + j + kT];
-                rw = A.hin_w[ebeg + j + kT];
-                rc = A.hin_c[ebeg + j + kT];
+            {  // next pass's record (index 0 after the last pass; no branch)
+                const size_t jn = j + kT < nhin ? (size_t)ebeg + j + kT : 0;
+                rh = A.hin_idx[jn];
+                rw = A.hin_w[jn];
+                rc = A.hin_c[jn];
             }
             const int q0 = id.x & 0xFFFF, q1 = id.x >> 16, q2 = id.y & 0xFFFF, q3 = id.y >> 16;
             f3 p, pe;
