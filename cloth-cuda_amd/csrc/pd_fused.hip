@@ -706,30 +706,27 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
     const double* xd = static_cast<const double*>(a.x);
     const double* xm = static_cast<const double*>(a.xkm1);
 
+    // (as in k_fused_iter, each trip's loads are issued without branches
+    // before any of their values is used; lanes with nothing to load read an
+    // in-range element)
+
     // ---- trip 1: descriptors, own vertices, pair halo ids, ring-1 metas -----
     const int* ti = A.tile_info + (size_t)tile * kTileInfo;
     const int* pi = P.pinfo + (size_t)tile * kPairInfo;
     const bool own = tid < nown;
     const int r1 = tid - nown;  // ring-1 index of this thread's vertex
     const size_t g = (size_t)vbeg + (own ? tid : 0);
-    int4 vi = make_int4(0, 0, -1, 0);
-    d3 od = d3{0.0, 0.0, 0.0}, oxp = d3{0.0, 0.0, 0.0};
-    f3 ob = mk(0.0f, 0.0f, 0.0f), oy = mk(0.0f, 0.0f, 0.0f);
-    if (own) {
-        vi = a.vinfo[g];
-        oy = ld3(a.y, g);
-        od = ld3d(xd, g);
-        ob = ld3(a.base, g);
-        if (a.blend || a.ctl) oxp = ld3d(xm, g);
-    }
+    const int4 vo = a.vinfo[g];
+    const f3 oyo = ld3(a.y, g);
+    const d3 odo = ld3d(xd, g);
+    const f3 obo = ld3(a.base, g);
+    d3 oxpo = d3{0.0, 0.0, 0.0};
+    if (a.blend || a.ctl) oxpo = ld3d(xm, g);
     const int* ph = P.phalo + (size_t)tile * P.pstride;
-    const int hid0 = tid < P.pstride ? ph[tid] : 0;
-    int rid = 0;
-    unsigned rmeta = 0u;
-    if (r1 >= 0 && r1 < A.hstride) {
-        rid = ph[r1];
-        rmeta = P.pmeta[(size_t)tile * A.hstride + r1];
-    }
+    const int hid0 = ph[min(tid, P.pstride - 1)];
+    const int r1c = min(max(r1, 0), A.hstride - 1);
+    const int rid = ph[r1c];
+    const unsigned rmeta = P.pmeta[(size_t)tile * A.hstride + r1c];
     const int nh1 = ti[0], tbeg = ti[1], ntri = ti[2], ebeg = ti[3], nhin = ti[4];
     const int gbeg = ti[5], glen = ti[6];
     const int nh2 = pi[0], tbeg2 = pi[1], ntri2 = pi[2], ebeg2 = pi[3], nhin2 = pi[4];
@@ -745,37 +742,55 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
     const bool cv = own || ring1;  // updated in the first iteration
 
     // ---- trip 2: halo positions, ring-1 data, first records, gather lists ---
-    for (int i = tid; i < nh1 + nh2; i += kT) {
-        const int hg = (i == tid) ? hid0 : ph[i];
-        stsX(sX, L, nown + i, to_d(ld3(a.base, hg)) + ld3d(xd, hg));
-    }
-    if (ring1) {
-        vi = a.vinfo[rid];
-        oy = ld3(a.y, rid);
-        od = ld3d(xd, rid);
-        ob = ld3(a.base, rid);
-        if (c0.blend) oxp = ld3d(xm, rid);
-    }
+    const int nhh = nh1 + nh2;
+    const int hsel = tid < nhh ? hid0 : vbeg;
+    const f3 hb = ld3(a.base, hsel);
+    const d3 hx = ld3d(xd, hsel);
+    const size_t rsel = ring1 ? (size_t)rid : g;
+    const int4 vr = a.vinfo[rsel];
+    const f3 oyr = ld3(a.y, rsel);
+    const d3 odr = ld3d(xd, rsel);
+    const f3 obr = ld3(a.base, rsel);
+    d3 oxpr = d3{0.0, 0.0, 0.0};
+    if (c0.blend) oxpr = ld3d(xm, rsel);
     const int ntA = ntri + ntri2, nhA = nhin + nhin2;
     // first records of both iterations: slot tid of the first iteration's
     // list, which for tid < ntri is also slot tid of the second's
-    uint2 rt0 = make_uint2(0, 0);
-    float4 rG0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float rwA0 = 0.0f;
-    if (a.use_stretch && tid < ntA) {
-        if (tid < ntri) {
-            rt0 = A.tri_idx[tbeg + tid];
-            rG0 = A.tri_c0[tbeg + tid];
-            rwA0 = A.tri_c1[tbeg + tid];
-        } else {
-            const int q = tbeg2 + tid - ntri;
-            rt0 = P.ptri_idx[q];
-            rG0 = P.ptri_c0[q];
-            rwA0 = P.ptri_c1[q];
-        }
+    const size_t ta = tid < ntri ? (size_t)tbeg + tid : 0;
+    const size_t tp = (tid >= ntri && tid < ntA) ? (size_t)tbeg2 + tid - ntri : 0;
+    const uint2 rta = A.tri_idx[ta];
+    const float4 rGa = A.tri_c0[ta];
+    const float rwAa = A.tri_c1[ta];
+    const uint2 rtp = P.ptri_idx[tp];
+    const float4 rGp = P.ptri_c0[tp];
+    const float rwAp = P.ptri_c1[tp];
+    const int nw = (glen + 1) >> 1, pnw = (pglen + 1) >> 1;
+    const unsigned gw0 = A.gl_words[tid < nw ? gbeg + tid : 0];
+    const unsigned pgw0 = P.pgl_words[tid < pnw ? pgbeg + tid : 0];
+    // consumers
+    if (tid < nhh) stsX(sX, L, nown + tid, to_d(hb) + hx);
+    for (int i = tid + kT; i < nhh; i += kT) {
+        const int hg = ph[i];
+        stsX(sX, L, nown + i, to_d(ld3(a.base, hg)) + ld3d(xd, hg));
     }
-    for (int j = tid; j < (glen + 1) >> 1; j += kT) sgw[j] = A.gl_words[gbeg + j];
-    for (int j = tid; j < (pglen + 1) >> 1; j += kT) sgw2[j] = P.pgl_words[pgbeg + j];
+    int4 vi = own ? vo : make_int4(0, 0, -1, 0);
+    f3 oy = oyo, ob = obo;
+    d3 od = odo, oxp = oxpo;
+    if (ring1) {
+        vi = vr;
+        oy = oyr;
+        od = odr;
+        ob = obr;
+        oxp = oxpr;
+    }
+    const bool t_own = tid < ntri;
+    const uint2 rt0 = t_own ? rta : rtp;
+    const float4 rG0 = t_own ? rGa : rGp;
+    const float rwA0 = t_own ? rwAa : rwAp;
+    if (tid < nw) sgw[tid] = gw0;
+    for (int j = tid + kT; j < nw; j += kT) sgw[j] = A.gl_words[gbeg + j];
+    if (tid < pnw) sgw2[tid] = pgw0;
+    for (int j = tid + kT; j < pnw; j += kT) sgw2[j] = P.pgl_words[pgbeg + j];
     if (own) stsX(sX, L, tid, to_d(ob) + od);
     __syncthreads();
 
@@ -792,18 +807,16 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             const uint2 id = rt;
             const float4 G = rG;
             const float wA = rwA;
-            const int jn = j + kT;
-            if (jn < ntA) {
-                if (jn < ntri) {
-                    rt = A.tri_idx[tbeg + jn];
-                    rG = A.tri_c0[tbeg + jn];
-                    rwA = A.tri_c1[tbeg + jn];
-                } else {
-                    const int q = tbeg2 + jn - ntri;
-                    rt = P.ptri_idx[q];
-                    rG = P.ptri_c0[q];
-                    rwA = P.ptri_c1[q];
-                }
+            {  // next pass's record from either list (index 0 when unused; no branch)
+                const int jn = j + kT;
+                const size_t qa = jn < ntri ? (size_t)tbeg + jn : 0;
+                const size_t qp = (jn >= ntri && jn < ntA) ? (size_t)tbeg2 + jn - ntri : 0;
+                const uint2 ta_ = A.tri_idx[qa], tp_ = P.ptri_idx[qp];
+                const float4 ga_ = A.tri_c0[qa], gp_ = P.ptri_c0[qp];
+                const float wa_ = A.tri_c1[qa], wp_ = P.ptri_c1[qp];
+                rt = jn < ntri ? ta_ : tp_;
+                rG = jn < ntri ? ga_ : gp_;
+                rwA = jn < ntri ? wa_ : wp_;
             }
             const int l0 = id.x & 0xFFFF, l1 = id.x >> 16, l2 = id.y & 0xFFFF;
             f3 cr[3], c[3];
@@ -814,21 +827,15 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             sts3(o, 2, c[2]);
         }
     }
-    uint2 rh0 = make_uint2(0, 0);
-    float4 rw0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float2 rc0 = make_float2(0.0f, 0.0f);
-    if (a.use_bend && tid < nhA) {
-        if (tid < nhin) {
-            rh0 = A.hin_idx[ebeg + tid];
-            rw0 = A.hin_w[ebeg + tid];
-            rc0 = A.hin_c[ebeg + tid];
-        } else {
-            const int q = ebeg2 + tid - nhin;
-            rh0 = P.phin_idx[q];
-            rw0 = P.phin_w[q];
-            rc0 = P.phin_c[q];
-        }
-    }
+    // first hinge records of both iterations (no branch)
+    const size_t ha = tid < nhin ? (size_t)ebeg + tid : 0;
+    const size_t hp = (tid >= nhin && tid < nhA) ? (size_t)ebeg2 + tid - nhin : 0;
+    const uint2 rha = A.hin_idx[ha], rhp = P.phin_idx[hp];
+    const float4 rwa = A.hin_w[ha], rwp = P.phin_w[hp];
+    const float2 rca = A.hin_c[ha], rcp = P.phin_c[hp];
+    const uint2 rh0 = tid < nhin ? rha : rhp;
+    const float4 rw0 = tid < nhin ? rwa : rwp;
+    const float2 rc0 = tid < nhin ? rca : rcp;
     __syncthreads();
 
     const unsigned meta = own ? (unsigned)vi.w : rmeta;
@@ -846,18 +853,16 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             const uint2 id = rh;
             const float4 w = rw;
             const float2 c = rc;
-            const int jn = j + kT;
-            if (jn < nhA) {
-                if (jn < nhin) {
-                    rh = A.hin_idx[ebeg + jn];
-                    rw = A.hin_w[ebeg + jn];
-                    rc = A.hin_c[ebeg + jn];
-                } else {
-                    const int q = ebeg2 + jn - nhin;
-                    rh = P.phin_idx[q];
-                    rw = P.phin_w[q];
-                    rc = P.phin_c[q];
-                }
+            {  // next pass's record from either list (index 0 when unused; no branch)
+                const int jn = j + kT;
+                const size_t qa = jn < nhin ? (size_t)ebeg + jn : 0;
+                const size_t qp = (jn >= nhin && jn < nhA) ? (size_t)ebeg2 + jn - nhin : 0;
+                const uint2 ha_ = A.hin_idx[qa], hp_ = P.phin_idx[qp];
+                const float4 wa_ = A.hin_w[qa], wp_ = P.phin_w[qp];
+                const float2 ca_ = A.hin_c[qa], cp_ = P.phin_c[qp];
+                rh = jn < nhin ? ha_ : hp_;
+                rw = jn < nhin ? wa_ : wp_;
+                rc = jn < nhin ? ca_ : cp_;
             }
             const int q0 = id.x & 0xFFFF, q1 = id.x >> 16, q2 = id.y & 0xFFFF, q3 = id.y >> 16;
             f3 p, pe;
@@ -899,10 +904,11 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             const uint2 id = rt;
             const float4 G = rG;
             const float wA = rwA;
-            if (j + kT < ntri) {
-                rt = A.tri_idx[tbeg + j + kT];
-                rG = A.tri_c0[tbeg + j + kT];
-                rwA = A.tri_c1[tbeg + j + kT];
+            {  // next pass's record (index 0 after the last pass; no branch)
+                const size_t jn = j + kT < ntri ? (size_t)tbeg + j + kT : 0;
+                rt = A.tri_idx[jn];
+                rG = A.tri_c0[jn];
+                rwA = A.tri_c1[jn];
             }
             const int l0 = id.x & 0xFFFF, l1 = id.x >> 16, l2 = id.y & 0xFFFF;
             f3 cr[3], c[3];
@@ -932,10 +938,11 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             const uint2 id = rh;
             const float4 w = rw;
             const float2 c = rc;
-            if (j + kT < nhin) {
-                rh = A.hin_idx[ebeg + j + kT];
-                rw = A.hin_w[ebeg + j + kT];
-                rc = A.hin_c[ebeg + j + kT];
+            {  // next pass's record (index 0 after the last pass; no branch)
+                const size_t jn = j + kT < nhin ? (size_t)ebeg + j + kT : 0;
+                rh = A.hin_idx[jn];
+                rw = A.hin_w[jn];
+                rc = A.hin_c[jn];
             }
             const int q0 = id.x & 0xFFFF, q1 = id.x >> 16, q2 = id.y & 0xFFFF, q3 = id.y >> 16;
             f3 p, pe;
