@@ -115,7 +115,7 @@ def fp64_error(ch, s, pos0, vel0, common, iterations):
     x_gpu, _ = s.get_state()
     m = oracle_mesh(scene_arrays(s))
     p = po.params(mode=po.PD, iterations=iterations, gravity=-9.81, dt=CFG2["dt"],
-                  rho=common["rho"], cheb_delay=1, ext_accel=CFG2_WIND,
+                  rho=common["rho"], cheb_delay=common["cheb_delay"], ext_accel=CFG2_WIND,
                   threads=min(os.cpu_count() or 1, 16))
     x64, *_ = po.step(m, p, pos0, vel0, "f64")
     return float(np.abs(x_gpu.astype(np.float64) - x64).max() / extent(pos0))
@@ -213,7 +213,7 @@ def warm_start_study(ch, n, rho, args, pos0=None, vel0=None, scene=None, gen_tol
             f"{res[mode]['iterations_per_frame']:.0f} iterations/frame, "
             f"{fl.fallbacks} fallbacks, {fl.paused_frames} paused")
         del full, warm, w1, fl
-    out = {"mesh": f"{n}x{n}", "rho": rho, "training_cloths": args.ws_cloths,
+    out = {"mesh": f"{n}x{n}", "rho": rho, "cheb_delay": 1, "training_cloths": args.ws_cloths,
            "training_frame_tolerance": gen_tol, "training_frame_start": gen_mode,
            "training_frames_per_cloth": args.ws_frames, "cloth_frames_kept": kept,
            "cloth_frames_offered": offered, "training_samples": int(n_samples),
@@ -278,6 +278,9 @@ def main():
     ap.add_argument("--max-iters", type=int, default=80000)
     ap.add_argument("--check-every", type=int, default=50)
     ap.add_argument("--rho", type=float, default=0.999999)
+    ap.add_argument("--cheb-delay", type=int, default=12,
+                    help="plain Jacobi iterations before the Chebyshev blend starts (and after "
+                         "each restart) in the 256x256 frames to the tolerance")
     ap.add_argument("--variant", type=int, default=0, help="0 auto, 1 unfused, 2 fused")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -319,8 +322,8 @@ def main():
     N, T, E = s.N, s.T, s.E
     args_c = s.step_args()
     pos0, vel0 = s.get_state()
-    common = dict(mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho, cheb_delay=1,
-                  ext_accel=CFG2_WIND, kernel_variant=args.variant)
+    common = dict(mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
+                  cheb_delay=args.cheb_delay, ext_accel=CFG2_WIND, kernel_variant=args.variant)
     sol_tol = ch.Solver(ch.config(max_iterations=args.max_iters, tolerance=args.tol,
                                   check_every=args.check_every, **common), N, T, E)
     rep256 = sol_tol.autotune(args_c, 200)
@@ -477,7 +480,7 @@ def main():
         threads = min(os.cpu_count() or 1, 16)
         m = oracle_mesh(scene_arrays(s))
         p = po.params(mode=po.PD, iterations=20, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
-                      cheb_delay=1, ext_accel=CFG2_WIND, threads=threads)
+                      cheb_delay=args.cheb_delay, ext_accel=CFG2_WIND, threads=threads)
         pos, vel = pos0.copy(), vel0.copy()
         t0 = time.time()
         frames = 0
@@ -488,7 +491,7 @@ def main():
         it_per_s = frames * 20 / dt_cpu
         # the same frames on one core, over a third of the time
         p1 = po.params(mode=po.PD, iterations=20, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
-                       cheb_delay=1, ext_accel=CFG2_WIND, threads=1)
+                       cheb_delay=args.cheb_delay, ext_accel=CFG2_WIND, threads=1)
         pos, vel = pos0.copy(), vel0.copy()
         t1 = time.time()
         frames1 = 0
@@ -525,12 +528,14 @@ def main():
         "data": "synthetic (seeded cloths, random-init network trained on solver-generated frames)",
         "config": {"workload": "256x256 PD cloth, top edge pinned, wind 0.5 N/m^2, "
                                "frames to relative residual 1e-5",
-                   "value_mode": "plain start x_n (fastest at 256x256: warm starts are "
-                                 "rejected there, see warm_start.at_256)",
+                   "value_mode": "plain start x_n at cheb_delay below (fastest at 256x256: "
+                                 "warm starts are rejected there; warm_start.at_256 compares "
+                                 "plain and learned starts at cheb_delay 1)",
                    "warm_start": ws,
                    "tolerance": args.tol, "residual": "FP64 ||b - K x|| / ||b|| on the GPU",
                    "max_iterations": args.max_iters, "check_every": args.check_every,
-                   "rho": args.rho, "iterations_per_frame": mean_iters,
+                   "rho": args.rho, "cheb_delay": args.cheb_delay,
+                   "iterations_per_frame": mean_iters,
                    "frames_converged": int(sum(conv)),
                    "final_rel_residual_mean": float(np.mean(resid)),
                    "frames_per_s_fixed_20_iterations": fps20,
