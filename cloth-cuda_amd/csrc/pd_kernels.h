@@ -404,7 +404,11 @@ __device__ __forceinline__ bool ctl_begin(FrameCtl* c, int par, float& omega, in
 // residual, and a Chebyshev restart when the residual grew since the previous
 // check. Every third restart in a frame doubles 1 - rho (a less aggressive
 // sequence, against the oscillation the nonlinear local step can feed; a rho
-// below the Jacobi spectral radius is not repaired by it). The check iteration
+// below the Jacobi spectral radius is not repaired by it), down to rho = 0,
+// where every weight is 1 and the iteration is plain Jacobi (unbounded, rho
+// turns negative once 1 - rho has doubled past 1, after 60 restarts from
+// 0.999999, and the weights 2 / (2 - rho^2) then pass 2 and the iterate
+// diverges to a non-finite residual). The check iteration
 // is slot `par`'s iteration plus kofs (1 when a two-iteration launch checked
 // its second). One thread.
 __device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, double b2, double* hist,
@@ -432,7 +436,7 @@ __device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, doubl
             n.k = k + 1;
             c->slot[par ^ 1] = n;
             c->restarts += 1;
-            if (c->restarts % 3 == 0) c->rho = 1.0f - 2.0f * (1.0f - c->rho);
+            if (c->restarts % 3 == 0) c->rho = fmaxf(0.0f, 1.0f - 2.0f * (1.0f - c->rho));
         }
         c->last_r2 = r2;
     }

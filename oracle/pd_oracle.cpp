@@ -548,10 +548,11 @@ int step(const or_mesh* mp, const or_params* pp, R* pos, R* vel, const R* init, 
             if (p.tolerance > 0 && ((k + 1) % every == 0)) {
                 if (r2 <= tol2 * b2) break;
                 // Chebyshev restart when the residual grew since the last
-                // test; every third restart doubles 1 - rho
+                // test; every third restart doubles 1 - rho, down to rho = 0
+                // (plain Jacobi)
                 if (p.use_chebyshev && last_r2 >= 0.0 && r2 > last_r2) {
                     ck = 0;
-                    if (++restarts % 3 == 0) q.rho = 1.0f - 2.0f * (1.0f - q.rho);
+                    if (++restarts % 3 == 0) q.rho = std::max(0.0f, 1.0f - 2.0f * (1.0f - q.rho));
                 }
                 last_r2 = r2;
             }
