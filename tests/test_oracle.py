@@ -349,3 +349,24 @@ def test_reference_test6_cpu_path_keeps_surface():
     for _ in range(20):
         pos, vel, *_ = po.step(m, p, pos, vel, "f32")
     assert np.ptp(pos[:, 1]) > 0.02 and np.ptp(pos[:, 2]) > 0.02
+
+
+def test_restart_rule_floors_rho():
+    """Every third Chebyshev restart doubles 1 - rho, down to rho = 0 (plain
+    Jacobi). A 12x12 cloth checked every 5 iterations from rho = 0.999999
+    restarts hundreds of times; more than 60 restarts would take rho below 0,
+    where the weights 2 / (2 - rho^2) pass 2 and the iterate diverges (a rule
+    without the floor ends this run non-finite). With the floor it converges
+    to the round-off level."""
+    s = ch.Scene(**dict(CFG1, nrows=12, ncols=12, size=1 / 11))
+    arr = scene_arrays(s)
+    m = oracle_mesh(arr)
+    pos = arr["pos0"].reshape(-1, 3).astype(np.float32)
+    p = po.params(mode=po.PD, iterations=4000, gravity=-9.81, dt=1 / 60, rho=0.999999,
+                  cheb_delay=1, tolerance=1e-12, check_every=5, threads=1)
+    x, v, _, rs, n = po.step(m, p, pos, np.zeros_like(pos), "f32", resid=True)
+    checks = rs[4::5, 0]
+    restarts = int(np.count_nonzero(checks[1:] > checks[:-1]))
+    assert n == 4000 and restarts > 60
+    assert np.isfinite(x).all() and np.isfinite(v).all() and np.isfinite(rs).all()
+    assert np.sqrt(rs[-1, 0] / rs[-1, 1]) < 1e-9
