@@ -21,6 +21,7 @@ CLH_OK = 0
 CLH_H2H, CLH_H2D, CLH_D2H, CLH_D2D = 0, 1, 2, 3
 MODE_REF_CUDA, MODE_PD = 0, 1
 KERNEL_AUTO, KERNEL_UNFUSED, KERNEL_FUSED = 0, 1, 2
+OK, ERR_INVALID, ERR_DEVICE, ERR_ALLOC, ERR_UNSUPPORTED, ERR_NUMERIC = range(6)
 PERSIST_OFF, PERSIST_ON = 0, 1
 PIN_NONE, PIN_TOP_ROW, PIN_CORNERS, PIN_RANDOM_BOUNDARY = 0, 1, 2, 3
 MESH_IRREGULAR = 4
@@ -127,7 +128,7 @@ CLH_SYMBOLS = [
     "clh_ws_dataset_reserve", "clh_ws_dataset_add", "clh_ws_dataset_size", "clh_ws_dataset_clear",
     "clh_ws_dataset_read", "clh_ws_fit_output_scale", "clh_ws_loss_grad", "clh_ws_train",
     "clh_ws_adam_step", "clh_ws_set_smoothing", "clh_solver_start_residual",
-    "clh_ws_dataset_append",
+    "clh_ws_dataset_append", "clh_step_noalloc",
 ]
 
 
@@ -201,6 +202,8 @@ def _load():
     hip.clh_solver_start_residual.argtypes = [_vp, C.POINTER(StepArgs), C.POINTER(C.c_double)]
     hip.clh_get_device_info.argtypes = [C.c_int, C.POINTER(DeviceInfo)]
     hip.clh_device_count.argtypes = [_ip]
+    hip.clh_step_noalloc.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(StepArgs), _vp]
 
     core.cc_scene_spec_default.argtypes = [C.POINTER(SceneSpec)]
     core.cc_scene_spec_default.restype = None

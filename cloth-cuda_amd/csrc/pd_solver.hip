@@ -91,11 +91,12 @@ k_prologue(const float* __restrict__ pos, const float* __restrict__ vel,
 __global__ void __launch_bounds__(kBlock)
 k_pin_scatter(const int* __restrict__ idx, const float* __restrict__ target,
               const float* __restrict__ xprev, int* __restrict__ pin_map,
-              void* __restrict__ x0, int P, int disp)
+              void* __restrict__ x0, int P, int N, int disp)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     const int v = idx[i];
+    if ((unsigned)v >= (unsigned)N) return;  // out of range: the pin is inert
     pin_map[v] = i;
     if (disp) st3d(static_cast<double*>(x0), v, to_d(ld3(target, i)) - to_d(ld3(xprev, v)));
 }
@@ -104,18 +105,27 @@ k_pin_scatter(const int* __restrict__ idx, const float* __restrict__ target,
 // Reference mode stores h^2 wA D^T R (src/pd_solver.cu:256-298, with the
 // rotation of pd_solver.cu:48-208). PD mode stores h^2 wA D^T (R - F) in
 // `fd` (residual) and h^2 wA D^T R in `fr` (right-hand side).
-// PD mode reads positions as base + displacement x.
+// PD mode reads positions as base + displacement x. A triangle with a vertex
+// index outside [0, N) is inert: zero corner forces.
 template <int kMode>
 __global__ void __launch_bounds__(kBlock)
 k_triangles(const float* __restrict__ base, const void* __restrict__ xv,
             const int* __restrict__ tris, const float* __restrict__ dminv,
             const float* __restrict__ area, const float* __restrict__ kst, float* __restrict__ fr,
-            float* __restrict__ fd, int T, float h2, const FrameCtl* __restrict__ ctl)
+            float* __restrict__ fd, int T, int N, float h2, const FrameCtl* __restrict__ ctl)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
     const int i0 = tris[(size_t)t * 3 + 0], i1 = tris[(size_t)t * 3 + 1],
               i2 = tris[(size_t)t * 3 + 2];
+    if ((unsigned)i0 >= (unsigned)N || (unsigned)i1 >= (unsigned)N ||
+        (unsigned)i2 >= (unsigned)N) {
+        for (int j = 0; j < 9; ++j) {
+            fr[(size_t)t * 9 + j] = 0.0f;
+            if (kMode == CLH_MODE_PD) fd[(size_t)t * 9 + j] = 0.0f;
+        }
+        return;
+    }
     const float g00 = dminv[(size_t)t * 4 + 0], g10 = dminv[(size_t)t * 4 + 1];
     const float g01 = dminv[(size_t)t * 4 + 2], g11 = dminv[(size_t)t * 4 + 3];
     const float wA = kst[t] * area[t] * h2;
@@ -149,15 +159,24 @@ k_triangles(const float* __restrict__ base, const void* __restrict__ xv,
 // src/pd_solver.cu:304-335. Stores p, and in PD mode also p - e, with e
 // evaluated from base + displacement as
 // sum_{i>0} w_i ((b_i - b_0) + (x_i - x_0)) + (sum_i w_i)(b_0 + x_0).
+// A hinge with a vertex index outside [0, N) is inert: zero vectors.
 template <int kMode>
 __global__ void __launch_bounds__(kBlock)
 k_hinges(const float* __restrict__ base, const void* __restrict__ xv,
          const int* __restrict__ quads, const float* __restrict__ w,
          const float* __restrict__ nrest, float* __restrict__ bp, float* __restrict__ bd, int E,
-         const FrameCtl* __restrict__ ctl)
+         int N, const FrameCtl* __restrict__ ctl)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
+    bool inert = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) inert |= (unsigned)quads[(size_t)e * 4 + i] >= (unsigned)N;
+    if (inert) {
+        st3(bp, e, mk(0.0f, 0.0f, 0.0f));
+        if (kMode == CLH_MODE_PD) st3(bd, e, mk(0.0f, 0.0f, 0.0f));
+        return;
+    }
     if (kMode == CLH_MODE_PD) {
         const double* x = static_cast<const double*>(xv);
         d3 X[4];
@@ -801,7 +820,7 @@ int enqueue_prologue(clh_solver* s, const clh_step_args& a)
     if (a.num_pinned > 0)
         hipLaunchKernelGGL(k_pin_scatter, dim3(grid_for(a.num_pinned)), dim3(kBlock), 0,
                            s->stream, a.d_pin_idx, a.d_pin_target, s->xprev.ptr, s->pin_map.ptr,
-                           s->it[0].ptr, a.num_pinned, c.mode == CLH_MODE_PD ? 1 : 0);
+                           s->it[0].ptr, a.num_pinned, s->N, c.mode == CLH_MODE_PD ? 1 : 0);
     return hip_status(hipGetLastError(), "prologue launch");
 }
 
@@ -861,21 +880,21 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
         if (pd)
             hipLaunchKernelGGL(k_triangles<CLH_MODE_PD>, dim3(grid_for(s->T)), dim3(kBlock), 0,
                                s->stream, s->xprev.ptr, xk, a.d_tris, a.d_Dm_inv, a.d_rest_area, a.d_stretch_k,
-                               s->fr.ptr, s->fd.ptr, s->T, h2, ctl);
+                               s->fr.ptr, s->fd.ptr, s->T, s->N, h2, ctl);
         else
             hipLaunchKernelGGL(k_triangles<CLH_MODE_REF_CUDA>, dim3(grid_for(s->T)), dim3(kBlock),
                                0, s->stream, s->xprev.ptr, xk, a.d_tris, a.d_Dm_inv, a.d_rest_area,
-                               a.d_stretch_k, s->fr.ptr, s->fd.ptr, s->T, h2, nullptr);
+                               a.d_stretch_k, s->fr.ptr, s->fd.ptr, s->T, s->N, h2, nullptr);
     }
     if (use_bend) {
         if (pd)
             hipLaunchKernelGGL(k_hinges<CLH_MODE_PD>, dim3(grid_for(s->E)), dim3(kBlock), 0,
                                s->stream, s->xprev.ptr, xk, a.d_bend_quads, a.d_bend_w, a.d_bend_n, s->bp.ptr,
-                               s->bd.ptr, s->E, ctl);
+                               s->bd.ptr, s->E, s->N, ctl);
         else
             hipLaunchKernelGGL(k_hinges<CLH_MODE_REF_CUDA>, dim3(grid_for(s->E)), dim3(kBlock), 0,
                                s->stream, s->xprev.ptr, xk, a.d_bend_quads, a.d_bend_w, a.d_bend_n, s->bp.ptr,
-                               s->bd.ptr, s->E, nullptr);
+                               s->bd.ptr, s->E, s->N, nullptr);
     }
     VertexArgs va;
     va.base = s->xprev.ptr;
@@ -1017,10 +1036,10 @@ int enqueue_fixed_frame(clh_solver* s, const clh_step_args& a)
     return CLH_OK;
 }
 
-int validate_args(const clh_solver* s, const clh_step_args* a)
+int validate_step_args(int N, const clh_step_args* a)
 {
     if (!a) return fail(CLH_ERR_INVALID, "clh_solver_step: null args");
-    if (s->N > 0 && (!a->d_pos || !a->d_vel || !a->d_mass || !a->d_jacobi_diag))
+    if (N > 0 && (!a->d_pos || !a->d_vel || !a->d_mass || !a->d_jacobi_diag))
         return fail(CLH_ERR_INVALID, "clh_solver_step: null pos, vel, mass or jacobi_diag");
     if (a->d_stretch_k && (!a->d_tris || !a->d_Dm_inv || !a->d_rest_area))
         return fail(CLH_ERR_INVALID, "clh_solver_step: stretch enabled with null triangle data");
@@ -1030,6 +1049,8 @@ int validate_args(const clh_solver* s, const clh_step_args* a)
         return fail(CLH_ERR_INVALID, "clh_solver_step: bad pin arguments");
     return CLH_OK;
 }
+
+int validate_args(const clh_solver* s, const clh_step_args* a) { return validate_step_args(s->N, a); }
 
 int step_impl(clh_solver* s, const clh_step_args& a)
 {
@@ -1630,4 +1651,311 @@ extern "C" int clh_solver_start_residual(clh_solver* s, const clh_step_args* a, 
     CLH_HIP(hipStreamSynchronize(s->stream));
     *rel = h[1] > 0.0 ? std::sqrt(h[0] / h[1]) : 0.0;
     return CLH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fixed-length frames without allocation
+// ---------------------------------------------------------------------------
+//
+// clh_step_noalloc runs the unfused kernels above (prologue, triangles,
+// hinges, vertex update, epilogue) with every buffer they need carved out of
+// one static device array, so a call allocates nothing. The per-vertex
+// incidence lists that clh_solver_step builds on the host at its first step
+// are built on the device at each call instead: count, scan, place, and a
+// sort of each vertex's list into ascending slot order, which gives the
+// host's lists exactly, and so the same gather order and the same bits.
+//
+// Scratch layout for N vertices, T triangles and E hinges, each region
+// aligned to 256 bytes: y [3N] f32, x_n [3N] f32, three iterates [3N] f64,
+// pin map [N], corner forces [9T] f32 twice, hinge vectors [3E] f32 twice,
+// triangle and hinge incidence (offsets [N+1], slots [3T] and [4E]), and a
+// counter per vertex: about 112 N + 84 T + 40 E bytes.
+
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = CLH_NOALLOC_SCRATCH_BYTES;
+constexpr int kNoAllocMaxDevices = 64;
+constexpr int kNoAllocBufs = 3;  // x_{k-1}, x_k, x_{k+1}
+
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Device address of `scratch` per device, taken once with that device current.
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+int scratch_address(void** out)
+{
+    int dev = 0;
+    CLH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return fail(CLH_ERR_UNSUPPORTED, "clh_step_noalloc: device ordinal beyond the table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        CLH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(CLH_ERR_DEVICE, "clh_step_noalloc: no address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return CLH_OK;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_i32(int* __restrict__ p, int n, int value)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+// count[v] = number of slots naming vertex v (indices outside [0, N) skipped)
+__global__ void __launch_bounds__(kBlock)
+k_count_slots(const int* __restrict__ elems, int nslots, int N, int* __restrict__ count)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const int v = elems[s];
+    if ((unsigned)v < (unsigned)N) atomicAdd(&count[v], 1);
+}
+
+// ptr[0] = 0, ptr[v + 1] = ptr[v] + count[v], in one workgroup: each thread
+// sums a contiguous run, the run sums are scanned in LDS, then each thread
+// writes its run's offsets.
+constexpr int kScanThreads = 1024;
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_counts(const int* __restrict__ count, int N, int* __restrict__ ptr)
+{
+    __shared__ int part[kScanThreads];
+    const int t = threadIdx.x;
+    const int run = (N + kScanThreads - 1) / kScanThreads;
+    const int b = min(N, t * run), e = min(N, b + run);
+    int sum = 0;
+    for (int v = b; v < e; ++v) sum += count[v];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int add = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    int acc = part[t] - sum;  // exclusive prefix of this run
+    if (t == 0) ptr[0] = 0;
+    for (int v = b; v < e; ++v) {
+        acc += count[v];
+        ptr[v + 1] = acc;
+    }
+}
+
+// idx[ptr[v] + j] = slot for the j-th slot of v to arrive (order fixed later)
+__global__ void __launch_bounds__(kBlock)
+k_place_slots(const int* __restrict__ elems, int nslots, int N, const int* __restrict__ ptr,
+              int* __restrict__ cursor, int* __restrict__ idx)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const int v = elems[s];
+    if ((unsigned)v >= (unsigned)N) return;
+    idx[ptr[v] + atomicAdd(&cursor[v], 1)] = s;
+}
+
+// Each vertex's slots into ascending order (insertion sort: lists are short).
+__global__ void __launch_bounds__(kBlock)
+k_sort_lists(const int* __restrict__ ptr, int* __restrict__ idx, int N)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const int b = ptr[v], e = ptr[v + 1];
+    for (int i = b + 1; i < e; ++i) {
+        const int key = idx[i];
+        int j = i - 1;
+        while (j >= b && idx[j] > key) {
+            idx[j + 1] = idx[j];
+            --j;
+        }
+        idx[j + 1] = key;
+    }
+}
+
+struct NoAllocLayout {
+    size_t y, xprev, it[kNoAllocBufs], pin_map, fr, fd, bp, bd, vt_ptr, vt_idx, ve_ptr, ve_idx,
+        cursor, total;
+};
+
+NoAllocLayout noalloc_layout(size_t N, size_t T, size_t E)
+{
+    NoAllocLayout L;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    L.y = take(N * 3 * sizeof(float));
+    L.xprev = take(N * 3 * sizeof(float));
+    for (auto& o : L.it) o = take(N * 3 * sizeof(double));
+    L.pin_map = take(N * sizeof(int));
+    L.fr = take(T * 9 * sizeof(float));
+    L.fd = take(T * 9 * sizeof(float));
+    L.bp = take(E * 3 * sizeof(float));
+    L.bd = take(E * 3 * sizeof(float));
+    L.vt_ptr = take((N + 1) * sizeof(int));
+    L.vt_idx = take(T * 3 * sizeof(int));
+    L.ve_ptr = take((N + 1) * sizeof(int));
+    L.ve_idx = take(E * 4 * sizeof(int));
+    L.cursor = take(N * sizeof(int));
+    L.total = off;
+    return L;
+}
+
+// CSR of vertex -> incident slots (element * arity + corner), ascending, on
+// the device.
+void enqueue_incidence(const int* elems, int nslots, int N, int* ptr, int* idx, int* cursor,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N)), dim3(kBlock), 0, st, cursor, N, 0);
+    if (nslots > 0)
+        hipLaunchKernelGGL(k_count_slots, dim3(grid_for(nslots)), dim3(kBlock), 0, st, elems,
+                           nslots, N, cursor);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, st, cursor, N, ptr);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N)), dim3(kBlock), 0, st, cursor, N, 0);
+    if (nslots > 0)
+        hipLaunchKernelGGL(k_place_slots, dim3(grid_for(nslots)), dim3(kBlock), 0, st, elems,
+                           nslots, N, ptr, cursor, idx);
+    hipLaunchKernelGGL(k_sort_lists, dim3(grid_for(N)), dim3(kBlock), 0, st, ptr, idx, N);
+}
+
+}  // namespace
+
+extern "C" int clh_step_noalloc(const clh_config* cfg, int num_verts, int num_tris, int num_bend,
+                                const clh_step_args* a, void* stream)
+{
+    if (!cfg || !a) return fail(CLH_ERR_INVALID, "clh_step_noalloc: null pointer");
+    if (num_verts < 0 || num_tris < 0 || num_bend < 0)
+        return fail(CLH_ERR_INVALID, "clh_step_noalloc: negative count");
+    if (cfg->mode != CLH_MODE_REF_CUDA && cfg->mode != CLH_MODE_PD)
+        return fail(CLH_ERR_INVALID, "clh_step_noalloc: unknown mode");
+    if (cfg->max_iterations < 0 || cfg->max_iterations > kMaxIterations)
+        return fail(CLH_ERR_INVALID, "clh_step_noalloc: max_iterations out of range");
+    if (!(cfg->dt > 0.0f)) return fail(CLH_ERR_INVALID, "clh_step_noalloc: dt must be > 0");
+    if (cfg->mode == CLH_MODE_PD && cfg->tolerance > 0.0f)
+        return fail(CLH_ERR_UNSUPPORTED,
+                    "clh_step_noalloc: fixed-length frames only (tolerance must be 0)");
+    if (a->d_trace) return fail(CLH_ERR_UNSUPPORTED, "clh_step_noalloc: no iterate trace");
+    const NoAllocLayout L = noalloc_layout((size_t)num_verts, (size_t)num_tris, (size_t)num_bend);
+    if (L.total > kNoAllocScratchBytes)
+        return fail(CLH_ERR_UNSUPPORTED,
+                    "clh_step_noalloc: the mesh needs " + std::to_string(L.total) +
+                        " bytes of scratch, more than the " +
+                        std::to_string(kNoAllocScratchBytes) + " bytes of the static array");
+    // the same argument checks as clh_solver_step
+    CLH_TRY(validate_step_args(num_verts, a));
+    if (num_tris > 0 && a->d_stretch_k && !a->d_tris)
+        return fail(CLH_ERR_INVALID, "clh_step_noalloc: null d_tris with num_tris > 0");
+    if (num_verts == 0) return CLH_OK;
+
+    void* base = nullptr;
+    CLH_TRY(scratch_address(&base));
+    unsigned char* sb = static_cast<unsigned char*>(base);
+    float* y = reinterpret_cast<float*>(sb + L.y);
+    float* xprev = reinterpret_cast<float*>(sb + L.xprev);
+    void* it[kNoAllocBufs];
+    for (int i = 0; i < kNoAllocBufs; ++i) it[i] = sb + L.it[i];
+    int* pin_map = reinterpret_cast<int*>(sb + L.pin_map);
+    float* fr = reinterpret_cast<float*>(sb + L.fr);
+    float* fd = reinterpret_cast<float*>(sb + L.fd);
+    float* bp = reinterpret_cast<float*>(sb + L.bp);
+    float* bd = reinterpret_cast<float*>(sb + L.bd);
+    int* vt_ptr = reinterpret_cast<int*>(sb + L.vt_ptr);
+    int* vt_idx = reinterpret_cast<int*>(sb + L.vt_idx);
+    int* ve_ptr = reinterpret_cast<int*>(sb + L.ve_ptr);
+    int* ve_idx = reinterpret_cast<int*>(sb + L.ve_idx);
+    int* cursor = reinterpret_cast<int*>(sb + L.cursor);
+
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const clh_config& c = *cfg;
+    const int N = num_verts, T = num_tris;
+    const bool use_stretch = T > 0 && a->d_stretch_k != nullptr;
+    const bool use_bend = num_bend > 0 && a->d_bend_quads != nullptr;
+    const int E = use_bend ? num_bend : 0;
+    const int pd = c.mode == CLH_MODE_PD ? 1 : 0;
+    const float h2 = c.dt * c.dt;
+
+    // incidence (an empty list where an element kind is off)
+    enqueue_incidence(a->d_tris, use_stretch ? 3 * T : 0, N, vt_ptr, vt_idx, cursor, st);
+    enqueue_incidence(a->d_bend_quads, 4 * E, N, ve_ptr, ve_idx, cursor, st);
+
+    // prologue: as enqueue_prologue's unfused branch
+    const float acc[3] = {c.ext_accel[0], c.gravity + c.ext_accel[1], c.ext_accel[2]};
+    hipLaunchKernelGGL(k_prologue, dim3(grid_for(N)), dim3(kBlock), 0, st, a->d_pos, a->d_vel,
+                       a->d_init_guess, a->d_cloth_accel, xprev, y, it[0], pin_map, N, c.dt, h2,
+                       acc[0], acc[1], acc[2], h2 * acc[0], h2 * acc[1], h2 * acc[2], pd);
+    if (a->num_pinned > 0)
+        hipLaunchKernelGGL(k_pin_scatter, dim3(grid_for(a->num_pinned)), dim3(kBlock), 0, st,
+                           a->d_pin_idx, a->d_pin_target, xprev, pin_map, it[0], a->num_pinned, N,
+                           pd);
+
+    // iterations: as enqueue_iteration's unfused branch, with the weights of
+    // enqueue_fixed_frame
+    float omega = 1.0f;
+    for (int k = 0; k < c.max_iterations; ++k) {
+        omega = chebyshev_omega(c, k, omega);
+        const bool blend = pd && c.use_chebyshev && k > 0 && omega != 1.0f;
+        const void* xk = it[k % kNoAllocBufs];
+        if (use_stretch) {
+            if (pd)
+                hipLaunchKernelGGL(k_triangles<CLH_MODE_PD>, dim3(grid_for(T)), dim3(kBlock), 0, st,
+                                   xprev, xk, a->d_tris, a->d_Dm_inv, a->d_rest_area,
+                                   a->d_stretch_k, fr, fd, T, N, h2, nullptr);
+            else
+                hipLaunchKernelGGL(k_triangles<CLH_MODE_REF_CUDA>, dim3(grid_for(T)), dim3(kBlock),
+                                   0, st, xprev, xk, a->d_tris, a->d_Dm_inv, a->d_rest_area,
+                                   a->d_stretch_k, fr, fd, T, N, h2, nullptr);
+        }
+        if (use_bend) {
+            if (pd)
+                hipLaunchKernelGGL(k_hinges<CLH_MODE_PD>, dim3(grid_for(E)), dim3(kBlock), 0, st,
+                                   xprev, xk, a->d_bend_quads, a->d_bend_w, a->d_bend_n, bp, bd, E,
+                                   N, nullptr);
+            else
+                hipLaunchKernelGGL(k_hinges<CLH_MODE_REF_CUDA>, dim3(grid_for(E)), dim3(kBlock), 0,
+                                   st, xprev, xk, a->d_bend_quads, a->d_bend_w, a->d_bend_n, bp, bd,
+                                   E, N, nullptr);
+        }
+        VertexArgs va;
+        va.base = xprev;
+        va.x = xk;
+        va.xkm1 = it[(k + kNoAllocBufs - 1) % kNoAllocBufs];
+        va.out = it[(k + 1) % kNoAllocBufs];
+        va.y = y;
+        va.mass = a->d_mass;
+        va.diag = a->d_jacobi_diag;
+        va.pin_map = pin_map;
+        va.pin_target = a->d_pin_target;
+        va.vt_ptr = vt_ptr;
+        va.vt_idx = vt_idx;
+        va.ve_ptr = ve_ptr;
+        va.ve_idx = ve_idx;
+        va.fr = fr;
+        va.fd = fd;
+        va.bp = bp;
+        va.bd = bd;
+        va.bk = a->d_bend_k;
+        va.bw = a->d_bend_w;
+        va.partials = nullptr;
+        va.N = N;
+        va.use_stretch = use_stretch ? 1 : 0;
+        va.use_bend = use_bend ? 1 : 0;
+        va.h2 = h2;
+        va.omega = omega;
+        va.blend = blend ? 1 : 0;
+        va.ctl = nullptr;
+        va.par = k & 1;
+        if (pd)
+            hipLaunchKernelGGL(k_vertex_pd, dim3(grid_for(N)), dim3(kBlock), 0, st, va);
+        else
+            hipLaunchKernelGGL(k_vertex_ref, dim3(grid_for(N)), dim3(kBlock), 0, st, va);
+    }
+    hipLaunchKernelGGL(k_epilogue, dim3(grid_for(N)), dim3(kBlock), 0, st,
+                       it[c.max_iterations % kNoAllocBufs], xprev, pin_map, a->d_pin_target,
+                       a->d_pos, a->d_vel, N, (1.0f - c.damping) / c.dt, pd);
+    return hip_status(hipGetLastError(), "clh_step_noalloc launch");
 }

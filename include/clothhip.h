@@ -418,6 +418,34 @@ int clh_ws_train(clh_ws* w, int steps, int batch, float lr, float* losses);
 /* One Adam step at learning rate lr with a given gradient [param_count] (host). */
 int clh_ws_adam_step(clh_ws* w, const float* grad, float lr);
 
+/* ---- fixed-length frames without allocation --------------------------------
+ * No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory while they step (code under hipGraph stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc). One fixed-length
+ * frame of cfg->max_iterations iterations in either mode, computed by the
+ * same kernels, in the same order, as clh_solver_step with kernel_variant
+ * CLH_KERNEL_UNFUSED, and so with the same bits. Every buffer it needs
+ * beyond the caller's inputs and outputs lies in one static device array of
+ * CLH_NOALLOC_SCRATCH_BYTES that each device the library runs on holds until
+ * the process ends; the call makes no allocation (no hipMalloc, hipFree or
+ * allocating library routine) and enqueues only kernels.
+ *   - Limit: about 112 N + 84 T + 40 E bytes of scratch (each region rounded
+ *     to 256 bytes) must fit; a 256x256 cloth (65,536 vertices, 130,050
+ *     triangles, 194,565 hinges) needs 26 MB. Larger meshes, tolerance > 0
+ *     and d_trace are rejected with CLH_ERR_UNSUPPORTED.
+ *   - Indices are not range-checked on the host (that would need a copy): a
+ *     triangle, hinge or pin that names a vertex outside [0, num_verts) is
+ *     inert instead (no force, no pin), and never addresses memory.
+ *   - Calls on one device share that device's array and leave their contents
+ *     in it: calls that overlap on one device race, so serialize them (one
+ *     stream, or events). Each call clears what it needs cleared.
+ *   - Work is enqueued on `stream` (a hipStream_t; NULL = the null stream,
+ *     after which a blocking copy observes the result) and the call returns
+ *     without waiting. No statistics are recorded. */
+#define CLH_NOALLOC_SCRATCH_BYTES (32u << 20)
+int clh_step_noalloc(const clh_config* cfg, int num_verts, int num_tris, int num_bend,
+                     const clh_step_args* a, void* stream);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

@@ -84,3 +84,26 @@ def test_config_defaults_mirror_reference():
     assert c.damping == 0.0 and c.mode == ch.MODE_REF_CUDA
     p = ch.pd_config()
     assert p.max_iterations == 50 and abs(p.tolerance - 1e-4) < 1e-9 and p.mode == 0
+
+
+def test_noalloc_refuses_without_device():
+    # clh_step_noalloc: sizes beyond its static array, tolerance frames and
+    # traces are refused before any device work
+    lib = ch.hip()
+    args = ch.StepArgs()
+    cfg = ch.config(mode=ch.MODE_PD, max_iterations=5, dt=1 / 60)
+    assert lib.clh_step_noalloc(C.byref(cfg), 2_000_000, 4_000_000, 6_000_000, C.byref(args),
+                                None) == ch.ERR_UNSUPPORTED
+    assert b"bytes of scratch" in lib.clh_last_error()
+    cfg.tolerance = 1e-5
+    assert lib.clh_step_noalloc(C.byref(cfg), 100, 162, 0, C.byref(args), None) == \
+        ch.ERR_UNSUPPORTED
+    cfg.tolerance = 0.0
+    args.d_trace = C.c_void_p(16)
+    assert lib.clh_step_noalloc(C.byref(cfg), 100, 162, 0, C.byref(args), None) == \
+        ch.ERR_UNSUPPORTED
+    assert lib.clh_step_noalloc(None, 100, 162, 0, C.byref(args), None) == ch.ERR_INVALID
+    assert lib.clh_step_noalloc(C.byref(cfg), -1, 162, 0, C.byref(args), None) == ch.ERR_INVALID
+    # a fixed frame of a fitting mesh with null state pointers: invalid, no launch
+    args.d_trace = None
+    assert lib.clh_step_noalloc(C.byref(cfg), 100, 162, 0, C.byref(args), None) == ch.ERR_INVALID
