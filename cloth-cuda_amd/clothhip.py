@@ -249,6 +249,18 @@ def _load():
     core.cc_scene_energy.restype = None
     core.cc_scene_apply_pins.argtypes = [_vp]
     core.cc_scene_apply_pins.restype = None
+    core.cc_scene_get_cpu_state.argtypes = [_vp, _vp, _vp]
+    core.cc_scene_get_cpu_state.restype = C.c_int
+    core.cc_scene_set_cpu_state.argtypes = [_vp, _vp, _vp]
+    core.cc_scene_set_cpu_state.restype = C.c_int
+    core.cc_cpuref_create.argtypes = [_vp, C.c_float, C.c_float, C.c_int]
+    core.cc_cpuref_create.restype = _vp
+    core.cc_cpuref_step.argtypes = [_vp, _vp, C.c_int]
+    core.cc_cpuref_step.restype = None
+    core.cc_cpuref_factorizations.argtypes = [_vp]
+    core.cc_cpuref_factorizations.restype = C.c_int
+    core.cc_cpuref_destroy.argtypes = [_vp]
+    core.cc_cpuref_destroy.restype = None
     _hip, _core = hip, core
     return hip, core
 
@@ -419,6 +431,20 @@ class Scene:
         core().cc_scene_energy(self.h, gravity, _ptr(pos), _ptr(vel), _ptr(out))
         return out
 
+    def get_cpu_state(self):
+        """The host state pos_cpu, vel_cpu of the CPU reference backend."""
+        pos = np.empty((self.N, 3), np.float32)
+        vel = np.empty((self.N, 3), np.float32)
+        if core().cc_scene_get_cpu_state(self.h, _ptr(pos), _ptr(vel)) != 0:
+            raise ClothHipError("cc_scene_get_cpu_state failed")
+        return pos, vel
+
+    def set_cpu_state(self, pos=None, vel=None):
+        pos = None if pos is None else np.ascontiguousarray(pos, dtype=np.float32)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32)
+        core().cc_scene_set_cpu_state(self.h, None if pos is None else _ptr(pos),
+                                      None if vel is None else _ptr(vel))
+
     def step_args(self, stretch=True, bend=True) -> StepArgs:
         a = StepArgs()
         for f in STEP_FIELDS:
@@ -533,6 +559,32 @@ class Solver:
     def close(self):
         if self.h:
             hip().clh_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CpuRefSolver:
+    """CpuStretchReferenceSolver (include/stretch_reference.h) on a scene: the
+    CPU reference backend, stepping the scene's host state pos_cpu / vel_cpu."""
+
+    def __init__(self, scene: "Scene", gravity=-9.81, damping=0.0, refactor=False):
+        self.scene = scene
+        self.h = core().cc_cpuref_create(scene.h, gravity, damping, 1 if refactor else 0)
+
+    def step(self, steps=1):
+        core().cc_cpuref_step(self.h, self.scene.h, steps)
+
+    def factorizations(self) -> int:
+        return core().cc_cpuref_factorizations(self.h)
+
+    def close(self):
+        if self.h:
+            core().cc_cpuref_destroy(self.h)
             self.h = None
 
     def __del__(self):

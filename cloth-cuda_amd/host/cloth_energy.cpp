@@ -60,10 +60,23 @@ void download_state(const ClothMesh& mesh, std::vector<float>& pos, std::vector<
     CLH_CHECK(clh_memcpy(vel.data(), mesh.d_vel, n * sizeof(float), CLH_D2H));
 }
 
-float reference_energy(const ClothMesh& mesh, const PDSolverConfig& cfg)
+void host_state(const ClothMesh& mesh, std::vector<float>& pos, std::vector<float>& vel)
+{
+    const size_t n = (size_t)mesh.num_verts;
+    pos.resize(n * 3);
+    vel.resize(n * 3);
+    for (size_t i = 0; i < n && i < mesh.pos_cpu.size() && i < mesh.vel_cpu.size(); ++i)
+        for (int k = 0; k < 3; ++k) {
+            pos[i * 3 + k] = mesh.pos_cpu[i](k);
+            vel[i * 3 + k] = mesh.vel_cpu[i](k);
+        }
+}
+
+float reference_energy(const ClothMesh& mesh, const PDSolverConfig& cfg, bool use_cpu_ref)
 {
     std::vector<float> pos, vel;
-    download_state(mesh, pos, vel);
+    if (use_cpu_ref) host_state(mesh, pos, vel);
+    else download_state(mesh, pos, vel);
     float kinetic = 0.0f, potential = 0.0f;
     for (int i = 0; i < mesh.num_verts; ++i) {
         const float m = mesh.mass[(size_t)i];

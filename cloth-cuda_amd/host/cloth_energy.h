@@ -23,8 +23,9 @@ struct ClothEnergy {
 };
 
 // The reference's compute_energy: kinetic + potential accumulated in FP32
-// over the device state (src/tools/sim_cloth.cpp:64-105).
-float reference_energy(const ClothMesh& mesh, const PDSolverConfig& cfg);
+// over the device state, or over the host state pos_cpu / vel_cpu of the CPU
+// reference backend (src/tools/sim_cloth.cpp:64-105).
+float reference_energy(const ClothMesh& mesh, const PDSolverConfig& cfg, bool use_cpu_ref = false);
 
 // All four terms from host arrays pos, vel [N*3].
 ClothEnergy cloth_energy(const ClothMesh& mesh, const SimConstraints& cons, float gravity,
@@ -32,3 +33,5 @@ ClothEnergy cloth_energy(const ClothMesh& mesh, const SimConstraints& cons, floa
 
 // Copies the device state of mesh into pos, vel.
 void download_state(const ClothMesh& mesh, std::vector<float>& pos, std::vector<float>& vel);
+// Copies the host state pos_cpu, vel_cpu of mesh into pos, vel.
+void host_state(const ClothMesh& mesh, std::vector<float>& pos, std::vector<float>& vel);

@@ -101,6 +101,19 @@ void  cc_scene_energy(const cc_scene* s, float gravity, const float* pos, const 
 /* Constraints::apply_gpu on the scene's device state. */
 void  cc_scene_apply_pins(cc_scene* s);
 
+/* The host state of the CPU reference backend, mesh.pos_cpu / vel_cpu
+ * (host arrays [N*3]; a null pointer skips that array). */
+int   cc_scene_get_cpu_state(const cc_scene* s, float* pos, float* vel);
+int   cc_scene_set_cpu_state(cc_scene* s, const float* pos, const float* vel);
+/* CpuStretchReferenceSolver (include/stretch_reference.h) on the scene's mesh,
+ * stretch constraints and pins, with the spec's dt; refactor != 0 re-factors the global matrix
+ * every iteration, as the reference does. step runs `steps` steps of the host
+ * state. */
+void* cc_cpuref_create(cc_scene* s, float gravity, float damping, int refactor);
+void  cc_cpuref_step(void* solver, cc_scene* s, int steps);
+int   cc_cpuref_factorizations(void* solver);
+void  cc_cpuref_destroy(void* solver);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@
 #include "mesh_topology.h"
 #include "pd_solver.h"
 #include "sim_constraints.h"
+#include "stretch_reference.h"
 
 #include <algorithm>
 #include <cstring>
@@ -444,3 +445,48 @@ extern "C" void cc_scene_energy(const cc_scene* s, float gravity, const float* p
 }
 
 extern "C" void cc_scene_apply_pins(cc_scene* s) { s->pins.apply_gpu(s->mesh.d_pos, s->mesh.d_vel); }
+
+extern "C" int cc_scene_get_cpu_state(const cc_scene* s, float* pos, float* vel)
+{
+    const ClothMesh& m = s->mesh;
+    if ((int)m.pos_cpu.size() != m.num_verts || (int)m.vel_cpu.size() != m.num_verts) return 1;
+    for (int i = 0; i < m.num_verts; ++i)
+        for (int k = 0; k < 3; ++k) {
+            if (pos) pos[(size_t)i * 3 + k] = m.pos_cpu[(size_t)i](k);
+            if (vel) vel[(size_t)i * 3 + k] = m.vel_cpu[(size_t)i](k);
+        }
+    return 0;
+}
+
+extern "C" int cc_scene_set_cpu_state(cc_scene* s, const float* pos, const float* vel)
+{
+    ClothMesh& m = s->mesh;
+    m.pos_cpu.resize((size_t)m.num_verts);
+    m.vel_cpu.resize((size_t)m.num_verts);
+    for (int i = 0; i < m.num_verts; ++i)
+        for (int k = 0; k < 3; ++k) {
+            if (pos) m.pos_cpu[(size_t)i](k) = pos[(size_t)i * 3 + k];
+            if (vel) m.vel_cpu[(size_t)i](k) = vel[(size_t)i * 3 + k];
+        }
+    return 0;
+}
+
+extern "C" void* cc_cpuref_create(cc_scene* s, float gravity, float damping, int refactor)
+{
+    auto* r = new CpuStretchReferenceSolver(s->mesh, s->sim, s->pins, s->spec.dt, gravity, damping);
+    r->refactor_every_iteration = refactor != 0;
+    return r;
+}
+
+extern "C" void cc_cpuref_step(void* solver, cc_scene* s, int steps)
+{
+    auto* r = static_cast<CpuStretchReferenceSolver*>(solver);
+    for (int i = 0; i < steps; ++i) r->step(s->mesh, s->pins);
+}
+
+extern "C" int cc_cpuref_factorizations(void* solver)
+{
+    return static_cast<CpuStretchReferenceSolver*>(solver)->factorizations();
+}
+
+extern "C" void cc_cpuref_destroy(void* solver) { delete static_cast<CpuStretchReferenceSolver*>(solver); }

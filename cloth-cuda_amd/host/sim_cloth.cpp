@@ -10,6 +10,7 @@
 #include "mesh_topology.h"
 #include "pd_solver.h"
 #include "sim_constraints.h"
+#include "stretch_reference.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -19,8 +20,9 @@
 
 namespace {
 
-// ASCII PLY of the device positions (format of src/tools/sim_cloth.cpp:20-61).
-void export_ply(const ClothMesh& mesh, const char* filename)
+// ASCII PLY of the device positions, or of the host positions of the CPU
+// reference backend (format of src/tools/sim_cloth.cpp:20-61).
+void export_ply(const ClothMesh& mesh, const char* filename, bool use_cpu_ref)
 {
     FILE* fp = std::fopen(filename, "w");
     if (!fp) {
@@ -28,7 +30,8 @@ void export_ply(const ClothMesh& mesh, const char* filename)
         return;
     }
     std::vector<float> pos, vel;
-    download_state(mesh, pos, vel);
+    if (use_cpu_ref) host_state(mesh, pos, vel);
+    else download_state(mesh, pos, vel);
     std::fprintf(fp, "ply\nformat ascii 1.0\n");
     std::fprintf(fp, "element vertex %d\n", mesh.num_verts);
     std::fprintf(fp, "property float x\nproperty float y\nproperty float z\n");
@@ -54,7 +57,7 @@ void print_usage(const char* prog)
     std::fprintf(stderr, "  --type <n>       Mesh type 0-3           (default: 3)\n");
     std::fprintf(stderr, "  --gravity <g>    Gravitational accel     (default: -9.8)\n");
     std::fprintf(stderr, "  --damping <d>    Velocity damping 0-1    (default: 0)\n");
-    std::fprintf(stderr, "  --stretch-backend <b>  gpu-current          (default: gpu-current)\n");
+    std::fprintf(stderr, "  --stretch-backend <b>  gpu-current | cpu-ref (default: gpu-current)\n");
     std::fprintf(stderr, "  --export <dir>   Export PLY frames\n");
     std::fprintf(stderr, "  --verbose        Per-frame energy output\n");
     std::fprintf(stderr, "Appended:\n");
@@ -114,11 +117,12 @@ int main(int argc, char* argv[])
         else if (a == "--obj" && more) obj = argv[++i];
         else if (a == "--energy") energy = true;
     }
-    if (stretch_backend != "gpu-current") {
-        std::fprintf(stderr, "stretch backend '%s' is not available: this build's solver is the "
-                             "HIP path only\n", stretch_backend.c_str());
+    if (stretch_backend != "gpu-current" && stretch_backend != "cpu-ref") {
+        std::fprintf(stderr, "unknown stretch backend '%s' (gpu-current | cpu-ref)\n",
+                     stretch_backend.c_str());
         return 1;
     }
+    const bool use_cpu_ref = stretch_backend == "cpu-ref";
     if (mode != "ref" && mode != "pd") {
         std::fprintf(stderr, "unknown mode '%s' (ref | pd)\n", mode.c_str());
         return 1;
@@ -151,10 +155,14 @@ int main(int argc, char* argv[])
     else if (pin_mode == "corners") pin_cons.pin_corners(mesh, pin_cols);
     std::printf("Pinned vertices: %d\n", (int)pin_cons.pinned_indices.size());
 
-    mesh.upload_to_gpu();
-    pin_cons.upload_to_gpu();
-    sim_cons.upload_to_gpu();
-    sim_cons.precompute_jacobi_diag(mesh, dt);
+    // the CPU reference backend steps the host state only, so it needs no
+    // device (the reference uploads the mesh and pins regardless)
+    if (!use_cpu_ref) {
+        mesh.upload_to_gpu();
+        pin_cons.upload_to_gpu();
+        sim_cons.upload_to_gpu();
+        sim_cons.precompute_jacobi_diag(mesh, dt);
+    }
 
     PDSolverConfig config;
     config.dt = dt;
@@ -168,52 +176,62 @@ int main(int argc, char* argv[])
     config.mode = mode == "pd" ? 1 : 0;
     config.tolerance = tol;
     config.cheb_delay = 1;
-    PDSolver solver(config, mesh, sim_cons);
+    PDSolver* gpu_solver = nullptr;
+    CpuStretchReferenceSolver* cpu_ref_solver = nullptr;
+    if (use_cpu_ref)
+        cpu_ref_solver = new CpuStretchReferenceSolver(mesh, sim_cons, pin_cons, dt, gravity, damping);
+    else
+        gpu_solver = new PDSolver(config, mesh, sim_cons);
 
     auto report = [&](const char* head) {
-        std::printf("%s%.6f\n", head, reference_energy(mesh, config));
+        std::printf("%s%.6f\n", head, reference_energy(mesh, config, use_cpu_ref));
     };
     auto report_terms = [&](int step) {
         std::vector<float> pos, vel;
-        download_state(mesh, pos, vel);
+        if (use_cpu_ref) host_state(mesh, pos, vel);
+        else download_state(mesh, pos, vel);
         const ClothEnergy e = cloth_energy(mesh, sim_cons, gravity, pos, vel);
         std::printf("Step %d: kinetic %.6e potential %.6e stretch %.6e bend %.6e total %.6e", step,
                     e.kinetic, e.potential, e.stretch, e.bend, e.total());
-        if (config.mode == 1)
-            std::printf(" | %d iterations, residual %.3e", solver.last_iterations(),
-                        solver.last_relative_residual());
+        if (gpu_solver && config.mode == 1)
+            std::printf(" | %d iterations, residual %.3e", gpu_solver->last_iterations(),
+                        gpu_solver->last_relative_residual());
         std::printf("\n");
     };
 
     if (!export_dir.empty()) {
         char fn[512];
         std::snprintf(fn, sizeof(fn), "%s/frame_0000.ply", export_dir.c_str());
-        export_ply(mesh, fn);
+        export_ply(mesh, fn, use_cpu_ref);
     }
     std::printf("\nSimulating...\n");
     report("Initial energy: ");
     const int export_interval = std::max(1, steps / 10);
     for (int step = 1; step <= steps; ++step) {
-        solver.step(mesh, sim_cons, pin_cons);
+        if (use_cpu_ref) cpu_ref_solver->step(mesh, pin_cons);
+        else gpu_solver->step(mesh, sim_cons, pin_cons);
         if (verbose && step % 10 == 0) {
             if (energy) report_terms(step);
-            else std::printf("Step %d: Energy = %.6f\n", step, reference_energy(mesh, config));
+            else std::printf("Step %d: Energy = %.6f\n", step,
+                             reference_energy(mesh, config, use_cpu_ref));
         }
         if (!export_dir.empty() && step % export_interval == 0) {
             char fn[512];
             std::snprintf(fn, sizeof(fn), "%s/frame_%04d.ply", export_dir.c_str(),
                           step / export_interval);
-            export_ply(mesh, fn);
+            export_ply(mesh, fn, use_cpu_ref);
         }
     }
     report("Final energy: ");
     if (!export_dir.empty()) {
         char fn[512];
         std::snprintf(fn, sizeof(fn), "%s/frame_final.ply", export_dir.c_str());
-        export_ply(mesh, fn);
+        export_ply(mesh, fn, use_cpu_ref);
         std::printf("\nExported frames to %s/\n", export_dir.c_str());
     }
-    pin_cons.free_gpu();
+    delete gpu_solver;
+    delete cpu_ref_solver;
+    if (!use_cpu_ref) pin_cons.free_gpu();
     std::printf("\nSimulation complete.\n");
     return 0;
 }

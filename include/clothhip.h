@@ -181,9 +181,11 @@ typedef struct clh_step_args {
                                     ext_accel (C = 1 for a solver of one cloth) */
 } clh_step_args;
 
-/* Replaces PDSolver::step (src/pd_solver.cu:496-621). Work is enqueued on the
- * solver's stream (the null stream unless clh_solver_set_stream was called),
- * so a following blocking copy observes the result, as in the reference. */
+/* Replaces PDSolver::step (src/pd_solver.cu:496-621). Work is enqueued on a
+ * non-blocking stream the solver creates, ordered after earlier null-stream
+ * work, and later null-stream work waits for it (events both ways), so a
+ * following blocking copy observes the result, as in the reference. After
+ * clh_solver_set_stream the work goes to the caller's stream, unfenced. */
 int clh_solver_step(clh_solver* s, const clh_step_args* a);
 
 typedef struct clh_step_stats {
@@ -209,7 +211,8 @@ int clh_solver_get_stats(clh_solver* s, clh_step_stats* out);
  * number available. */
 int clh_solver_residual_history(clh_solver* s, double* out, int max_count,
                                 int* count);
-/* Sets the stream (a hipStream_t) that step and run enqueue on. */
+/* Sets the stream (a hipStream_t) that step and run enqueue on; NULL returns
+ * to the solver's own stream with the null-stream fences of clh_solver_step. */
 int clh_solver_set_stream(clh_solver* s, void* stream);
 
 /* ---- measurement ---------------------------------------------------------- */

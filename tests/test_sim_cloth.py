@@ -22,8 +22,8 @@ def run(*args):
 def test_usage_and_unavailable_backend():
     r = run()
     assert r.returncode == 1 and "Usage:" in r.stderr and "--stretch-backend" in r.stderr
-    r = run(4, 4, 0.1, "--stretch-backend", "cpu-ref")
-    assert r.returncode == 1 and "not available" in r.stderr
+    r = run(4, 4, 0.1, "--stretch-backend", "cpu-fast")
+    assert r.returncode == 1 and "unknown stretch backend" in r.stderr
     r = run(4, 4, 0.1, "--mode", "fast")
     assert r.returncode == 1
 
