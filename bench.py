@@ -121,6 +121,117 @@ def fp64_error(ch, s, pos0, vel0, common, iterations):
     return float(np.abs(x_gpu.astype(np.float64) - x64).max() / extent(pos0))
 
 
+def host_cpu():
+    """nproc, the OpenMP thread count the environment grants, and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    return {"nproc": os.cpu_count(), "omp_threads": threads, "model": model}
+
+
+def tolerance_frame_error(ch, s, sol, args_c, pos_w, vel_w, args):
+    """The error of one benchmark frame (to the tolerance, from the state the
+    timed frames start from) against the oracle: the same algorithm and
+    settings in FP32 (its stopping iteration) and in FP64 (its positions), and
+    the FP64 oracle run on to a relative residual of 1e-8 (the frame's PD
+    fixed point, three orders of magnitude closer than the tolerance), all as
+    max |x_gpu - x_oracle| / extent."""
+    import numpy as np
+    import pyoracle as po
+    from helpers import CFG2, CFG2_WIND, extent, oracle_mesh, scene_arrays
+    s.set_state(pos_w, vel_w)
+    sol.step(args_c)
+    st = sol.stats()
+    x_gpu, _ = s.get_state()
+    m = oracle_mesh(scene_arrays(s))
+    threads = host_cpu()["omp_threads"]
+    kw = dict(mode=po.PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho, ext_accel=CFG2_WIND,
+              check_every=args.check_every, threads=threads)
+    e = extent(pos_w)
+    out = {"gpu_iterations": int(st.iterations)}
+    for name, prec, tol, delay, cap in (("fp32", "f32", args.tol, args.cheb_delay, args.max_iters),
+                                         ("fp64", "f64", args.tol, args.cheb_delay, args.max_iters),
+                                         ("fp64_converged", "f64", 1e-8, args.cheb_delay,
+                                          40000)):
+        p = po.params(iterations=cap, tolerance=tol, cheb_delay=delay, **kw)
+        t0 = time.time()
+        x, _, _, _, it = po.step(m, p, pos_w, vel_w, prec)
+        out[name] = {"error": float(np.abs(x_gpu - x).max() / e), "iterations": int(it),
+                     "tolerance": tol, "seconds": time.time() - t0}
+    return out
+
+
+def cpu_baseline(ch, s, pos_w, vel_w, args):
+    """The oracle (C++, FP32 storage and FP64 iterates, as the GPU) timed on
+    the host: frames of the benchmark's workload to the tolerance at the
+    benchmark's settings, from the state the GPU's timed frames start from,
+    for about args.cpu_seconds on the OpenMP threads the environment grants;
+    fixed-length iterations on one thread; and the CPU reference backend
+    (CpuStretchReferenceSolver) on the 32x32 workload, factored once and
+    refactored every iteration as the reference does."""
+    import pyoracle as po
+    from helpers import CFG1, CFG2, CFG2_WIND, oracle_mesh, scene_arrays
+    host = host_cpu()
+    threads = host["omp_threads"]
+    m = oracle_mesh(scene_arrays(s))
+    kw = dict(mode=po.PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho, cheb_delay=args.cheb_delay,
+              ext_accel=CFG2_WIND)
+    p = po.params(iterations=args.max_iters, tolerance=args.tol, check_every=args.check_every,
+                  threads=threads, **kw)
+    pos, vel = pos_w.copy(), vel_w.copy()
+    t0 = time.time()
+    frames, its = 0, 0
+    while time.time() - t0 < args.cpu_seconds:
+        pos, vel, _, _, n = po.step(m, p, pos, vel, "f32")
+        frames += 1
+        its += n
+    dt_cpu = time.time() - t0
+    log(f"cpu baseline: {frames} frames to {args.tol} in {dt_cpu:.1f} s on {threads} threads "
+        f"({its / frames:.0f} iterations/frame)")
+    # one thread: iterations/s of fixed 20-iteration frames over a third of the time
+    p1 = po.params(iterations=20, threads=1, **kw)
+    pos, vel = pos_w.copy(), vel_w.copy()
+    t1 = time.time()
+    frames1 = 0
+    while time.time() - t1 < args.cpu_seconds / 3:
+        pos, vel, *_ = po.step(m, p1, pos, vel, "f32")
+        frames1 += 1
+    dt1 = time.time() - t1
+    it1 = frames1 * 20 / dt1
+    # the CPU reference backend on the 32x32 workload, 50 iterations a frame
+    c1 = ch.Scene(**CFG1)
+    ref_cpu = {"workload": "32x32 workload, CpuStretchReferenceSolver (stretch only, 50 "
+                           "iterations of exact solves per frame)"}
+    for refactor, frames_c in ((False, 200), (True, 20)):
+        c1.set_cpu_state(c1.get("pos0"))
+        r = ch.CpuRefSolver(c1, gravity=-9.81, refactor=refactor)
+        t2 = time.time()
+        r.step(frames_c)
+        key = "refactor_every_iteration" if refactor else "factor_once"
+        ref_cpu[key] = {"frames_per_s": frames_c / (time.time() - t2), "frames": frames_c,
+                        "factorizations": r.factorizations()}
+        r.close()
+    log(f"cpu reference backend at 32x32: {ref_cpu['factor_once']['frames_per_s']:.1f} frames/s "
+        f"factored once, {ref_cpu['refactor_every_iteration']['frames_per_s']:.1f} refactored")
+    return {"value": frames / dt_cpu, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} frames of the 256x256 workload to relative residual {args.tol} "
+                      f"at the benchmark's settings (rho {args.rho}, delay {args.cheb_delay}, "
+                      f"check every {args.check_every}), from the state the GPU's timed frames "
+                      f"start from: C++ oracle, FP32 storage and FP64 iterates, OpenMP "
+                      f"{threads} threads, {dt_cpu:.1f} s, {its / frames:.0f} iterations/frame",
+            "host": host,
+            "single_core": {"iterations_per_s": it1, "cores": 1,
+                            "sample": f"{frames1} frames x 20 iterations in {dt1:.1f} s"},
+            "ref_cpu_32x32": ref_cpu}
+
+
 def warm_start_study(ch, n, rho, args, pos0=None, vel0=None, scene=None, gen_tol=None,
                      gen_mode=None):
     """Trains the warm start on frames of args.ws_cloths batched n x n cloths
@@ -282,6 +393,9 @@ def main():
                     help="plain Jacobi iterations before the Chebyshev blend starts (and after "
                          "each restart) in the 256x256 frames to the tolerance")
     ap.add_argument("--variant", type=int, default=0, help="0 auto, 1 unfused, 2 fused")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="enqueue the 256x256 frames' launches directly instead of replaying "
+                         "captured hipGraphs (the kernels and their bits are the same)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-1024", action="store_true")
@@ -323,7 +437,8 @@ def main():
     args_c = s.step_args()
     pos0, vel0 = s.get_state()
     common = dict(mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
-                  cheb_delay=args.cheb_delay, ext_accel=CFG2_WIND, kernel_variant=args.variant)
+                  cheb_delay=args.cheb_delay, ext_accel=CFG2_WIND, kernel_variant=args.variant,
+                  use_graph=0 if args.no_graph else 1)
     sol_tol = ch.Solver(ch.config(max_iterations=args.max_iters, tolerance=args.tol,
                                   check_every=args.check_every, **common), N, T, E)
     rep256 = sol_tol.autotune(args_c, 200)
@@ -333,6 +448,7 @@ def main():
     log(f"256x256: N={N} T={T} E={E}; warmup {args.warmup} frames")
     for _ in range(args.warmup):
         sol_tol.step(args_c)
+    pos_w, vel_w = s.get_state()  # the state the timed frames start from
     iters, resid, conv = [], [], []
     ms_total = 0.0
     for f in range(args.steps):
@@ -387,6 +503,10 @@ def main():
                                            f"{c['us_per_launch']:.2f} us" for c in copy_small))
     err64 = fp64_error(ch, s, pos0, vel0, common, 20)
     log(f"FP64 oracle error after one 20-iteration frame: {err64:.2e} of extent")
+    tol_err = None
+    if not args.no_cpu:
+        tol_err = tolerance_frame_error(ch, s, sol_tol, args_c, pos_w, vel_w, args)
+        log("frame to the tolerance vs FP64 oracle: " + json.dumps(tol_err))
 
     roof_1024 = None
     if not args.no_1024:
@@ -472,45 +592,10 @@ def main():
             "frame missing the tolerance is redone from (x_n, v_n) with the plain start, and "
             "3 misses in a row pause the warm start for 20 frames")
 
-    # ---- CPU baseline: the oracle's FP32 PD on the host cores --------------------
+    # ---- CPU baseline: the oracle's FP32 PD frames on the host cores -------------
     cpu = None
     if not args.no_cpu:
-        import pyoracle as po
-        from helpers import oracle_mesh, scene_arrays
-        threads = min(os.cpu_count() or 1, 16)
-        m = oracle_mesh(scene_arrays(s))
-        p = po.params(mode=po.PD, iterations=20, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
-                      cheb_delay=args.cheb_delay, ext_accel=CFG2_WIND, threads=threads)
-        pos, vel = pos0.copy(), vel0.copy()
-        t0 = time.time()
-        frames = 0
-        while time.time() - t0 < args.cpu_seconds:
-            pos, vel, *_ = po.step(m, p, pos, vel, "f32")
-            frames += 1
-        dt_cpu = time.time() - t0
-        it_per_s = frames * 20 / dt_cpu
-        # the same frames on one core, over a third of the time
-        p1 = po.params(mode=po.PD, iterations=20, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
-                       cheb_delay=args.cheb_delay, ext_accel=CFG2_WIND, threads=1)
-        pos, vel = pos0.copy(), vel0.copy()
-        t1 = time.time()
-        frames1 = 0
-        while time.time() - t1 < args.cpu_seconds / 3:
-            pos, vel, *_ = po.step(m, p1, pos, vel, "f32")
-            frames1 += 1
-        dt1 = time.time() - t1
-        it1 = frames1 * 20 / dt1
-        cpu = {"value": it_per_s / mean_iters, "unit": "frames/s", "cores": threads,
-               "kind": "port",
-               "sample": f"{frames} frames x 20 iterations of the 256x256 PD frame (C++ oracle, "
-                         f"FP32, OpenMP {threads} threads) in {dt_cpu:.1f} s = "
-                         f"{it_per_s:.1f} iterations/s, divided by the GPU's "
-                         f"{mean_iters:.0f} iterations/frame to the same tolerance",
-               "single_core": {"value": it1 / mean_iters, "unit": "frames/s", "cores": 1,
-                               "sample": f"{frames1} frames x 20 iterations in {dt1:.1f} s = "
-                                         f"{it1:.1f} iterations/s"}}
-        log(f"cpu baseline: {it_per_s:.1f} iterations/s on {threads} threads, "
-            f"{it1:.1f} on 1 thread")
+        cpu = cpu_baseline(ch, s, pos_w, vel_w, args)
 
     value = fps_tol
     line = {
@@ -533,6 +618,7 @@ def main():
                                  "plain and learned starts at cheb_delay 1)",
                    "warm_start": ws,
                    "tolerance": args.tol, "residual": "FP64 ||b - K x|| / ||b|| on the GPU",
+                   "graphs": not args.no_graph,
                    "max_iterations": args.max_iters, "check_every": args.check_every,
                    "rho": args.rho, "cheb_delay": args.cheb_delay,
                    "iterations_per_frame": mean_iters,
@@ -540,6 +626,7 @@ def main():
                    "final_rel_residual_mean": float(np.mean(resid)),
                    "frames_per_s_fixed_20_iterations": fps20,
                    "fp64_oracle_error_20_iterations": err64,
+                   "fp64_oracle_error_to_tolerance": tol_err,
                    "kernel_variant": int(plan.variant), "iters_per_launch": int(ipl),
                    "element_regions": int(plan.element_regions),
                    "tile_verts": int(plan.tile_verts),

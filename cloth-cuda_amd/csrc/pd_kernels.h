@@ -405,10 +405,11 @@ __device__ __forceinline__ bool ctl_begin(FrameCtl* c, int par, float& omega, in
 // check. Every third restart in a frame doubles 1 - rho (a less aggressive
 // sequence, against the oscillation the nonlinear local step can feed; a rho
 // below the Jacobi spectral radius is not repaired by it), down to rho = 0,
-// where every weight is 1 and the iteration is plain Jacobi (unbounded, rho
-// turns negative once 1 - rho has doubled past 1, after 60 restarts from
-// 0.999999, and the weights 2 / (2 - rho^2) then pass 2 and the iterate
-// diverges to a non-finite residual). The check iteration
+// where every weight is 1 and the iteration is plain Jacobi. Without the
+// floor, rho from 0.999999 turns negative at the 20th doubling (60 restarts);
+// the weights depend on rho^2, so they pass 2 only once rho < -1, at the
+// 21st doubling (63 restarts), after which the iterate diverges to a
+// non-finite residual. The check iteration
 // is slot `par`'s iteration plus kofs (1 when a two-iteration launch checked
 // its second). One thread.
 __device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, double b2, double* hist,

@@ -93,6 +93,13 @@ double PDSolver::last_relative_residual() const
     return st.rhs_norm > 0.0 ? st.residual / st.rhs_norm : 0.0;
 }
 
+bool PDSolver::last_converged() const
+{
+    clh_step_stats st;
+    CLH_CHECK(clh_solver_get_stats(solver_, &st));
+    return st.converged != 0;
+}
+
 void launch_apply_constraints_kernel(float* d_pos, float* d_vel, const int* d_indices,
                                      const float* d_target, int num_pinned)
 {

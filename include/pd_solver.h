@@ -55,9 +55,13 @@ public:
     void reset();
 
     // ---- appended ----
-    // Iterations used and FP64 residuals of the last frame (PD mode).
+    // Iterations used and FP64 residuals of the last frame (PD mode). A frame
+    // run to a tolerance that reaches max_iterations first is still
+    // completed (velocity update, pins) from its last iterate: callers that
+    // need the tolerance check last_converged() after each frame.
     int    last_iterations() const;
     double last_relative_residual() const;
+    bool   last_converged() const;
     clh_solver* native_handle() const { return solver_; }
 
 private:

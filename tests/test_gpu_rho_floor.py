@@ -2,8 +2,12 @@
 rho = 0 (plain Jacobi). The 256x256 workload from rest with a plain-Jacobi
 delay of 8 iterations restarts at nearly every check in its third frame (the
 C++ oracle counts 252 restarts in 20,000 iterations there); without the floor
-rho turned negative, the weights passed 2 and the frame ended in a non-finite
-residual. With the floor the frame stays finite, whether or not it converges."""
+rho from 0.999999 passes -1 at the 63rd restart, the weights 2 / (2 - rho^2)
+pass 2 and the frame ends in a non-finite residual. With the floor the frame
+stays finite, whether or not it converges. A restart is a check whose
+residual grew since the previous one, so the residual history counts them:
+the third frame must restart more than 63 times, past the point where rho
+would have fallen below -1."""
 import numpy as np
 import pytest
 
@@ -36,3 +40,7 @@ def test_repeated_restarts_stay_finite():
             # the first two frames converge (the oracle: 7,150 iterations
             # with 15 restarts, then 1,400 with none)
             assert st.converged, f"frame {frame}"
+        else:
+            growth = int(np.sum(hist[1:] > hist[:-1]))
+            print(f"frame 3: {st.iterations} iterations, {growth} restarts", flush=True)
+            assert growth > 63

@@ -1,0 +1,160 @@
+"""The workloads at their own sizes and settings, against the oracle:
+
+- the 512-cloth batch (sides 16-256, per-cloth material, 1-4 random pins),
+  2 frames x 15 iterations: the largest, the smallest and 4 random cloths
+  against the FP64 oracle of that cloth alone, every cloth finite, pins
+  exact;
+- the 256x256 workload for 200 frames x 20 iterations against the FP64
+  oracle: within 1e-4 of the extent;
+- the benchmark's frames to a tolerance (rho 0.999999, 12 plain Jacobi
+  iterations before the Chebyshev blend, residual checked every 50, relative
+  1e-5), 3 frames from the start: the same stopping iterations as the FP32
+  oracle within one check, positions within 1e-5 of the extent;
+- a learned-warm-start frame on the 64-cloth 256x256 batch with random wind
+  up to 2 N/m^2: each cloth meets the tolerance, from the warm start or from
+  the plain fallback frame."""
+import os
+
+import numpy as np
+import pytest
+
+import clothhip as ch
+import pyoracle as po
+from helpers import CFG2, CFG2_WIND, cfg4_specs, cfg5_specs, extent, oracle_mesh, scene_arrays
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(os.cpu_count() or 1, 16)
+
+
+def build(spec):
+    s = ch.Scene(**spec)
+    s.upload()
+    return s
+
+
+def test_cfg4_full_batch_against_fp64():
+    specs = cfg4_specs()
+    print(f"512-cloth batch: building {len(specs)} cloths", flush=True)
+    b = build(dict(specs=specs))
+    assert b.C == 512
+    args = b.step_args()
+    pos0, vel0 = b.get_state()
+    sol = ch.Solver(ch.config(max_iterations=15, mode=ch.MODE_PD, gravity=-9.81,
+                              dt=specs[0]["dt"], rho=0.99, cheb_delay=1), b.N, b.T, b.E,
+                    cloth_offsets=b.vert_off)
+    for _ in range(2):
+        sol.step(args)
+    pos, vel = b.get_state()
+    assert np.all(np.isfinite(pos)) and np.all(np.isfinite(vel))
+    pins = b.get("pin_idx")
+    np.testing.assert_array_equal(pos[pins], b.get("pin_target"))
+    sizes = np.diff(b.vert_off)
+    rng = np.random.default_rng(44)
+    pick = {int(np.argmax(sizes)), int(np.argmin(sizes))}
+    while len(pick) < 6:
+        pick.add(int(rng.integers(0, b.C)))
+    p = po.params(mode=po.PD, iterations=15, gravity=-9.81, dt=specs[0]["dt"], rho=0.99,
+                  cheb_delay=1, threads=THREADS)
+    for c in sorted(pick):
+        v0, v1 = b.vert_off[c], b.vert_off[c + 1]
+        one = ch.Scene(**specs[c])
+        arr = scene_arrays(one)
+        np.testing.assert_array_equal(arr["pos0"], pos0[v0:v1])
+        x, v = pos0[v0:v1], vel0[v0:v1]
+        for _ in range(2):
+            x, v, *_ = po.step(oracle_mesh(arr), p, x, v, "f64")
+        err = np.abs(pos[v0:v1] - x).max() / extent(arr["rest_pos"])
+        print(f"  cloth {c} ({int(sizes[c])} vertices): {err:.2e} of extent", flush=True)
+        assert err < 1e-5, (c, err)
+
+
+def test_cfg2_200_frames_against_fp64():
+    s = build(CFG2)
+    args = s.step_args()
+    wind = ch.DeviceArray.from_host(np.float32([CFG2_WIND]))
+    args.d_cloth_accel = wind.ptr
+    pos0, vel0 = s.get_state()
+    sol = ch.Solver(ch.config(max_iterations=20, mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"],
+                              rho=0.99, cheb_delay=1), s.N, s.T, s.E)
+    for _ in range(200):
+        sol.step(args)
+    pos, _ = s.get_state()
+    print("256x256: 200 GPU frames done, FP64 oracle frames follow", flush=True)
+    arr = scene_arrays(s)
+    p = po.params(mode=po.PD, iterations=20, gravity=-9.81, dt=CFG2["dt"], rho=0.99,
+                  cheb_delay=1, ext_accel=CFG2_WIND, threads=THREADS)
+    m = oracle_mesh(arr)
+    x, v = pos0.astype(np.float64), vel0.astype(np.float64)
+    for f in range(200):
+        x, v, *_ = po.step(m, p, x, v, "f64")
+        if f % 50 == 49:
+            print(f"  oracle frame {f + 1}", flush=True)
+    e = extent(arr["rest_pos"])
+    err = np.abs(pos - x).max() / e
+    print(f"256x256 after 200 frames: {err:.2e} of extent", flush=True)
+    assert np.all(np.isfinite(pos))
+    assert err < 1e-4
+    # the cloth fell and swung in the wind: the comparison is not of a rest state
+    assert np.abs(pos - pos0).max() > 0.05 * e
+
+
+def test_benchmark_frames_against_fp32_oracle():
+    s = build(CFG2)
+    args = s.step_args()
+    pos0, vel0 = s.get_state()
+    cfg = dict(mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=0.999999, cheb_delay=12,
+               ext_accel=CFG2_WIND, max_iterations=80000, tolerance=1e-5, check_every=50)
+    sol = ch.Solver(ch.config(**cfg), s.N, s.T, s.E)
+    gpu = []
+    for _ in range(3):
+        sol.step(args)
+        st = sol.stats()
+        gpu.append((s.get_state()[0].copy(), st.iterations, st.converged))
+    arr = scene_arrays(s)
+    p = po.params(mode=po.PD, iterations=80000, gravity=-9.81, dt=CFG2["dt"], rho=0.999999,
+                  cheb_delay=12, ext_accel=CFG2_WIND, tolerance=1e-5, check_every=50,
+                  threads=THREADS)
+    m = oracle_mesh(arr)
+    x, v = pos0, vel0
+    e = extent(arr["rest_pos"])
+    for f, (xg, it_g, conv) in enumerate(gpu):
+        x, v, _, _, it_o = po.step(m, p, x, v, "f32")
+        err = np.abs(xg - x).max() / e
+        print(f"  frame {f}: GPU {it_g} iterations, oracle {it_o}, {err:.2e} of extent",
+              flush=True)
+        assert conv == 1
+        assert abs(it_g - it_o) <= 50, (f, it_g, it_o)
+        assert err < 1e-5, (f, err)
+
+
+def test_cfg5_learned_frame_meets_tolerance_or_falls_back():
+    from warmstart import FrameLoop
+    specs, acc = cfg5_specs(64)
+    print("64 x 256x256 batch with random wind", flush=True)
+    b = build(dict(specs=specs))
+    assert b.C == 64 and b.N == 64 * 256 * 256
+    kw = dict(mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=0.999999, cheb_delay=1,
+              tolerance=1e-5, check_every=50)
+    full = ch.Solver(ch.config(max_iterations=20000, **kw), b.N, b.T, b.E,
+                     cloth_offsets=b.vert_off)
+    warm = ch.Solver(ch.config(max_iterations=3000, **kw), b.N, b.T, b.E,
+                     cloth_offsets=b.vert_off)
+    ws = ch.WarmStart.for_scene(b, CFG2["dt"], -9.81, seed=3)
+    # the initial network has W3 = 0 (the extrapolation); give its three
+    # output rows random weights so the network's correction enters the start
+    params, _ = ws.params()
+    w3 = 64 * 16 + 64 + 64 * 64 + 64
+    rng = np.random.default_rng(5)
+    params[w3:w3 + 3 * 64] = 0.05 * rng.normal(size=3 * 64).astype(np.float32)
+    ws.set_params(params, 1.0)
+    loop = FrameLoop(b, warm, ws, acc, fallback=full)
+    loop.run(2, mode="plain")
+    its = loop.step(mode="learned")
+    used = full if loop.fallbacks else warm
+    r = used.cloth_residuals()
+    pos, vel = b.get_state()
+    print(f"  learned frame: {its} iterations, {loop.fallbacks} fallback(s), "
+          f"max cloth residual {r.max():.2e}", flush=True)
+    assert np.all(np.isfinite(pos)) and np.all(np.isfinite(vel))
+    assert len(r) == 64 and np.all(r <= 1e-5)
