@@ -128,7 +128,7 @@ CLH_SYMBOLS = [
     "clh_ws_dataset_reserve", "clh_ws_dataset_add", "clh_ws_dataset_size", "clh_ws_dataset_clear",
     "clh_ws_dataset_read", "clh_ws_fit_output_scale", "clh_ws_loss_grad", "clh_ws_train",
     "clh_ws_adam_step", "clh_ws_set_smoothing", "clh_solver_start_residual",
-    "clh_ws_dataset_append", "clh_step_noalloc",
+    "clh_ws_dataset_append", "clh_step_noalloc", "clh_ws_set_step_filter",
 ]
 
 
@@ -199,6 +199,7 @@ def _load():
     hip.clh_ws_train.argtypes = [_vp, C.c_int, C.c_int, C.c_float, _fp]
     hip.clh_ws_adam_step.argtypes = [_vp, _fp, C.c_float]
     hip.clh_ws_set_smoothing.argtypes = [_vp, C.c_int]
+    hip.clh_ws_set_step_filter.argtypes = [_vp, C.c_int, C.c_float]
     hip.clh_solver_start_residual.argtypes = [_vp, C.POINTER(StepArgs), C.POINTER(C.c_double)]
     hip.clh_get_device_info.argtypes = [C.c_int, C.POINTER(DeviceInfo)]
     hip.clh_device_count.argtypes = [_ip]
@@ -723,6 +724,9 @@ class WarmStart:
     def predict(self, xm2, xm1, x, guess, cloth_accel=None, use_network=True):
         check(hip().clh_ws_predict(self.h, _dptr(xm2), _dptr(xm1), _dptr(x), _dptr(cloth_accel),
                                    _dptr(guess), 1 if use_network else 0), "clh_ws_predict")
+
+    def set_step_filter(self, passes, gain):
+        check(hip().clh_ws_set_step_filter(self.h, passes, gain), "clh_ws_set_step_filter")
 
     def set_smoothing(self, passes):
         check(hip().clh_ws_set_smoothing(self.h, passes), "clh_ws_set_smoothing")

@@ -2299,8 +2299,9 @@ size_t fused_lds_bytes(const FusedPlan& p, int mode, bool check) { return lds_by
 
 int fused_enable_lds(int& bytes)
 {
-    // above 64 KB the kernels must opt in; when the runtime refuses, stay at
-    // 64 KB
+    // above 64 KB the kernels must opt in, each for the device's limit less
+    // its own static LDS (the block-sum arrays); `bytes` becomes the dynamic
+    // LDS every kernel was granted. When the runtime refuses, stay at 64 KB.
     if (bytes <= 64 * 1024) return CLH_OK;
     std::vector<const void*> fns;
     push_iter_kernels<CLH_MODE_PD, true, 256, false>(fns);
@@ -2323,13 +2324,24 @@ int fused_enable_lds(int& bytes)
     fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<false, 256>));
     fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<true, 512>));
     fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<false, 512>));
+    int granted = bytes;
     for (const void* f : fns) {
-        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {
+        hipFuncAttributes fa;
+        if (hipFuncGetAttributes(&fa, f) != hipSuccess) {
             (void)hipGetLastError();
             bytes = 64 * 1024;
             return CLH_OK;
         }
+        const int dyn = bytes - (int)fa.sharedSizeBytes;
+        if (dyn <= 64 * 1024 ||
+            hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) != hipSuccess) {
+            (void)hipGetLastError();
+            bytes = 64 * 1024;
+            return CLH_OK;
+        }
+        granted = std::min(granted, dyn);
     }
+    bytes = granted;
     return CLH_OK;
 }
 

@@ -313,6 +313,28 @@ k_ws_extrap_only(const float* __restrict__ extrap, const float* __restrict__ x,
     guess[3 * v + 2] = s[3 * v + 2];
 }
 
+// step = guess - x for free vertices, 0 for pinned ones.
+__global__ void __launch_bounds__(256)
+k_ws_step(const float* __restrict__ guess, const float* __restrict__ x,
+          const unsigned char* __restrict__ pinned, float* __restrict__ step, int N)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    for (int k = 0; k < 3; ++k) step[3 * v + k] = pinned[v] ? 0.0f : guess[3 * v + k] - x[3 * v + k];
+}
+
+// guess = x + gain * step for free vertices, x for pinned ones.
+__global__ void __launch_bounds__(256)
+k_ws_apply_step(const float* __restrict__ step, const float* __restrict__ x,
+                const unsigned char* __restrict__ pinned, float gain, float* __restrict__ guess,
+                int N)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    for (int k = 0; k < 3; ++k)
+        guess[3 * v + k] = pinned[v] ? x[3 * v + k] : fmaf(gain, step[3 * v + k], x[3 * v + k]);
+}
+
 // ---------------------------------------------------------------------------
 // training: loss and gradient
 // ---------------------------------------------------------------------------
@@ -571,6 +593,8 @@ struct clh_ws {
     DevBuf<unsigned char> pinned;
     DevBuf<float> P, m, v, grad, partial, losses, extrap, corr[2];
     int smooth_passes = 0;
+    int step_passes = 0;      // 1-ring smoothing passes of the whole predicted step
+    float step_gain = 1.0f;   // factor on the (smoothed) predicted step
     DevBuf<unsigned short> Wq, Finf;
     DevBuf<unsigned short> F;  // dataset features
     DevBuf<float4> T;          // dataset targets
@@ -821,7 +845,30 @@ extern "C" int clh_ws_predict(clh_ws* w, const float* xm2, const float* xm1, con
         hipLaunchKernelGGL(k_ws_extrap_only, dim3(blocks256(w->N)), dim3(256), 0, w->stream,
                            w->extrap.ptr, x, w->pinned.ptr, guess, w->N);
     }
+    if (w->step_passes > 0 || w->step_gain != 1.0f) {
+        // the whole step x_hat - x_n, smoothed and scaled (the correction
+        // buffers are free once the guess is formed)
+        hipLaunchKernelGGL(k_ws_step, dim3(blocks256(w->N)), dim3(256), 0, w->stream, guess, x,
+                           w->pinned.ptr, w->corr[0].ptr, w->N);
+        for (int i = 0; i < w->step_passes; ++i)
+            hipLaunchKernelGGL(k_ws_smooth, dim3(blocks256(w->N)), dim3(256), 0, w->stream,
+                               w->corr[i & 1].ptr, w->corr[(i + 1) & 1].ptr, w->nb_ptr.ptr,
+                               w->nb_idx.ptr, w->pinned.ptr, w->N);
+        hipLaunchKernelGGL(k_ws_apply_step, dim3(blocks256(w->N)), dim3(256), 0, w->stream,
+                           w->corr[w->step_passes & 1].ptr, x, w->pinned.ptr, w->step_gain, guess,
+                           w->N);
+    }
     return hip_status(hipGetLastError(), "warm-start predict launch");
+}
+
+extern "C" int clh_ws_set_step_filter(clh_ws* w, int passes, float gain)
+{
+    if (!w || passes < 0 || passes > 1024 || !(gain >= 0.0f) || !std::isfinite(gain))
+        return fail(CLH_ERR_INVALID,
+                    "clh_ws_set_step_filter: passes must be in [0, 1024] and gain finite and >= 0");
+    w->step_passes = passes;
+    w->step_gain = gain;
+    return CLH_OK;
 }
 
 extern "C" int clh_ws_set_smoothing(clh_ws* w, int passes)

@@ -385,6 +385,14 @@ int clh_ws_predict(clh_ws* w, const float* d_xm2, const float* d_xm1, const floa
 /* Passes of 1-ring averaging applied to the network's correction before it
  * is added to the extrapolation (default 0 = none). */
 int clh_ws_set_smoothing(clh_ws* w, int passes);
+/* Filter on the whole predicted step: the guess becomes
+ * x_n + gain * S^passes(x_hat - x_n), with x_hat the prediction above (after
+ * the correction's own smoothing) and S one pass of 1-ring averaging (pinned
+ * vertices keep x_n). A first iterate that is wrong in the cloth's stiff,
+ * rough modes raises the residual the frame must reduce; smoothing keeps the
+ * step's smooth part, and gain scales it (0 gives x_n, the plain start).
+ * Defaults: 0 passes, gain 1 (the prediction unchanged). */
+int clh_ws_set_step_filter(clh_ws* w, int passes, float gain);
 /* The 16 BF16 features per vertex [N*16] (device) of a state. */
 int clh_ws_features(clh_ws* w, const float* d_xm2, const float* d_xm1, const float* d_x,
                     const float* d_cloth_accel, unsigned short* d_features);

@@ -16,8 +16,11 @@
   extent from the FP64 solution converged to 3e-8 (measured on the host);
   the bound is twice the spread of the two oracles;
 - a learned-warm-start frame on the 64-cloth 256x256 batch with random wind
-  up to 2 N/m^2: each cloth meets the tolerance, from the warm start or from
-  the plain fallback frame."""
+  up to 2 N/m^2: either every cloth meets the tolerance from the warm start,
+  or the frame falls back and its result is the plain frame's, bit for bit.
+  (Under the strongest winds the plain frame itself can stall above the
+  tolerance within its cap: 5e-4 after 20,000 iterations was measured on
+  this batch's third frame.)"""
 import os
 
 import numpy as np
@@ -157,6 +160,7 @@ def test_cfg5_learned_frame_meets_tolerance_or_falls_back():
     ws.set_params(params, 1.0)
     loop = FrameLoop(b, warm, ws, acc, fallback=full)
     loop.run(2, mode="plain")
+    x_n, v_n = b.get_state()
     its = loop.step(mode="learned")
     used = full if loop.fallbacks else warm
     r = used.cloth_residuals()
@@ -164,4 +168,15 @@ def test_cfg5_learned_frame_meets_tolerance_or_falls_back():
     print(f"  learned frame: {its} iterations, {loop.fallbacks} fallback(s), "
           f"max cloth residual {r.max():.2e}", flush=True)
     assert np.all(np.isfinite(pos)) and np.all(np.isfinite(vel))
-    assert len(r) == 64 and np.all(r <= 1e-5)
+    assert len(r) == 64
+    if loop.fallbacks == 0:
+        assert warm.stats().converged == 1 and np.all(r <= 1e-5)
+    else:
+        # the fallback frame is the plain frame from the same state
+        b.set_state(x_n, v_n)
+        plain = FrameLoop(b, full, None, acc)
+        plain.step(mode="plain")
+        pp, pv = b.get_state()
+        np.testing.assert_array_equal(pos, pp)
+        np.testing.assert_array_equal(vel, pv)
+        np.testing.assert_array_equal(r, full.cloth_residuals())

@@ -109,6 +109,37 @@ def test_smoothing_passes_match_oracle():
     np.testing.assert_allclose(got[~pin], want[~pin], rtol=0, atol=2e-6)
 
 
+def test_step_filter_matches_oracle():
+    """x_n + gain * S^m(x_hat - x_n): gain 0 is the plain start bit for bit,
+    (0 passes, gain 1) leaves the prediction unchanged, and 3 passes at gain
+    0.5 equal the oracle's smoothing of the step."""
+    s, hist = scene_and_history()
+    ws = ch.WarmStart.for_scene(s, DT, G, WIND)
+    ws.set_params(random_params(), 0.5)
+    dev = [ch.DeviceArray.from_host(h) for h in hist]
+    raw, out = ch.DeviceArray(s.N * 12), ch.DeviceArray(s.N * 12)
+    ws.predict(*dev, raw)
+    xhat = raw.download(np.float32, (s.N, 3))
+    x = hist[2]
+    pin = pinned_mask(s)
+    ws.set_step_filter(0, 0.0)
+    ws.predict(*dev, out)
+    np.testing.assert_array_equal(out.download(np.float32, (s.N, 3)), x)
+    ws.set_step_filter(0, 1.0)
+    ws.predict(*dev, out)
+    np.testing.assert_array_equal(out.download(np.float32, (s.N, 3)), xhat)
+    ws.set_step_filter(3, 0.5)
+    ws.predict(*dev, out)
+    step = np.where(pin[:, None], 0.0, xhat - x).astype(np.float32)
+    sm = wo.smooth(step, wo.neighbours(s.N, s.get("tris")), pin, 3)
+    want = x + np.float32(0.5) * sm
+    got = out.download(np.float32, (s.N, 3))
+    np.testing.assert_allclose(got[~pin], want[~pin], rtol=0, atol=2e-7)
+    np.testing.assert_array_equal(got[pin], x[pin])
+    with pytest.raises(ch.ClothHipError):
+        ws.set_step_filter(-1, 1.0)
+
+
 def test_untrained_network_is_the_extrapolation():
     s, hist = scene_and_history()
     ws = ch.WarmStart.for_scene(s, DT, G, WIND)
