@@ -22,6 +22,7 @@ CLH_H2H, CLH_H2D, CLH_D2H, CLH_D2D = 0, 1, 2, 3
 MODE_REF_CUDA, MODE_PD = 0, 1
 KERNEL_AUTO, KERNEL_UNFUSED, KERNEL_FUSED = 0, 1, 2
 OK, ERR_INVALID, ERR_DEVICE, ERR_ALLOC, ERR_UNSUPPORTED, ERR_NUMERIC = range(6)
+CLOTH_RUNNING, CLOTH_CONVERGED, CLOTH_STALLED, CLOTH_NONFINITE = range(4)
 PERSIST_OFF, PERSIST_ON = 0, 1
 PIN_NONE, PIN_TOP_ROW, PIN_CORNERS, PIN_RANDOM_BOUNDARY = 0, 1, 2, 3
 MESH_IRREGULAR = 4
@@ -44,7 +45,8 @@ class Config(C.Structure):
                 ("cheb_delay", C.c_int), ("ext_accel", C.c_float * 3),
                 ("kernel_variant", C.c_int), ("use_graph", C.c_int), ("tile_verts", C.c_int),
                 ("block_threads", C.c_int), ("persistent", C.c_int),
-                ("iters_per_launch", C.c_int), ("element_regions", C.c_int)]
+                ("iters_per_launch", C.c_int), ("element_regions", C.c_int),
+                ("stall_checks", C.c_int)]
 
 
 class MeshDesc(C.Structure):
@@ -129,6 +131,7 @@ CLH_SYMBOLS = [
     "clh_ws_dataset_read", "clh_ws_fit_output_scale", "clh_ws_loss_grad", "clh_ws_train",
     "clh_ws_adam_step", "clh_ws_set_smoothing", "clh_solver_start_residual",
     "clh_ws_dataset_append", "clh_step_noalloc", "clh_ws_set_step_filter",
+    "clh_batch_cloth_status",
 ]
 
 
@@ -178,6 +181,7 @@ def _load():
     hip.clh_batch_create.argtypes = [C.POINTER(Config), C.POINTER(MeshDesc), C.c_int, _vp,
                                      C.POINTER(_vp)]
     hip.clh_batch_cloth_residuals.argtypes = [_vp, C.POINTER(C.c_double), C.c_int, _ip]
+    hip.clh_batch_cloth_status.argtypes = [_vp, _vp, _vp, C.c_int, _ip]
     hip.clh_solver_autotune.argtypes = [_vp, C.POINTER(StepArgs), C.c_int, C.POINTER(TuneReport)]
     _dp = C.POINTER(C.c_double)
     hip.clh_ws_create.argtypes = [C.POINTER(WsDesc), C.POINTER(_vp)]
@@ -515,6 +519,16 @@ class Solver:
         check(hip().clh_batch_cloth_residuals(self.h, out.ctypes.data_as(C.POINTER(C.c_double)),
                                               max_count, C.byref(n)), "cloth_residuals")
         return out[:min(n.value, max_count)]
+
+    def cloth_status(self, max_count=1 << 16):
+        """(state CLH_CLOTH_* per cloth, iterations per cloth) of the last frame."""
+        n = C.c_int()
+        check(hip().clh_batch_cloth_status(self.h, None, None, 0, C.byref(n)), "status")
+        st = np.zeros(n.value, np.int32)
+        it = np.zeros(n.value, np.int32)
+        check(hip().clh_batch_cloth_status(self.h, _ptr(st), _ptr(it), n.value, C.byref(n)),
+              "clh_batch_cloth_status")
+        return st, it
 
     def autotune(self, args: StepArgs, launches: int = 20) -> TuneReport:
         rep = TuneReport()

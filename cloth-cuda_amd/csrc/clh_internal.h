@@ -53,10 +53,34 @@ struct FrameCtl {
     int    barrier_error;   // a grid barrier of the persistent kernel timed out
     double last_r2;         // residual^2 at the previous check (-1: none)
     double tol2;            // tolerance^2
+    int    stall_checks;    // > 0: stop, not converged, after this many checks in a row
+                            // without the relative residual falling below 0.9 of its best
+    int    since_best;      // checks since the relative residual last set a best
+    double best_rel2;       // best (r / b)^2 of the frame so far
+    int    stalled;         // the frame stopped on the stall rule
     // host-visible copies of this block (pinned, coherent): after check c
     // the reduction writes the block to mirror[c & 1], so the host reads the
     // control state of a chunk without a copy on the stream (null: no copies)
     FrameCtl* mirror;
+};
+
+// Per-cloth control of a batch run to a tolerance: each cloth has its own
+// Chebyshev sequence, restarts, rho, convergence and stall test, so cloths do
+// not steer one another. A cloth that stops (converged, stalled or non-finite)
+// is frozen: its tiles keep x_{k+1} = x_k for the rest of the frame, and the
+// batch frame ends when no cloth runs. Tiles read slot[par] of their cloth;
+// the cloth's first tile writes slot[par ^ 1].
+struct ClothCtl {
+    CtlSlot slot[2];
+    float  rho;
+    int    state;       // CLH_CLOTH_*
+    int    restarts;
+    int    iters;       // iterations when the cloth stopped (0 while it runs)
+    int    since_best;
+    int    pad;
+    double last_r2;
+    double best_rel2;
+    double r2, b2;      // residual^2 and ||b||^2 at the cloth's last check
 };
 
 // Device buffer owned by the solver.

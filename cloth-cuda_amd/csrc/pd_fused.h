@@ -40,6 +40,7 @@ struct FusedIterArgs {
     int   use_bend;
     FrameCtl* ctl;         // when non-null, omega and blend come from ctl->slot[par]
     int   par;
+    ClothCtl* cctl;        // batches run to a tolerance: per-cloth control (with ctl), or null
 };
 
 struct FusedPlan {

@@ -137,6 +137,8 @@ struct FusedKernelArgs {
     const float4*   hin_w;
     const float2*   hin_c;
     const unsigned* gl_words;
+    const int*      tile_cloth;   // [num_tiles] cloth of each tile
+    const int*      cloth_tiles;  // [C + 1] first tile of each cloth
     int N, tile_verts, hstride;
     int lmax, tmax, hmax;
     int off_g, off_r;  // LDS byte offsets: gather list, element region
@@ -442,6 +444,8 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
     const int hid0 = hids[min(tid, A.hstride - 1)];
     const int4 vi = own ? vld : make_int4(0, 0, -1, 0);
 
+    const int cloth = a.cctl ? A.tile_cloth[tile] : 0;
+
     const int nhalo = ti[0], tbeg = ti[1], ntri = ti[2], ebeg = ti[3], nhin = ti[4];
     const int gbeg = ti[5], glen = ti[6];
     if (a.ctl) {
@@ -449,6 +453,17 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
         a.omega = cs.omega;
         a.blend = cs.blend;
         ctl_write_next(a.ctl, a.par, cs);
+    }
+    // per-cloth control: the cloth's state arrives with trip 2
+    ClothCtl* cc = a.cctl ? a.cctl + cloth : nullptr;
+    CtlSlot ccs = cs;
+    int cstate = 0, cfirst = -1;
+    float crho = 0.0f;
+    if (cc) {
+        ccs = cc->slot[a.par];
+        cstate = cc->state;
+        crho = cc->rho;
+        cfirst = A.cloth_tiles[cloth];
     }
 
     // ---- trip 2: halo positions, first records, gather list -------------------
@@ -491,6 +506,24 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
     if (lead) {
         if (PD) stsX(sX, L, vl, to_d(ob) + od);
         else    sts3(sx, vl, ob);
+    }
+    if (cc) {
+        // a cloth that has stopped keeps x_{k+1} = x_k, and one that diverged
+        // goes back to x_n (pinned vertices at their targets) for the rest of
+        // the frame; the whole workgroup returns (the condition is the tile's)
+        if (cstate != CLH_CLOTH_RUNNING) {
+            if (lead && PD) {
+                d3 keep = od;
+                if (cstate == CLH_CLOTH_NONFINITE)
+                    keep = vi.z >= 0 ? to_d(ld3(a.pin_target, vi.z)) - to_d(ob) : d3{0.0, 0.0, 0.0};
+                st3d(static_cast<double*>(a.out), g, keep);
+            }
+            return;
+        }
+        a.omega = ccs.omega;
+        a.blend = ccs.blend;
+        if (tile == cfirst && tid == 0)
+            cc->slot[a.par ^ 1] = ctl_next_rho(crho, a.ctl->delay, a.ctl->use_cheb, ccs);
     }
     __syncthreads();
 
@@ -1436,6 +1469,8 @@ void fill_kernel_args(const FusedPlan& p, const FusedIterArgs& a, FusedKernelArg
     A.hin_w = p.hin_w.ptr;
     A.hin_c = p.hin_c.ptr;
     A.gl_words = p.gl_words.ptr;
+    A.tile_cloth = p.tile_cloth.ptr;
+    A.cloth_tiles = p.cloth_tiles.ptr;
     A.N = p.N;
     A.tile_verts = p.tile_verts;
     A.hstride = p.hstride;

@@ -360,13 +360,31 @@ __device__ __forceinline__ void polar_rotation(f3 F0, f3 F1, f3& R0, f3& R1)
 // in FP32 as the host computes them.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ float ctl_omega(const FrameCtl& c, int ck, float prev)
+__device__ __forceinline__ float ctl_omega_rho(float rho, int delay, int ck, float prev)
 {
-    const int S = c.delay > 1 ? c.delay : 1;
-    const float r2 = c.rho * c.rho;
+    const int S = delay > 1 ? delay : 1;
+    const float r2 = rho * rho;
     if (ck < S) return 1.0f;
     if (ck == S) return 2.0f / (2.0f - r2);
     return 4.0f / (4.0f - r2 * prev);
+}
+
+__device__ __forceinline__ float ctl_omega(const FrameCtl& c, int ck, float prev)
+{
+    return ctl_omega_rho(c.rho, c.delay, ck, prev);
+}
+
+// The state of the iteration after the one in `s`, for a sequence with
+// spectral radius estimate rho.
+__device__ __forceinline__ CtlSlot ctl_next_rho(float rho, int delay, int use_cheb,
+                                                const CtlSlot& s)
+{
+    CtlSlot n;
+    n.ck = s.ck + 1;
+    n.omega = ctl_omega_rho(rho, delay, n.ck, s.omega);
+    n.blend = (use_cheb && n.ck > 0 && n.omega != 1.0f) ? 1 : 0;
+    n.k = s.k + 1;
+    return n;
 }
 
 // Writes the state of the iteration after the one in `s` into slot par ^ 1
@@ -374,12 +392,23 @@ __device__ __forceinline__ float ctl_omega(const FrameCtl& c, int ck, float prev
 // `par`).
 __device__ __forceinline__ CtlSlot ctl_next(const FrameCtl& c, const CtlSlot& s)
 {
-    CtlSlot n;
-    n.ck = s.ck + 1;
-    n.omega = ctl_omega(c, n.ck, s.omega);
-    n.blend = (c.use_cheb && n.ck > 0 && n.omega != 1.0f) ? 1 : 0;
-    n.k = s.k + 1;
-    return n;
+    return ctl_next_rho(c.rho, c.delay, c.use_cheb, s);
+}
+
+// Stall rule: true when the relative residual (r2 / b2) has not fallen below
+// 0.9 of its best for `limit` checks in a row (limit > 0; the best is tracked
+// either way).
+__device__ __forceinline__ bool stall_update(double r2, double b2, double& best_rel2,
+                                             int& since_best, int limit)
+{
+    const double rel2 = b2 > 0.0 ? r2 / b2 : r2;
+    if (rel2 < 0.81 * best_rel2) {
+        best_rel2 = rel2;
+        since_best = 0;
+        return false;
+    }
+    if (rel2 < best_rel2) best_rel2 = rel2;
+    return limit > 0 && ++since_best >= limit;
 }
 
 __device__ __forceinline__ void ctl_write_next(FrameCtl* c, int par, const CtlSlot& s)
@@ -428,6 +457,10 @@ __device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, doubl
         c->converged = 1;
         c->stop = 1;
         c->iters = k + 1;
+    } else if (stall_update(r2, b2, c->best_rel2, c->since_best, c->stall_checks)) {
+        c->stalled = 1;
+        c->stop = 1;
+        c->iters = k + 1;
     } else {
         if (c->use_cheb && c->last_r2 >= 0.0 && r2 > c->last_r2) {
             CtlSlot n;
@@ -442,6 +475,43 @@ __device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, doubl
         c->last_r2 = r2;
     }
     if (c->mirror) c->mirror[slot & 1] = *c;
+}
+
+// The check of one cloth of a batch (per-cloth control): convergence,
+// divergence and the stall rule freeze the cloth; otherwise the restart rule
+// of ctl_check, on the cloth's own sequence and rho. A cloth diverges when its
+// residual is non-finite or its relative residual exceeds 1000 times its best
+// of the frame: where bending dominates, some modes of the Jacobi iteration
+// have eigenvalues below -1 and grow whatever the Chebyshev weights. k is the
+// check iteration, par the control slot of the check launch. One thread.
+__device__ __forceinline__ void cloth_check(ClothCtl* q, const FrameCtl* c, int par, int k,
+                                            double r2, double b2, bool conv)
+{
+    q->r2 = r2;
+    q->b2 = b2;
+    const double rel2 = b2 > 0.0 ? r2 / b2 : r2;
+    if (!(r2 == r2) || r2 > 1.7e308 || (q->best_rel2 < 1.0e300 && rel2 > 1.0e6 * q->best_rel2)) {
+        q->state = CLH_CLOTH_NONFINITE;
+        q->iters = k + 1;
+    } else if (conv) {
+        q->state = CLH_CLOTH_CONVERGED;
+        q->iters = k + 1;
+    } else if (stall_update(r2, b2, q->best_rel2, q->since_best, c->stall_checks)) {
+        q->state = CLH_CLOTH_STALLED;
+        q->iters = k + 1;
+    } else {
+        if (c->use_cheb && q->last_r2 >= 0.0 && r2 > q->last_r2) {
+            CtlSlot n;
+            n.ck = 0;
+            n.omega = 1.0f;
+            n.blend = 0;
+            n.k = k + 1;
+            q->slot[par ^ 1] = n;
+            q->restarts += 1;
+            if (q->restarts % 3 == 0) q->rho = fmaxf(0.0f, 1.0f - 2.0f * (1.0f - q->rho));
+        }
+        q->last_r2 = r2;
+    }
 }
 
 }  // namespace clh

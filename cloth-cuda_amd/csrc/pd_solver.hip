@@ -402,28 +402,63 @@ k_reduce_partials(const double* __restrict__ partials, int nblocks, double* __re
 // over its tiles and a fixed-order butterfly, so every sum is reproducible.
 // Writes (r^2, b^2) of each cloth to cloth_res, the batch totals to the
 // history, and converges only when every cloth meets the tolerance.
+// With per-cloth control (cctl), each cloth is checked on its own
+// (cloth_check): a cloth that has stopped keeps the residual it stopped at,
+// and the frame stops when no cloth runs (converged when every cloth did).
 constexpr int kReduceCloths = 1024;
 __global__ void __launch_bounds__(kReduceCloths)
 k_reduce_cloths(const double* __restrict__ partials, const int* __restrict__ cloth_tiles, int C,
                 double* __restrict__ cloth_res, double* __restrict__ hist, int slot,
-                FrameCtl* __restrict__ ctl, int par, int kofs)
+                FrameCtl* __restrict__ ctl, int par, int kofs, ClothCtl* __restrict__ cctl)
 {
     if (ctl && ctl->stop) return;
     constexpr int kWaves = kReduceCloths / 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const double tol2 = ctl ? ctl->tol2 : 0.0;
+    const int kchk = ctl ? ctl->slot[par].k + kofs : 0;
     double wr = 0.0, wb = 0.0;
-    int wconv = 1;
+    int wconv = 1, wrun = 0, wall = 1, wnan = 0;
     for (int c = wave; c < C; c += kWaves) {
         const int t0 = cloth_tiles[c], t1 = cloth_tiles[c + 1];
         double r = 0.0, b = 0.0;
-        for (int t = t0 + lane; t < t1; t += 64) {
-            r += partials[2 * t + 0];
-            b += partials[2 * t + 1];
-        }
+        const bool active = !cctl || cctl[c].state == CLH_CLOTH_RUNNING;
+        if (active)
+            for (int t = t0 + lane; t < t1; t += 64) {
+                r += partials[2 * t + 0];
+                b += partials[2 * t + 1];
+            }
         for (int off = 32; off > 0; off >>= 1) {
             r += __shfl_xor(r, off, 64);
             b += __shfl_xor(b, off, 64);
+        }
+        if (cctl) {
+            ClothCtl* q = cctl + c;
+            int st;
+            if (active) {
+                // lane 0 checks the cloth; its new state reaches the other
+                // lanes by a shuffle, not through memory
+                st = CLH_CLOTH_RUNNING;
+                if (lane == 0) {
+                    cloth_check(q, ctl, par, kchk, r, b, r <= tol2 * b);
+                    st = q->state;
+                }
+                st = __shfl(st, 0, 64);
+            } else {
+                r = q->r2;
+                b = q->b2;
+                st = q->state;
+            }
+            if (st == CLH_CLOTH_RUNNING) wrun = 1;
+            if (st != CLH_CLOTH_CONVERGED) wall = 0;
+            if (st == CLH_CLOTH_NONFINITE) {
+                // a diverged cloth keeps x_n and stays out of the batch totals
+                wnan = 1;
+                if (lane == 0) {
+                    cloth_res[2 * c + 0] = r;
+                    cloth_res[2 * c + 1] = b;
+                }
+                continue;
+            }
         }
         if (lane == 0) {
             cloth_res[2 * c + 0] = r;
@@ -434,22 +469,43 @@ k_reduce_cloths(const double* __restrict__ partials, const int* __restrict__ clo
         if (!(r <= tol2 * b)) wconv = 0;
     }
     __shared__ double sr[kWaves], sb[kWaves];
-    __shared__ int sc[kWaves];
+    __shared__ int sc[kWaves], srun[kWaves], sall[kWaves], snan[kWaves];
     if (lane == 0) {
         sr[wave] = wr;
         sb[wave] = wb;
         sc[wave] = wconv;
+        srun[wave] = wrun;
+        sall[wave] = wall;
+        snan[wave] = wnan;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         double tr = 0.0, tb = 0.0;
-        int conv = 1;
+        int conv = 1, run = 0, all = 1, nan = 0;
         for (int i = 0; i < kWaves; ++i) {
             tr += sr[i];
             tb += sb[i];
             conv &= sc[i];
+            run |= srun[i];
+            all &= sall[i];
+            nan |= snan[i];
         }
-        if (ctl) {
+        if (ctl && cctl) {
+            // per-cloth control: the cloths decided their own restarts; the
+            // frame stops when none runs
+            const int hs = ctl->checks;
+            hist[2 * hs + 0] = tr;
+            hist[2 * hs + 1] = tb;
+            ctl->checks = hs + 1;
+            (void)nan;
+            if (!run) {
+                ctl->converged = all;
+                ctl->stalled = all ? 0 : 1;
+                ctl->stop = 1;
+                ctl->iters = kchk + 1;
+            }
+            if (ctl->mirror) ctl->mirror[hs & 1] = *ctl;
+        } else if (ctl) {
             ctl_check(ctl, par, tr, tb, hist, conv != 0, kofs);
         } else {
             hist[2 * slot + 0] = tr;
@@ -458,9 +514,34 @@ k_reduce_cloths(const double* __restrict__ partials, const int* __restrict__ clo
     }
 }
 
+// Starts the per-cloth control of a batch frame run to a tolerance.
+__global__ void __launch_bounds__(256)
+k_cloth_ctl_init(ClothCtl* __restrict__ q, int C, float rho)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    ClothCtl z;
+    z.slot[0].omega = 1.0f;
+    z.slot[0].blend = 0;
+    z.slot[0].ck = 0;
+    z.slot[0].k = 0;
+    z.slot[1] = z.slot[0];
+    z.rho = rho;
+    z.state = CLH_CLOTH_RUNNING;
+    z.restarts = 0;
+    z.iters = 0;
+    z.since_best = 0;
+    z.pad = 0;
+    z.last_r2 = -1.0;
+    z.best_rel2 = 1.0e300;
+    z.r2 = 0.0;
+    z.b2 = 0.0;
+    q[c] = z;
+}
+
 // Starts the control state of a frame run to a tolerance.
 __global__ void k_ctl_frame_init(FrameCtl* __restrict__ c, float rho, int use_cheb, int delay,
-                                 double tol2, FrameCtl* mirror)
+                                 double tol2, FrameCtl* mirror, int stall_checks)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     FrameCtl f;
@@ -481,6 +562,10 @@ __global__ void k_ctl_frame_init(FrameCtl* __restrict__ c, float rho, int use_ch
     f.barrier_error = 0;
     f.last_r2 = -1.0;
     f.tol2 = tol2;
+    f.stall_checks = stall_checks;
+    f.since_best = 0;
+    f.best_rel2 = 1.0e300;
+    f.stalled = 0;
     f.mirror = mirror;
     *c = f;
 }
@@ -577,6 +662,8 @@ struct clh_solver {
     bool pair_fits = false;      // the plan's two-iteration tables fit in LDS
     DevBuf<unsigned> bar;        // grid barrier words of the persistent kernel
     DevBuf<double> cloth_res;    // per-cloth (r^2, b^2) of the last residual evaluation
+    DevBuf<ClothCtl> cctl;       // per-cloth control of batch frames run to a tolerance
+    bool last_cloth_ctl = false; // the last frame used it
     int variant = CLH_KERNEL_UNFUSED;
     int tile_verts = 128;
     int block_threads = 256;
@@ -749,6 +836,7 @@ int ensure_scratch(clh_solver* s)
         CLH_TRY(s->vinfo.resize(Nt));
     }
     CLH_TRY(s->cloth_res.resize((size_t)s->num_cloths * 2));
+    if (s->num_cloths > 1) CLH_TRY(s->cctl.resize((size_t)s->num_cloths));
     if (!s->bar.ptr) {
         CLH_TRY(s->bar.resize(2));
         CLH_HIP(hipMemset(s->bar.ptr, 0, 2 * sizeof(unsigned)));
@@ -832,7 +920,8 @@ int enqueue_fused_reduce(clh_solver* s, int check_slot, FrameCtl* ctl, int par, 
     if (s->num_cloths > 1)
         hipLaunchKernelGGL(k_reduce_cloths, dim3(1), dim3(kReduceCloths), 0, s->stream,
                            s->partials.ptr, s->fused.cloth_tiles.ptr, s->num_cloths,
-                           s->cloth_res.ptr, s->hist.ptr, check_slot, ctl, par, kofs);
+                           s->cloth_res.ptr, s->hist.ptr, check_slot, ctl, par, kofs,
+                           ctl ? s->cctl.ptr : nullptr);
     else
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->stream,
                            s->partials.ptr, s->fused.num_tiles, s->hist.ptr, check_slot, ctl, par,
@@ -871,6 +960,7 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
         f.use_bend = use_bend ? 1 : 0;
         f.ctl = ctl;
         f.par = par;
+        f.cctl = (ctl && s->num_cloths > 1) ? s->cctl.ptr : nullptr;
         CLH_TRY(fused_launch_iteration(s->fused, f, c.mode, s->stream));
         if (check) CLH_TRY(enqueue_fused_reduce(s, check_slot, ctl, par, 0));
         return hip_status(hipGetLastError(), "fused iteration launch");
@@ -966,6 +1056,7 @@ int enqueue_pair(clh_solver* s, const clh_step_args& a, int k, float omega0, boo
     f.use_bend = (s->E > 0 && a.d_bend_quads != nullptr) ? 1 : 0;
     f.ctl = ctl;
     f.par = par;
+    f.cctl = nullptr;  // batches to a tolerance run one iteration per launch
     const int blend1 = (c.use_chebyshev && blend1_in && omega1 != 1.0f) ? 1 : 0;
     CLH_TRY(fused_launch_pair(s->fused, f, s->it[(k + 2) % kIterBufs].ptr, omega1, blend1,
                               s->stream));
@@ -1069,6 +1160,7 @@ int step_impl(clh_solver* s, const clh_step_args& a)
     }
     const clh_config& c = s->cfg;
     const bool fixed = !(c.mode == CLH_MODE_PD && c.tolerance > 0.0f);
+    if (fixed) s->last_cloth_ctl = false;
 
     if (fixed && c.use_graph && !a.d_trace) {
         if (!s->graph_valid || !same_args(s->graph_args, a)) {
@@ -1118,13 +1210,20 @@ int step_impl(clh_solver* s, const clh_step_args& a)
     hipLaunchKernelGGL(k_ctl_frame_init, dim3(1), dim3(64), 0, s->stream, s->ctl.ptr, c.rho,
                        c.use_chebyshev ? 1 : 0, c.cheb_delay,
                        (double)c.tolerance * (double)c.tolerance,
-                       mirrored ? s->d_ctl_mirror : nullptr);
+                       mirrored ? s->d_ctl_mirror : nullptr, c.stall_checks);
     CLH_HIP(hipGetLastError());
+    // a batch: every cloth has its own control (fused variant)
+    s->last_cloth_ctl = s->num_cloths > 1 && s->variant == CLH_KERNEL_FUSED;
+    if (s->last_cloth_ctl) {
+        hipLaunchKernelGGL(k_cloth_ctl_init, dim3(grid_for(s->num_cloths)), dim3(kBlock), 0,
+                           s->stream, s->cctl.ptr, s->num_cloths, c.rho);
+        CLH_HIP(hipGetLastError());
+    }
     // two-iteration launches when every check falls on a pair's second
     // iteration: chunks of even length start at even k, so pairs run at even
     // k, and a last odd iteration (K odd) runs alone; the control slot then
     // alternates per launch, (k >> 1) & 1
-    const bool pairs = pair_active(s) && !a.d_trace && P % 2 == 0;
+    const bool pairs = pair_active(s) && !a.d_trace && P % 2 == 0 && s->num_cloths == 1;
     auto enqueue_chunk = [&](int k0, int k1) -> int {
         for (int k = k0; k < k1;) {
             if (pairs && k + 1 < k1 && !is_check_iteration(c, k)) {
@@ -1501,6 +1600,31 @@ extern "C" int clh_batch_cloth_residuals(clh_solver* s, double* out, int max_cou
     }
     for (int c = 0; c < n; ++c)
         out[c] = h[2 * (size_t)c + 1] > 0.0 ? std::sqrt(h[2 * (size_t)c] / h[2 * (size_t)c + 1]) : 0.0;
+    return CLH_OK;
+}
+
+extern "C" int clh_batch_cloth_status(clh_solver* s, int* state, int* iterations, int max_count,
+                                      int* count)
+{
+    if (!s || !count) return fail(CLH_ERR_INVALID, "clh_batch_cloth_status: null pointer");
+    CLH_HIP(hipStreamSynchronize(s->stream));
+    *count = s->num_cloths;
+    const int n = std::min(s->num_cloths, std::max(max_count, 0));
+    if (n <= 0 || (!state && !iterations)) return CLH_OK;
+    std::vector<ClothCtl> h;
+    if (s->last_cloth_ctl) {
+        h.resize((size_t)s->num_cloths);
+        CLH_HIP(hipMemcpy(h.data(), s->cctl.ptr, h.size() * sizeof(ClothCtl), hipMemcpyDeviceToHost));
+    }
+    for (int c = 0; c < n; ++c) {
+        // without per-cloth control (fixed frames, one cloth) every cloth ran
+        // the frame's iterations; a cloth still running at the end ran them all
+        const int st = h.empty() ? (s->converged ? CLH_CLOTH_CONVERGED : CLH_CLOTH_RUNNING)
+                                 : h[(size_t)c].state;
+        if (state) state[c] = st;
+        if (iterations)
+            iterations[c] = (h.empty() || st == CLH_CLOTH_RUNNING) ? s->last_iters : h[(size_t)c].iters;
+    }
     return CLH_OK;
 }
 

@@ -127,7 +127,22 @@ typedef struct clh_config {
                                (evaluated in one phase, two barriers fewer) where they
                                fit, 1 = one shared region, 0 = the default; the
                                autotuner times both. Same bits either way. */
+    int   stall_checks;     /* CLH_MODE_PD with tolerance > 0: a frame (in a batch: a
+                               cloth) whose relative residual has not fallen below 0.9
+                               of its best for this many checks in a row stops, not
+                               converged, instead of running to max_iterations; 0 = off */
 } clh_config;
+
+/* Per-cloth state of a batch frame run to a tolerance (clh_batch_cloth_status). */
+enum {
+    CLH_CLOTH_RUNNING   = 0,  /* still iterating when the frame reached max_iterations */
+    CLH_CLOTH_CONVERGED = 1,
+    CLH_CLOTH_STALLED   = 2,  /* stopped by the stall rule (clh_config.stall_checks) */
+    CLH_CLOTH_NONFINITE = 3   /* diverged: a non-finite residual, or a relative residual
+                                 above 1000 times its best of the frame; the cloth is
+                                 returned to x_n (zero velocity) for this frame and the
+                                 other cloths' results stand */
+};
 
 enum { CLH_PERSIST_OFF = 0, CLH_PERSIST_ON = 1 };
 
@@ -296,8 +311,18 @@ int clh_batch_create(const clh_config* cfg, const clh_mesh_desc* mesh, int num_c
 
 /* FP64 relative residual ||b_c - K_c x_c|| / ||b_c|| of each cloth at the last
  * residual evaluation of the last frame (CLH_MODE_PD); *count gets the number
- * of cloths. */
+ * of cloths. In a frame run to a tolerance, a cloth's value is the one at the
+ * check where it stopped. */
 int clh_batch_cloth_residuals(clh_solver* s, double* out, int max_count, int* count);
+
+/* State (CLH_CLOTH_*) and iterations of each cloth in the last frame of a
+ * batch run to a tolerance: in such frames every cloth has its own Chebyshev
+ * sequence, restarts and convergence test, and is frozen once it stops, so a
+ * cloth of a batch follows the frame it would follow alone, and the frame
+ * ends when no cloth runs. A diverged cloth does not fail the frame (a single
+ * cloth's does, with CLH_ERR_NUMERIC). Either output may be NULL; *count gets
+ * the number of cloths. */
+int clh_batch_cloth_status(clh_solver* s, int* state, int* iterations, int max_count, int* count);
 
 /* Autotuner of the fused kernel's launch shape: times every (tile_verts in
  * {32, 64, 128, 256}, block_threads in {256, 512, 1024}; in CLH_MODE_PD
