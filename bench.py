@@ -342,6 +342,164 @@ def warm_start_study(ch, n, rho, args, pos0=None, vel0=None, scene=None, gen_tol
     return out
 
 
+def batch_frames(ch, scene, solver, accel, frames, warmup, ws=None, mode="plain"):
+    """Frames of a batch run to the tolerance with per-cloth control: batch
+    frames/s, mean batch iterations per frame, and over all cloth-frames the
+    converged count and the mean iterations of the converged ones."""
+    import numpy as np
+    from warmstart import FrameLoop
+    loop = FrameLoop(scene, solver, ws, accel)
+    loop.run(warmup, mode="plain")
+    its, conv_its, conv, stalled, diverged = [], [], 0, 0, 0
+    ch.check(ch.hip().clh_synchronize(), "sync")
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        its.append(loop.step(mode))
+        st, ci = solver.cloth_status()
+        conv += int((st == ch.CLOTH_CONVERGED).sum())
+        stalled += int((st == ch.CLOTH_STALLED).sum())
+        diverged += int((st == ch.CLOTH_NONFINITE).sum())
+        conv_its += ci[st == ch.CLOTH_CONVERGED].tolist()
+    sec = time.perf_counter() - t0
+    return {"frames_per_s": frames / sec, "converged_cloth_frames_per_s": conv / sec,
+            "batch_iterations_per_frame": float(np.mean(its)),
+            "cloth_frames": frames * scene.C, "converged": conv, "stalled": stalled,
+            "diverged": diverged,
+            "converged_cloth_iterations_mean": float(np.mean(conv_its)) if conv_its else None,
+            "converged_cloth_iterations_median": float(np.median(conv_its)) if conv_its else None}
+
+
+def warm_start_cfg5(ch, args):
+    """The learned-warm-start workload: 64 cloths of 256x256 (the 256x256
+    workload's material and pins) with wind U(0, 2) N/m^2 in random
+    directions, frames to the tolerance with per-cloth control (a cloth
+    stops when it converges, stalls for 40 checks or diverges).
+      1. The plain start at Chebyshev delays 1 and 12 on 64 held-out cloths;
+         the faster is the baseline.
+      2. Training frames at that delay on 64 other cloths until at least
+         args.ws_train_frames converged cloth-frames are kept (stalled and
+         diverged cloth-frames get weight 0).
+      3. The network trained on the GPU, then the start (gain on the
+         network's correction, smoothing passes and gain on the whole
+         step) chosen on 8 validation cloths by converged cloth-frames per
+         second (frames that end by the stall rule make batch frames/s
+         alone a misleading score).
+      4. The best warm start of the validation cloths against the plain
+         start on the held-out cloths; the start deployed is the plain one
+         when the validation cloths chose it."""
+    import numpy as np
+    from helpers import CFG2, cfg5_specs
+    from warmstart import FrameLoop
+    dt = CFG2["dt"]
+    t_all = time.time()
+    common = dict(mode=ch.MODE_PD, gravity=-9.81, dt=dt, rho=args.rho, tolerance=args.tol,
+                  check_every=args.check_every, stall_checks=40, max_iterations=8000)
+
+    def solver_for(b, delay):
+        return ch.Solver(ch.config(cheb_delay=delay, **common), b.N, b.T, b.E,
+                         cloth_offsets=b.vert_off)
+
+    especs, eacc = cfg5_specs(64, seed=9, cloth_seed=9000)
+    E = build_scene(dict(specs=especs))
+    epos, evel = E.get_state()
+    plain = {}
+    for delay in (1, 12):
+        E.set_state(epos, evel)
+        sol = solver_for(E, delay)
+        plain[delay] = batch_frames(ch, E, sol, eacc, args.ws_eval_frames, args.warmup)
+        log(f"cfg5 plain, delay {delay}: " + json.dumps(plain[delay]))
+        del sol
+    best_delay = max(plain, key=lambda d: plain[d]["frames_per_s"])
+
+    # training set
+    tspecs, tacc = cfg5_specs(64)
+    T = build_scene(dict(specs=tspecs))
+    gen = solver_for(T, best_delay)
+    wsm = ch.WarmStart.for_scene(T, dt, -9.81, seed=7)
+    wsm.dataset_reserve(args.ws_max_frames * (T.N + 16 * T.C))
+    loop = FrameLoop(T, gen, wsm, tacc)
+    t0 = time.time()
+    frames = 0
+    while loop.kept < args.ws_train_frames and frames < args.ws_max_frames:
+        loop.step("plain", dataset=True)
+        frames += 1
+    gen_s = time.time() - t0
+    kept, offered = loop.kept, loop.offered
+    log(f"cfg5 training set: {frames} frames of 64 cloths in {gen_s:.1f} s, {kept}/{offered} "
+        f"converged cloth-frames kept")
+    del gen, loop
+    t1 = time.time()
+    wsm.fit_output_scale()
+    losses = wsm.train(args.ws_steps, 16384, 2e-3)
+    ch.check(ch.hip().clh_synchronize(), "sync")
+    train_s = time.time() - t1
+    params, scale = wsm.params()
+    del wsm, T
+
+    # the start, chosen on 8 validation cloths
+    vspecs, vacc = cfg5_specs(8, seed=11, cloth_seed=11000)
+    V = build_scene(dict(specs=vspecs))
+    vpos, vvel = V.get_state()
+    vsol = solver_for(V, best_delay)
+    vws = ch.WarmStart.for_scene(V, dt, -9.81)
+    cands = [("plain", None), ("extrapolation", (0.0, 0, 1.0)), ("learned", (1.0, 0, 1.0)),
+             ("learned, step smoothed 100", (1.0, 100, 1.0)),
+             ("learned, step smoothed 100, gain 0.5", (1.0, 100, 0.5)),
+             ("learned, step smoothed 10, gain 0.25", (1.0, 10, 0.25)),
+             ("extrapolation, step smoothed 100", (0.0, 100, 1.0))]
+    table = {}
+    for name, c in cands:
+        V.set_state(vpos, vvel)
+        if c is None:
+            r = batch_frames(ch, V, vsol, vacc, 6, 3)
+        else:
+            vws.set_params(params, scale * c[0])
+            vws.set_step_filter(c[1], c[2])
+            r = batch_frames(ch, V, vsol, vacc, 6, 3, ws=vws, mode="learned")
+        table[name] = r
+        log(f"cfg5 validation, {name}: {r['frames_per_s']:.2f} batch frames/s, "
+            f"{r['converged_cloth_frames_per_s']:.1f} converged cloth-frames/s, "
+            f"{r['batch_iterations_per_frame']:.0f} iterations, {r['converged']}/{r['cloth_frames']} "
+            f"converged at {r['converged_cloth_iterations_mean']}")
+    score = lambda n: table[n]["converged_cloth_frames_per_s"]
+    deployed = max(table, key=score)
+    best = max((n for n, c in cands if c is not None), key=score)
+    chosen = dict(cands)[best]
+    del vsol, vws, V
+
+    # the chosen start (learned start with the chosen filter) on the held-out cloths
+    E.set_state(epos, evel)
+    esol = solver_for(E, best_delay)
+    ews = ch.WarmStart.for_scene(E, dt, -9.81)
+    ews.set_params(params, scale * chosen[0])
+    ews.set_step_filter(chosen[1], chosen[2])
+    learned = batch_frames(ch, E, esol, eacc, args.ws_eval_frames, args.warmup, ws=ews,
+                           mode="learned")
+    log(f"cfg5 best warm start of the validation cloths ({best}): " + json.dumps(learned))
+    base = plain[best_delay]
+    out = {"workload": "64 held-out 256x256 cloths, wind U(0, 2) N/m^2 in random directions, "
+                       "frames to relative residual 1e-5 with per-cloth control (stall window 40 "
+                       "checks, divergence guard)",
+           "plain": {str(d): r for d, r in plain.items()}, "best_plain_delay": best_delay,
+           "training_cloth_frames_kept": kept, "training_cloth_frames_offered": offered,
+           "training_frames": frames, "training_generate_seconds": gen_s,
+           "training_steps": args.ws_steps, "train_seconds": train_s,
+           "loss_first": float(np.mean(losses[:20])), "loss_last": float(np.mean(losses[-100:])),
+           "validation": table, "best_warm_start": best, "deployed_start": deployed,
+           "best_warm_filter": {"correction_gain": chosen[0], "step_smoothing_passes": chosen[1],
+                                "step_gain": chosen[2]},
+           "learned": learned,
+           "speedup_frames_per_s": learned["frames_per_s"] / base["frames_per_s"],
+           "speedup_converged_cloth_frames_per_s":
+               learned["converged_cloth_frames_per_s"] / base["converged_cloth_frames_per_s"],
+           "iterations_saved_per_converged_cloth_frame":
+               (base["converged_cloth_iterations_mean"] or 0.0) -
+               (learned["converged_cloth_iterations_mean"] or 0.0),
+           "total_seconds": time.time() - t_all}
+    log(f"cfg5: learned / best plain = {out['speedup_frames_per_s']:.2f}x batch frames/s")
+    return out
+
+
 def launch_timing(ch, args):
     """The launch timing of the roofline alone: on the 256x256, 1024x1024
     and 512-cloth meshes, the autotuner's candidates and then launch_ms of
@@ -404,6 +562,12 @@ def main():
     ap.add_argument("--launch-timing", action="store_true",
                     help="only autotune and time the iteration kernel's launches on the three "
                          "meshes (no frames): for kernel-trace profiling")
+    ap.add_argument("--ws-train-frames", type=int, default=2000,
+                    help="converged cloth-frames of the 64-cloth 256x256 training set")
+    ap.add_argument("--ws-max-frames", type=int, default=60,
+                    help="cap on the 64-cloth training frames generated to reach them")
+    ap.add_argument("--ws-eval-frames", type=int, default=6,
+                    help="timed frames of each start on the 64 held-out 256x256 cloths")
     ap.add_argument("--ws-cloths", type=int, default=64)
     ap.add_argument("--ws-frames", type=int, default=32, help="training frames per cloth")
     ap.add_argument("--ws-steps", type=int, default=4000)
@@ -583,8 +747,7 @@ def main():
     # ---- learned warm start: train on batched cloths, evaluate on the workload --
     ws = None
     if not args.no_ws:
-        ws = {"at_256": warm_start_study(ch, 256, args.rho, args, pos0=pos0, vel0=vel0, scene=s,
-                                         gen_tol=1e-5, gen_mode="plain"),
+        ws = {"cfg5_64x256": warm_start_cfg5(ch, args),
               "at_64": warm_start_study(ch, 64, 0.9999, args),
               "at_32": warm_start_study(ch, 32, 0.999, args)}
         ws["plain_start_fallback"] = (
@@ -613,9 +776,9 @@ def main():
         "data": "synthetic (seeded cloths, random-init network trained on solver-generated frames)",
         "config": {"workload": "256x256 PD cloth, top edge pinned, wind 0.5 N/m^2, "
                                "frames to relative residual 1e-5",
-                   "value_mode": "plain start x_n at cheb_delay below (fastest at 256x256: "
-                                 "warm starts are rejected there; warm_start.at_256 compares "
-                                 "plain and learned starts at cheb_delay 1)",
+                   "value_mode": "plain start x_n at cheb_delay below; warm_start.cfg5_64x256 "
+                                 "compares the learned start with the best plain schedule on the "
+                                 "64-cloth random-wind workload",
                    "warm_start": ws,
                    "tolerance": args.tol, "residual": "FP64 ||b - K x|| / ||b|| on the GPU",
                    "graphs": not args.no_graph,
