@@ -127,9 +127,18 @@ size_t fused_pair_lds_bytes(const FusedPlan& p, bool check);
 int  fused_launch_pair(const FusedPlan& p, const FusedIterArgs& a, void* out2, float omega1,
                        int blend1, hipStream_t stream);
 
+// Grid barrier words of the persistent kernel: one arrival word per group of
+// workgroups (blockIdx % kBarGroups), a global arrival word and the
+// generation, each on a 128-byte line of its own.
+constexpr int kBarGroups = 8;
+constexpr int kBarLine = 32;                      // unsigned per 128-byte line
+constexpr int kBarAll = kBarGroups;               // line of the global arrival word
+constexpr int kBarGen = kBarGroups + 1;           // line of the generation word
+constexpr int kBarWords = (kBarGroups + 2) * kBarLine;
+
 // Persistent PD iterations [k0, k1) in one cooperative launch (device control
-// through ctl; iterates in it[k % kIterBufs]; grid barrier words bar[2], zero at
-// rest). fused_persist_fits tells whether every tile can be resident at once.
+// through ctl; iterates in it[k % kIterBufs]; grid barrier words bar[kBarWords],
+// zero at rest). fused_persist_fits tells whether every tile can be resident at once.
 size_t fused_persist_lds_bytes(const FusedPlan& p);
 bool fused_persist_fits(const FusedPlan& p);
 int  fused_launch_persist(const FusedPlan& p, const FusedIterArgs& a, double* const it[kIterBufs], int k0,

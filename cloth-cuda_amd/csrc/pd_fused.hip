@@ -1038,31 +1038,49 @@ struct PersistArgs {
     FrameCtl* ctl;
     double* partials;    // [num_tiles * 2]
     double* hist;        // residual history
-    unsigned* bar;       // arrivals, generation
+    unsigned* bar;       // barrier words (kBarWords unsigned, zero at rest)
     int off_rt, off_rh, off_hb;  // LDS byte offsets: tri records, hinge records, halo ids + base
 };
 
 constexpr long kSpinLimit = 1l << 24;
 
 // Grid-wide barrier of a cooperative launch: release of this workgroup's
-// writes at agent scope, arrival count, and a wait on the generation word;
-// acquire on the way out. A wait longer than kSpinLimit sleeps raises the
-// error flag `err`, which every later barrier honours, so a stuck launch
-// still ends. Returns false once the flag is up.
+// writes at agent scope, a two-level arrival, and a wait on the generation
+// word; acquire on the way out. Workgroup b counts on the arrival word of its
+// group b % 8 (the XCD it runs on under round-robin dispatch, so the
+// group's arrivals meet in one L2); the group's last arrival counts on the
+// global word, whose last arrival resets it and advances the generation. Each
+// word has a 128-byte line of its own (kBarLine unsigned apart), so polls of
+// the generation do not share a line with the arrivals. A wait longer than
+// kSpinLimit sleeps raises the error flag `err`, which every later barrier
+// honours, so a stuck launch still ends. Returns false once the flag is up.
 __device__ __forceinline__ bool grid_barrier(unsigned* bar, int* err, unsigned nblocks, int* sflag)
 {
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned* gen_w = bar + kBarGen * kBarLine;
+        unsigned* all_w = bar + kBarAll * kBarLine;
+        const unsigned grp = blockIdx.x % kBarGroups;
+        const unsigned ngrp = nblocks < kBarGroups ? nblocks : kBarGroups;
+        const unsigned in_grp = nblocks / kBarGroups + (grp < nblocks % kBarGroups ? 1u : 0u);
+        unsigned* grp_w = bar + grp * kBarLine;
+        const unsigned gen = __hip_atomic_load(gen_w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
-        const unsigned arrived =
-            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-        if (arrived == nblocks) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        bool last = false;
+        const unsigned a1 =
+            __hip_atomic_fetch_add(grp_w, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        if (a1 == in_grp) {
+            __hip_atomic_store(grp_w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned a2 =
+                __hip_atomic_fetch_add(all_w, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+            last = a2 == ngrp;
+        }
+        if (last) {
+            __hip_atomic_store(all_w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gen_w, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             long spins = 0;
-            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            while (__hip_atomic_load(gen_w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
                 if (++spins > kSpinLimit) {
                     __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -838,8 +838,8 @@ int ensure_scratch(clh_solver* s)
     CLH_TRY(s->cloth_res.resize((size_t)s->num_cloths * 2));
     if (s->num_cloths > 1) CLH_TRY(s->cctl.resize((size_t)s->num_cloths));
     if (!s->bar.ptr) {
-        CLH_TRY(s->bar.resize(2));
-        CLH_HIP(hipMemset(s->bar.ptr, 0, 2 * sizeof(unsigned)));
+        CLH_TRY(s->bar.resize(kBarWords));
+        CLH_HIP(hipMemset(s->bar.ptr, 0, kBarWords * sizeof(unsigned)));
     }
     CLH_TRY(s->fr.resize(T * 9));
     CLH_TRY(s->bp.resize(E * 3));
