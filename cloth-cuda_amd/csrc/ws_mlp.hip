@@ -1,6 +1,7 @@
 // Learned warm start of the PD frame: per-vertex features, a BF16 MLP on
-// MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation) for inference and
-// training, and Adam on FP32 master weights.
+// MFMA (v_mfma_f32_16x16x32_bf16 for the 64-wide layers, 16x16x16 where K is
+// 16; FP32 accumulation) for inference and training, and Adam on FP32 master
+// weights.
 //
 // Prediction of the frame's converged positions x_{n+1} from the previous
 // positions x_n, x_{n-1}, x_{n-2}:
@@ -21,7 +22,8 @@
 // The layers are evaluated transposed, Z^T = W X^T, with weights as A and a
 // group of 16 vertices as the columns: the accumulator of one layer (hidden
 // units 4q..4q+3 of vertex r) is then exactly the B operand of the next, so
-// the forward chain moves no data between lanes. Weight gradients reduce
+// the forward chain moves no data between lanes (the 16x16x32 steps join two
+// such 4-wide blocks per lane, for A and B alike). Weight gradients reduce
 // over vertices, which the operands hold along K; their inputs go through a
 // per-wave LDS transpose.
 #include "clh_internal.h"
@@ -54,9 +56,30 @@ constexpr int kMaxGradWaves = 1024;
 typedef short bf4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
+typedef short s8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// 16x16x16 product: lane l holds row (A) or column (B) l % 16 and the four K
+// entries 4 (l / 16) + j. Used where K is 16 (the 16 features, the 16 samples
+// of a weight-gradient tile).
 __device__ __forceinline__ f4 mfma16(bf4 a, bf4 b, f4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// gfx950's 16x16x32 product for the K = 64 layers, in two steps instead of
+// four. Each lane joins two of its 4-wide blocks, cat8(x[2c], x[2c + 1]): its
+// eight K entries are then units 32c + 4q + j and 32c + 16 + 4q + j (q = l / 16,
+// j < 4). A and B are joined the same way, so the pairs multiplied are those
+// of the 16x16x16 chain; only the order of the FP32 sum differs.
+__device__ __forceinline__ bf8 cat8(bf4 a, bf4 b)
+{
+    const s8v s = __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_bit_cast(bf8, s);
+}
+__device__ __forceinline__ f4 mfma32(bf8 a, bf8 b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
 // FP32 -> BF16, round to nearest even (finite inputs; NaN stays NaN).
@@ -185,7 +208,8 @@ __global__ void __launch_bounds__(256) k_ws_features(FeatArgs a)
 // ---------------------------------------------------------------------------
 
 struct WaveWeights {
-    bf4 w1[4], w2[4][4], w3[4];
+    bf4 w1[4];
+    bf8 w2[4][2], w3[2];
     f4 b1[4], b2[4], b3;
 };
 
@@ -196,10 +220,17 @@ __device__ __forceinline__ void load_forward_weights(const unsigned short* Wq, c
     for (int m = 0; m < 4; ++m) {
         w.w1[m] = ld4(Wq + qW1 + (m * 16 + r) * kF + 4 * q);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) w.w2[m][k] = ld4(Wq + qW2 + (m * 16 + r) * kH + k * 16 + 4 * q);
-        w.w3[m] = ld4(Wq + qW3 + r * kH + m * 16 + 4 * q);
+        for (int c = 0; c < 2; ++c) {
+            const unsigned short* row = Wq + qW2 + (m * 16 + r) * kH + 32 * c + 4 * q;
+            w.w2[m][c] = cat8(ld4(row), ld4(row + 16));
+        }
         w.b1[m] = ldf4(P + oB1 + m * 16 + 4 * q);
         w.b2[m] = ldf4(P + oB2 + m * 16 + 4 * q);
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const unsigned short* row = Wq + qW3 + r * kH + 32 * c + 4 * q;
+        w.w3[c] = cat8(ld4(row), ld4(row + 16));
     }
     w.b3 = ldf4(P + oB3 + 4 * q);
 }
@@ -223,7 +254,7 @@ __device__ __forceinline__ f4 mlp_forward(const WaveWeights& w, bf4 fb, f4 z1[4]
     for (int m = 0; m < 4; ++m) {
         f4 acc = zero;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) acc = mfma16(w.w2[m][k], h1[k], acc);
+        for (int c = 0; c < 2; ++c) acc = mfma32(w.w2[m][c], cat8(h1[2 * c], h1[2 * c + 1]), acc);
         z2[m] = acc + w.b2[m];
         f4 h;
 #pragma unroll
@@ -232,7 +263,7 @@ __device__ __forceinline__ f4 mlp_forward(const WaveWeights& w, bf4 fb, f4 z1[4]
     }
     f4 o = zero;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) o = mfma16(w.w3[k], h2[k], o);
+    for (int c = 0; c < 2; ++c) o = mfma32(w.w3[c], cat8(h2[2 * c], h2[2 * c + 1]), o);
     return o + w.b3;
 }
 
@@ -379,11 +410,15 @@ __global__ void __launch_bounds__(64 * kWaves) k_ws_grad(GradArgs a)
 
     WaveWeights w;
     load_forward_weights(a.Wq, a.P, r, q, w);
-    bf4 w2t[4][4], w3t[4];
+    bf8 w2t[4][2];
+    bf4 w3t[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) w2t[m][k] = ld4(a.Wq + qW2T + (m * 16 + r) * kH + k * 16 + 4 * q);
+        for (int c = 0; c < 2; ++c) {
+            const unsigned short* row = a.Wq + qW2T + (m * 16 + r) * kH + 32 * c + 4 * q;
+            w2t[m][c] = cat8(ld4(row), ld4(row + 16));
+        }
         w3t[m] = ld4(a.Wq + qW3T + (m * 16 + r) * kO + 4 * q);
     }
     const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -435,7 +470,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_ws_grad(GradArgs a)
         for (int m = 0; m < 4; ++m) {
             f4 dh = zero;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) dh = mfma16(w2t[m][k], dz2b[k], dh);
+            for (int c = 0; c < 2; ++c) dh = mfma32(w2t[m][c], cat8(dz2b[2 * c], dz2b[2 * c + 1]), dh);
 #pragma unroll
             for (int j = 0; j < 4; ++j) dz1[m][j] = dh[j] * dsilu(z1[m][j]);
             gb1[m] += dz1[m];
