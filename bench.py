@@ -10,12 +10,14 @@ falls to the tolerance (default 1e-5). `value` is frames per second of that
 loop with the state resident in HBM, each frame started from x_n (timed
 with device events).
 
-config.warm_start compares the plain start with the second-order
-extrapolation and the learned warm start (trained here, on the GPU, by the
-library's own kernels, on frames the batched solver generates for 64 cloths
-with random wind; the gain on its correction chosen on 8 held-out cloths) on
-the same workload, at 256x256, 64x64 and 32x32, each frame to the same
-tolerance with the plain start as arbiter and fallback.
+config.warm_start compares the plain start with the learned warm start
+(trained here, on the GPU, by the library's own kernels, on frames the
+batched solver generates): `cfg5_64x256` on 64 held-out 256x256 cloths with
+random wind of 0-2 N/m^2 (2,000 converged training cloth-frames, the start
+chosen on 8 validation cloths, against the best plain schedule), and
+`at_64` / `at_32` on the workload's scene at 64x64 and 32x32 (beside the
+second-order extrapolation), each frame to the same tolerance with the plain
+start as arbiter and fallback.
 
 The roofline object describes the kernel that dominates the run, the fused
 iteration kernel at 256x256 in the launch shape the autotuner chose (one or
@@ -500,6 +502,43 @@ def warm_start_cfg5(ch, args):
     return out
 
 
+def ws_kernels(ch, args):
+    """The learned warm start's kernels at its workload's size, for
+    kernel-trace profiling: the 64-cloth 256x256 random-wind batch, 4 plain
+    frames to the tolerance written into the training set, args.ws_steps
+    Adam steps of 16,384 samples, then 2 frames from the learned start.
+    Chunks are enqueued directly (no graph capture); prints one JSON line of
+    the host-side times."""
+    from helpers import CFG2, cfg5_specs
+    from warmstart import FrameLoop
+    specs, acc = cfg5_specs(64)
+    b = build_scene(dict(specs=specs))
+    sol = ch.Solver(ch.config(mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
+                              cheb_delay=args.cheb_delay, tolerance=args.tol,
+                              check_every=args.check_every, max_iterations=3000,
+                              stall_checks=40, use_graph=0),
+                    b.N, b.T, b.E, cloth_offsets=b.vert_off)
+    ws = ch.WarmStart.for_scene(b, CFG2["dt"], -9.81, seed=7)
+    ws.dataset_reserve(6 * (b.N + 16 * b.C))
+    loop = FrameLoop(b, sol, ws, acc)
+    out = {"vertices": b.N, "plain_frame_s": [], "learned_frame_s": []}
+    for _ in range(4):
+        t = time.time()
+        loop.step("plain", dataset=True)
+        out["plain_frame_s"].append(time.time() - t)
+    ws.fit_output_scale()
+    t = time.time()
+    ws.train(args.ws_steps, 16384, 2e-3)
+    ch.check(ch.hip().clh_synchronize(), "sync")
+    out["train_s"] = time.time() - t
+    out["train_steps"] = args.ws_steps
+    for _ in range(2):
+        t = time.time()
+        loop.step("learned")
+        out["learned_frame_s"].append(time.time() - t)
+    print(json.dumps({"ws_kernels": out}), flush=True)
+
+
 def launch_timing(ch, args):
     """The launch timing of the roofline alone: on the 256x256, 1024x1024
     and 512-cloth meshes, the autotuner's candidates and then launch_ms of
@@ -562,6 +601,10 @@ def main():
     ap.add_argument("--launch-timing", action="store_true",
                     help="only autotune and time the iteration kernel's launches on the three "
                          "meshes (no frames): for kernel-trace profiling")
+    ap.add_argument("--ws-kernels", action="store_true",
+                    help="only run the learned warm start's kernels at the 64-cloth 256x256 "
+                         "workload's size (4 plain frames, --ws-steps training steps, 2 learned "
+                         "frames; no graphs): for kernel-trace profiling")
     ap.add_argument("--ws-train-frames", type=int, default=2000,
                     help="converged cloth-frames of the 64-cloth 256x256 training set")
     ap.add_argument("--ws-max-frames", type=int, default=60,
@@ -593,6 +636,9 @@ def main():
     log(f"device: {info.name.decode()} {info.arch.decode()} CUs={info.compute_units}")
     if args.launch_timing:
         launch_timing(ch, args)
+        return
+    if args.ws_kernels:
+        ws_kernels(ch, args)
         return
 
     # ---- 256x256: frames/s to the residual tolerance ---------------------------

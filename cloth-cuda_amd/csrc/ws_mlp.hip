@@ -100,10 +100,19 @@ __device__ __forceinline__ bf4 pack4(f4 v)
     o[3] = (short)f2bf(v[3]);
     return o;
 }
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+// SiLU and its derivative from the hardware exponential and reciprocal
+// (v_exp_f32, v_rcp_f32; about 1e-7 relative, far below the BF16 rounding of
+// the activations, 4e-3). With the full-precision expf and division the
+// forward over 4.2 M vertices took 516 us against 279. Large |z| saturates
+// correctly: e = inf gives 1 / inf = 0.
+__device__ __forceinline__ float sigmoid_fast(float z)
+{
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid_fast(z); }
 __device__ __forceinline__ float dsilu(float z)
 {
-    const float s = 1.0f / (1.0f + expf(-z));
+    const float s = sigmoid_fast(z);
     return s * (1.0f + z * (1.0f - s));
 }
 __device__ __forceinline__ bf4 ld4(const unsigned short* p) { return *reinterpret_cast<const bf4*>(p); }
@@ -547,15 +556,37 @@ __global__ void __launch_bounds__(64 * kWaves) k_ws_grad(GradArgs a)
     if (lane == 0) out[kParams] = loss * a.inv_norm;
 }
 
-// grad[p] = sum over waves of partial[w][p], in wave order.
-__global__ void __launch_bounds__(256)
+// grad[p] = sum over waves of partial[w][p] in a fixed order: the waves are
+// cut into kRedSlices contiguous runs, each summed in wave order by one wave
+// of the workgroup (64 parameters per workgroup, one per lane, so every load
+// of a wave is 256 contiguous bytes), and the run sums are added pairwise,
+// (s_0 + s_8) + ..., in a fixed tree.
+constexpr int kRedSlices = 16;
+__global__ void __launch_bounds__(64 * kRedSlices)
 k_ws_reduce(const float* __restrict__ partial, int nwaves, float* __restrict__ grad)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > kParams) return;
+    __shared__ float run[kRedSlices][64];
+    const int lane = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
+    const int per = (nwaves + kRedSlices - 1) / kRedSlices;
+    const int w0 = sl * per, w1 = min(nwaves, w0 + per);
     float s = 0.0f;
-    for (int w = 0; w < nwaves; ++w) s += partial[(size_t)w * (kParams + 1) + p];
-    grad[p] = s;
+    if (p <= kParams) {
+#pragma unroll 8
+        for (int w = w0; w < w1; ++w) s += partial[(size_t)w * (kParams + 1) + p];
+    }
+    run[sl][lane] = s;
+    __syncthreads();
+    if (sl == 0 && p <= kParams) {
+        float t[kRedSlices];
+#pragma unroll
+        for (int i = 0; i < kRedSlices; ++i) t[i] = run[i][lane];
+#pragma unroll
+        for (int h = kRedSlices / 2; h > 0; h >>= 1)
+#pragma unroll
+            for (int i = 0; i < h; ++i) t[i] = t[i] + t[i + h];
+        grad[p] = t[0];
+    }
 }
 
 // BF16 copies of one master weight (row-major and, for W2 and W3, transposed).
@@ -700,8 +731,8 @@ int launch_grad(clh_ws* w, long long first, int count, unsigned long long seed)
     a.inv_norm = 1.0f / (3.0f * (float)count);
     a.partial = w->partial.ptr;
     hipLaunchKernelGGL(k_ws_grad, dim3(blocks), dim3(64 * kWaves), 0, w->stream, a);
-    hipLaunchKernelGGL(k_ws_reduce, dim3(blocks256(kParams + 1)), dim3(256), 0, w->stream,
-                       w->partial.ptr, blocks * kWaves, w->grad.ptr);
+    hipLaunchKernelGGL(k_ws_reduce, dim3((kParams + 1 + 63) / 64), dim3(64 * kRedSlices), 0,
+                       w->stream, w->partial.ptr, blocks * kWaves, w->grad.ptr);
     return hip_status(hipGetLastError(), "warm-start gradient launch");
 }
 
