@@ -54,9 +54,10 @@ struct FrameCtl {
     double last_r2;         // residual^2 at the previous check (-1: none)
     double tol2;            // tolerance^2
     int    stall_checks;    // > 0: stop, not converged, after this many checks in a row
-                            // without the relative residual falling below 0.9 of its best
-    int    since_best;      // checks since the relative residual last set a best
+                            // without the relative residual dropping by 10 %
+    int    since_drop;      // checks since the relative residual last dropped by 10 %
     double best_rel2;       // best (r / b)^2 of the frame so far
+    double stall_ref;       // (r / b)^2 at the last 10 % drop
     int    stalled;         // the frame stopped on the stall rule
     // host-visible copies of this block (pinned, coherent): after check c
     // the reduction writes the block to mirror[c & 1], so the host reads the
@@ -76,10 +77,11 @@ struct ClothCtl {
     int    state;       // CLH_CLOTH_*
     int    restarts;
     int    iters;       // iterations when the cloth stopped (0 while it runs)
-    int    since_best;
+    int    since_drop;
     int    pad;
     double last_r2;
     double best_rel2;
+    double stall_ref;
     double r2, b2;      // residual^2 and ||b||^2 at the cloth's last check
 };
 

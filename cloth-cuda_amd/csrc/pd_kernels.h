@@ -395,20 +395,23 @@ __device__ __forceinline__ CtlSlot ctl_next(const FrameCtl& c, const CtlSlot& s)
     return ctl_next_rho(c.rho, c.delay, c.use_cheb, s);
 }
 
-// Stall rule: true when the relative residual (r2 / b2) has not fallen below
-// 0.9 of its best for `limit` checks in a row (limit > 0; the best is tracked
-// either way).
+// Stall rule: true when the relative residual (r2 / b2) has not dropped by
+// 10 % for `limit` checks in a row (limit > 0): a drop is a value below 0.9
+// of the one at the last drop (stall_ref, squared as rel2 is), so a residual
+// that falls steadily, however slowly per check, keeps resetting the count.
+// best_rel2, the frame's smallest relative residual, is tracked either way
+// (the divergence guard compares with it).
 __device__ __forceinline__ bool stall_update(double r2, double b2, double& best_rel2,
-                                             int& since_best, int limit)
+                                             double& stall_ref, int& since_drop, int limit)
 {
     const double rel2 = b2 > 0.0 ? r2 / b2 : r2;
-    if (rel2 < 0.81 * best_rel2) {
-        best_rel2 = rel2;
-        since_best = 0;
+    if (rel2 < best_rel2) best_rel2 = rel2;
+    if (rel2 < 0.81 * stall_ref) {
+        stall_ref = rel2;
+        since_drop = 0;
         return false;
     }
-    if (rel2 < best_rel2) best_rel2 = rel2;
-    return limit > 0 && ++since_best >= limit;
+    return limit > 0 && ++since_drop >= limit;
 }
 
 __device__ __forceinline__ void ctl_write_next(FrameCtl* c, int par, const CtlSlot& s)
@@ -457,7 +460,7 @@ __device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, doubl
         c->converged = 1;
         c->stop = 1;
         c->iters = k + 1;
-    } else if (stall_update(r2, b2, c->best_rel2, c->since_best, c->stall_checks)) {
+    } else if (stall_update(r2, b2, c->best_rel2, c->stall_ref, c->since_drop, c->stall_checks)) {
         c->stalled = 1;
         c->stop = 1;
         c->iters = k + 1;
@@ -496,7 +499,7 @@ __device__ __forceinline__ void cloth_check(ClothCtl* q, const FrameCtl* c, int 
     } else if (conv) {
         q->state = CLH_CLOTH_CONVERGED;
         q->iters = k + 1;
-    } else if (stall_update(r2, b2, q->best_rel2, q->since_best, c->stall_checks)) {
+    } else if (stall_update(r2, b2, q->best_rel2, q->stall_ref, q->since_drop, c->stall_checks)) {
         q->state = CLH_CLOTH_STALLED;
         q->iters = k + 1;
     } else {

@@ -530,10 +530,11 @@ k_cloth_ctl_init(ClothCtl* __restrict__ q, int C, float rho)
     z.state = CLH_CLOTH_RUNNING;
     z.restarts = 0;
     z.iters = 0;
-    z.since_best = 0;
+    z.since_drop = 0;
     z.pad = 0;
     z.last_r2 = -1.0;
     z.best_rel2 = 1.0e300;
+    z.stall_ref = 1.0e300;
     z.r2 = 0.0;
     z.b2 = 0.0;
     q[c] = z;
@@ -563,8 +564,9 @@ __global__ void k_ctl_frame_init(FrameCtl* __restrict__ c, float rho, int use_ch
     f.last_r2 = -1.0;
     f.tol2 = tol2;
     f.stall_checks = stall_checks;
-    f.since_best = 0;
+    f.since_drop = 0;
     f.best_rel2 = 1.0e300;
+    f.stall_ref = 1.0e300;
     f.stalled = 0;
     f.mirror = mirror;
     *c = f;

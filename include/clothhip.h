@@ -128,9 +128,10 @@ typedef struct clh_config {
                                fit, 1 = one shared region, 0 = the default; the
                                autotuner times both. Same bits either way. */
     int   stall_checks;     /* CLH_MODE_PD with tolerance > 0: a frame (in a batch: a
-                               cloth) whose relative residual has not fallen below 0.9
-                               of its best for this many checks in a row stops, not
-                               converged, instead of running to max_iterations; 0 = off */
+                               cloth) whose relative residual has not dropped by 10 %
+                               (below 0.9 of its value at the last such drop) for this
+                               many checks in a row stops, not converged, instead of
+                               running to max_iterations; 0 = off */
 } clh_config;
 
 /* Per-cloth state of a batch frame run to a tolerance (clh_batch_cloth_status). */

@@ -175,10 +175,8 @@ def test_batch_with_folding_cloth_ends_early():
     stalls on it near 4e-2 in the oracle, or diverges: where bending
     dominates, Jacobi modes with eigenvalues below -1 grow, and on the GPU
     this cloth's residual grows about 1e6 times per check from its tenth
-    check on, alone as in a batch. The cloth stops as stalled (no 10 %
-    improvement of its best relative residual in 100 checks, 2,000
-    iterations: Chebyshev's residual is not monotone, and a window of 25
-    checks stopped a cloth at 1.6e-5 that converges at 940) or diverged
+    check on, alone as in a batch. The cloth stops as stalled (no 10 % drop
+    of its relative residual in 100 checks, 2,000 iterations) or diverged
     (1000 times its best), and a diverged cloth returns to x_n with zero
     velocity; the other cloths converge and are frozen, the frame raises no
     error, and it ends long before the iteration cap."""
