@@ -71,17 +71,21 @@ def tune_line(cands, digits):
 
 
 def profiled_traffic(path, plan):
-    """HBM bytes per launch of the plan's launch shape (tile vertices, block
-    threads, iterations per launch), as recorded in a committed rocprofv3
-    summary."""
+    """HBM bytes per launch of the plan's launch shape, as recorded in a
+    committed rocprofv3 summary: the `bytes / launch` column of the row
+    `| tile / threads | dispatches | FETCH KiB | WRITE KiB | bytes / launch |`
+    of one-iteration launches (None for two iterations per launch, which the
+    summaries do not cover, or a shape they lack)."""
+    if plan.iters_per_launch != 1:
+        return None
     try:
         with open(path) as f:
             text = f.read()
     except OSError:
         return None
-    m = re.search(r"traffic_bytes_per_launch\[%d,%d,%d\]:\s*([0-9.eE+]+)"
-                  % (plan.tile_verts, plan.block_threads, plan.iters_per_launch), text)
-    return float(m.group(1)) if m else None
+    m = re.search(r"^\|\s*%d\s*/\s*%d\s*\|\s*\d+\s*\|\s*[0-9.]+\s*\|\s*[0-9.]+\s*\|"
+                  r"\s*([0-9,]+)\s*\|" % (plan.tile_verts, plan.block_threads), text, re.M)
+    return float(m.group(1).replace(",", "")) if m else None
 
 
 def kernel_name(plan, mesh):
