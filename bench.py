@@ -54,6 +54,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def schedule(args):
+    """The frame schedule's damping and restart settings (clh_config)."""
+    return dict(relax=args.relax, restart_rule=args.restart_rule,
+                restart_factor=args.restart_factor)
+
+
+def oracle_schedule(args, sol):
+    """The same settings for the oracle, with the step scale the solver used."""
+    return dict(relax=sol.stats().relax, restart_rule=args.restart_rule,
+                restart_factor=args.restart_factor)
+
+
 def algorithmic_bytes(N, T, E, iters_per_launch=1):
     """Bytes one launch of the iteration kernel must move at the least: per
     vertex x_k, x_{k-1} and y (12 B each), mass and Jacobi diagonal (4 B each)
@@ -122,7 +134,7 @@ def fp64_error(ch, s, pos0, vel0, common, iterations):
     m = oracle_mesh(scene_arrays(s))
     p = po.params(mode=po.PD, iterations=iterations, gravity=-9.81, dt=CFG2["dt"],
                   rho=common["rho"], cheb_delay=common["cheb_delay"], ext_accel=CFG2_WIND,
-                  threads=min(os.cpu_count() or 1, 16))
+                  threads=host_cpu()["omp_threads"], relax=sol.stats().relax)
     x64, *_ = po.step(m, p, pos0, vel0, "f64")
     return float(np.abs(x_gpu.astype(np.float64) - x64).max() / extent(pos0))
 
@@ -159,7 +171,7 @@ def tolerance_frame_error(ch, s, sol, args_c, pos_w, vel_w, args):
     m = oracle_mesh(scene_arrays(s))
     threads = host_cpu()["omp_threads"]
     kw = dict(mode=po.PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho, ext_accel=CFG2_WIND,
-              check_every=args.check_every, threads=threads)
+              check_every=args.check_every, threads=threads, **oracle_schedule(args, sol))
     e = extent(pos_w)
     out = {"gpu_iterations": int(st.iterations)}
     for name, prec, tol, delay, cap in (("fp32", "f32", args.tol, args.cheb_delay, args.max_iters),
@@ -174,7 +186,7 @@ def tolerance_frame_error(ch, s, sol, args_c, pos_w, vel_w, args):
     return out
 
 
-def cpu_baseline(ch, s, pos_w, vel_w, args):
+def cpu_baseline(ch, s, sol, pos_w, vel_w, args):
     """The oracle (C++, FP32 storage and FP64 iterates, as the GPU) timed on
     the host: frames of the benchmark's workload to the tolerance at the
     benchmark's settings, from the state the GPU's timed frames start from,
@@ -188,7 +200,7 @@ def cpu_baseline(ch, s, pos_w, vel_w, args):
     threads = host["omp_threads"]
     m = oracle_mesh(scene_arrays(s))
     kw = dict(mode=po.PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho, cheb_delay=args.cheb_delay,
-              ext_accel=CFG2_WIND)
+              ext_accel=CFG2_WIND, **oracle_schedule(args, sol))
     p = po.params(iterations=args.max_iters, tolerance=args.tol, check_every=args.check_every,
                   threads=threads, **kw)
     pos, vel = pos_w.copy(), vel_w.copy()
@@ -229,6 +241,7 @@ def cpu_baseline(ch, s, pos_w, vel_w, args):
     return {"value": frames / dt_cpu, "unit": "frames/s", "cores": threads, "kind": "port",
             "sample": f"{frames} frames of the 256x256 workload to relative residual {args.tol} "
                       f"at the benchmark's settings (rho {args.rho}, delay {args.cheb_delay}, "
+                      f"step scale {kw['relax']:.6f}, restart rule {args.restart_rule}, "
                       f"check every {args.check_every}), from the state the GPU's timed frames "
                       f"start from: C++ oracle, FP32 storage and FP64 iterates, OpenMP "
                       f"{threads} threads, {dt_cpu:.1f} s, {its / frames:.0f} iterations/frame",
@@ -380,7 +393,8 @@ def warm_start_cfg5(ch, args):
     workload's material and pins) with wind U(0, 2) N/m^2 in random
     directions, frames to the tolerance with per-cloth control (a cloth
     stops when it converges, stalls for 40 checks or diverges).
-      1. The plain start at Chebyshev delays 1 and 12 on 64 held-out cloths;
+      1. The plain start at Chebyshev delays 1 and --cheb-delay on 64 held-out
+         cloths, with the benchmark's step scale and restart rule;
          the faster is the baseline.
       2. Training frames at that delay on 64 other cloths until at least
          args.ws_train_frames converged cloth-frames are kept (stalled and
@@ -399,7 +413,8 @@ def warm_start_cfg5(ch, args):
     dt = CFG2["dt"]
     t_all = time.time()
     common = dict(mode=ch.MODE_PD, gravity=-9.81, dt=dt, rho=args.rho, tolerance=args.tol,
-                  check_every=args.check_every, stall_checks=40, max_iterations=8000)
+                  check_every=args.check_every, stall_checks=40, max_iterations=8000,
+                  **schedule(args))
 
     def solver_for(b, delay):
         return ch.Solver(ch.config(cheb_delay=delay, **common), b.N, b.T, b.E,
@@ -409,7 +424,7 @@ def warm_start_cfg5(ch, args):
     E = build_scene(dict(specs=especs))
     epos, evel = E.get_state()
     plain = {}
-    for delay in (1, 12):
+    for delay in (1, args.cheb_delay):
         E.set_state(epos, evel)
         sol = solver_for(E, delay)
         plain[delay] = batch_frames(ch, E, sol, eacc, args.ws_eval_frames, args.warmup)
@@ -590,9 +605,17 @@ def main():
     ap.add_argument("--max-iters", type=int, default=80000)
     ap.add_argument("--check-every", type=int, default=50)
     ap.add_argument("--rho", type=float, default=0.999999)
-    ap.add_argument("--cheb-delay", type=int, default=12,
+    ap.add_argument("--cheb-delay", type=int, default=10,
                     help="plain Jacobi iterations before the Chebyshev blend starts (and after "
-                         "each restart) in the 256x256 frames to the tolerance")
+                         "each restart) in the frames to the tolerance (10, Wang 2015; with the "
+                         "damped step any delay from 4 to 20 gives the same rate)")
+    ap.add_argument("--relax", type=float, default=-1.0,
+                    help="scale gamma of the Jacobi step; -1: from the Gershgorin bound of "
+                         "D^-1 K (CLH_RELAX_AUTO), 1: plain Jacobi")
+    ap.add_argument("--restart-rule", type=int, default=1,
+                    help="Chebyshev restart rule: 0 on any growth between checks, 1 on growth "
+                         "by --restart-factor over the sequence's minimum, 2 none")
+    ap.add_argument("--restart-factor", type=float, default=10.0)
     ap.add_argument("--variant", type=int, default=0, help="0 auto, 1 unfused, 2 fused")
     ap.add_argument("--no-graph", action="store_true",
                     help="enqueue the 256x256 frames' launches directly instead of replaying "
@@ -652,7 +675,7 @@ def main():
     pos0, vel0 = s.get_state()
     common = dict(mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
                   cheb_delay=args.cheb_delay, ext_accel=CFG2_WIND, kernel_variant=args.variant,
-                  use_graph=0 if args.no_graph else 1)
+                  use_graph=0 if args.no_graph else 1, **schedule(args))
     sol_tol = ch.Solver(ch.config(max_iterations=args.max_iters, tolerance=args.tol,
                                   check_every=args.check_every, **common), N, T, E)
     rep256 = sol_tol.autotune(args_c, 200)
@@ -675,6 +698,9 @@ def main():
         if f % 5 == 0:
             log(f"  frame {f}: {st.iterations} iterations, rel residual {resid[-1]:.3e}, "
                 f"{ms:.2f} ms")
+    st_tol = sol_tol.stats()
+    log(f"schedule: rho {args.rho}, delay {args.cheb_delay}, step scale {st_tol.relax:.6f} "
+        f"(Gershgorin bound of D^-1 K {st_tol.jacobi_bound:.6f}), restart rule {args.restart_rule}")
     fps_tol = args.steps / (ms_total / 1e3)
     ms_per_frame = ms_total / args.steps
     mean_iters = float(np.mean(iters))
@@ -808,7 +834,7 @@ def main():
     # ---- CPU baseline: the oracle's FP32 PD frames on the host cores -------------
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(ch, s, pos_w, vel_w, args)
+        cpu = cpu_baseline(ch, s, sol_tol, pos_w, vel_w, args)
 
     value = fps_tol
     line = {
@@ -826,7 +852,8 @@ def main():
         "data": "synthetic (seeded cloths, random-init network trained on solver-generated frames)",
         "config": {"workload": "256x256 PD cloth, top edge pinned, wind 0.5 N/m^2, "
                                "frames to relative residual 1e-5",
-                   "value_mode": "plain start x_n at cheb_delay below; warm_start.cfg5_64x256 "
+                   "value_mode": "plain start x_n at cheb_delay, relax and restart_rule "
+                                 "below; warm_start.cfg5_64x256 "
                                  "compares the learned start with the best plain schedule on the "
                                  "64-cloth random-wind workload",
                    "warm_start": ws,
@@ -834,6 +861,8 @@ def main():
                    "graphs": not args.no_graph,
                    "max_iterations": args.max_iters, "check_every": args.check_every,
                    "rho": args.rho, "cheb_delay": args.cheb_delay,
+                   "relax": float(st_tol.relax), "jacobi_bound": float(st_tol.jacobi_bound),
+                   "restart_rule": args.restart_rule,
                    "iterations_per_frame": mean_iters,
                    "frames_converged": int(sum(conv)),
                    "final_rel_residual_mean": float(np.mean(resid)),

@@ -24,6 +24,8 @@ KERNEL_AUTO, KERNEL_UNFUSED, KERNEL_FUSED = 0, 1, 2
 OK, ERR_INVALID, ERR_DEVICE, ERR_ALLOC, ERR_UNSUPPORTED, ERR_NUMERIC = range(6)
 CLOTH_RUNNING, CLOTH_CONVERGED, CLOTH_STALLED, CLOTH_NONFINITE = range(4)
 PERSIST_OFF, PERSIST_ON = 0, 1
+RESTART_PREV, RESTART_SEQ_MIN, RESTART_NONE = 0, 1, 2
+RELAX_AUTO = -1.0
 PIN_NONE, PIN_TOP_ROW, PIN_CORNERS, PIN_RANDOM_BOUNDARY = 0, 1, 2, 3
 MESH_IRREGULAR = 4
 
@@ -46,7 +48,8 @@ class Config(C.Structure):
                 ("kernel_variant", C.c_int), ("use_graph", C.c_int), ("tile_verts", C.c_int),
                 ("block_threads", C.c_int), ("persistent", C.c_int),
                 ("iters_per_launch", C.c_int), ("element_regions", C.c_int),
-                ("stall_checks", C.c_int)]
+                ("stall_checks", C.c_int), ("restart_rule", C.c_int),
+                ("restart_factor", C.c_float), ("relax", C.c_float)]
 
 
 class MeshDesc(C.Structure):
@@ -67,7 +70,9 @@ class StepArgs(C.Structure):
 
 class StepStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
-                ("residual0", C.c_double), ("rhs_norm", C.c_double), ("frames", C.c_int)]
+                ("residual0", C.c_double), ("rhs_norm", C.c_double), ("frames", C.c_int),
+                ("restarts", C.c_int), ("rho", C.c_float), ("relax", C.c_float),
+                ("jacobi_bound", C.c_double)]
 
 
 class PlanInfo(C.Structure):

@@ -59,6 +59,9 @@ struct FrameCtl {
     double best_rel2;       // best (r / b)^2 of the frame so far
     double stall_ref;       // (r / b)^2 at the last 10 % drop
     int    stalled;         // the frame stopped on the stall rule
+    int    restart_rule;    // CLH_RESTART_* (ctl_restart_due)
+    float  restart_factor;  // CLH_RESTART_SEQ_MIN: growth factor over the sequence's minimum
+    double seq_min_r2;      // smallest residual^2 of the current sequence (-1: none yet)
     // host-visible copies of this block (pinned, coherent): after check c
     // the reduction writes the block to mirror[c & 1], so the host reads the
     // control state of a chunk without a copy on the stream (null: no copies)
@@ -83,6 +86,7 @@ struct ClothCtl {
     double best_rel2;
     double stall_ref;
     double r2, b2;      // residual^2 and ||b||^2 at the cloth's last check
+    double seq_min_r2;  // smallest residual^2 of the cloth's current sequence (-1: none yet)
 };
 
 // Device buffer owned by the solver.

@@ -41,6 +41,7 @@ struct FusedIterArgs {
     FrameCtl* ctl;         // when non-null, omega and blend come from ctl->slot[par]
     int   par;
     ClothCtl* cctl;        // batches run to a tolerance: per-cloth control (with ctl), or null
+    float relax;           // PD: scale gamma of the Jacobi step (0 means 1)
 };
 
 struct FusedPlan {

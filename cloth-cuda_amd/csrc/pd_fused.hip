@@ -187,22 +187,30 @@ __device__ __forceinline__ int xcd_tile(int b, int n)
 // does not depend on what the compiler does with the surrounding code.
 
 // PD update of a free vertex: residual r = f + m (dy - d) of the global
-// system at the displacement d, the Jacobi step d + r / D, then the Chebyshev
-// blend with d_{k-1} when `blend`.
+// system at the displacement d, the Jacobi step d + gamma r / D (gamma = 1:
+// plain Jacobi; below 1 it pulls the modes with Jacobi eigenvalues below -1,
+// which stiff bending produces, back inside the interval the Chebyshev
+// weights assume), then the Chebyshev blend with d_{k-1} when `blend`.
 __device__ __forceinline__ d3 pd_update(f3 ar, float om, float odg, f3 oy, d3 od, d3 oxp,
-                                        float omega, int blend, d3& r)
+                                        float omega, int blend, float gamma, d3& r)
 {
 #pragma clang fp contract(off)
     const double md = om;
     r = d3{fma(md, (double)oy.x - od.x, (double)ar.x), fma(md, (double)oy.y - od.y, (double)ar.y),
            fma(md, (double)oy.z - od.z, (double)ar.z)};
-    const double inv = (odg > 1e-10f) ? 1.0 / (double)odg : 0.0;
+    const double inv = (odg > 1e-10f) ? (double)gamma / (double)odg : 0.0;
     d3 xn = d3{fma(r.x, inv, od.x), fma(r.y, inv, od.y), fma(r.z, inv, od.z)};
     if (blend) {
         const double w = omega;
         xn = d3{fma(xn.x - oxp.x, w, oxp.x), fma(xn.y - oxp.y, w, oxp.y), fma(xn.z - oxp.z, w, oxp.z)};
     }
     return xn;
+}
+
+// The Jacobi step scale of a launch (0 in the arguments means 1).
+__device__ __forceinline__ float relax_gamma(const FusedIterArgs& a)
+{
+    return a.relax > 0.0f ? a.relax : 1.0f;
 }
 
 // acc + kw * v, component-wise, fused.
@@ -640,7 +648,7 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
             st3(static_cast<float*>(a.out), g, madd_nc(ar, om, oy) * inv);
         } else {
             d3 r;
-            const d3 xn = pd_update(ar, om, odg, oy, od, oxp, a.omega, a.blend, r);
+            const d3 xn = pd_update(ar, om, odg, oy, od, oxp, a.omega, a.blend, relax_gamma(a), r);
             st3d(static_cast<double*>(a.out), g, xn);
             if (kCheck) {
                 pd_residual_terms(r, ab, om, ob, oy, rr, bb);
@@ -919,7 +927,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             x1 = to_d(ld3(a.pin_target, opin)) - to_d(ob);
         } else {
             d3 r;
-            x1 = pd_update(ar1, om, odg, oy, od, oxp, c0.omega, c0.blend, r);
+            x1 = pd_update(ar1, om, odg, oy, od, oxp, c0.omega, c0.blend, relax_gamma(a), r);
         }
         if (own) st3d(static_cast<double*>(a.out), g, x1);
         stsX(sX, L, tid, to_d(ob) + x1);
@@ -1002,7 +1010,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             x2 = to_d(ld3(a.pin_target, opin)) - to_d(ob);
         } else {
             d3 r;
-            x2 = pd_update(ar, om, odg, oy, x1, od, c1.omega, c1.blend, r);
+            x2 = pd_update(ar, om, odg, oy, x1, od, c1.omega, c1.blend, relax_gamma(a), r);
             if (kCheck) {
                 pd_residual_terms(r, ab, om, ob, oy, rr, bb);
             }
@@ -1264,7 +1272,7 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
             } else {
                 const f3 oy = lds3(soy, tid);
                 d3 r;
-                xn = pd_update(ar, om, odg, oy, od, oxp, cs.omega, cs.blend, r);
+                xn = pd_update(ar, om, odg, oy, od, oxp, cs.omega, cs.blend, relax_gamma(a), r);
                 if (check) {
                     pd_residual_terms(r, ab, om, ob, oy, rr, bb);
                 }

@@ -414,6 +414,21 @@ __device__ __forceinline__ bool stall_update(double r2, double b2, double& best_
     return limit > 0 && ++since_drop >= limit;
 }
 
+// Restart test of a residual check (clh_config.restart_rule):
+// CLH_RESTART_PREV, the residual grew since the previous check;
+// CLH_RESTART_SEQ_MIN, it exceeds `factor` times the smallest residual of the
+// current sequence (since the frame began or the last restart), so the
+// oscillation of an early Chebyshev sequence below that factor is left alone;
+// CLH_RESTART_NONE, never. r2 values are squared norms.
+__device__ __forceinline__ bool ctl_restart_due(int rule, float factor, double r2, double last_r2,
+                                                double seq_min_r2)
+{
+    if (rule == CLH_RESTART_NONE) return false;
+    if (rule == CLH_RESTART_SEQ_MIN)
+        return seq_min_r2 >= 0.0 && r2 > (double)factor * (double)factor * seq_min_r2;
+    return last_r2 >= 0.0 && r2 > last_r2;
+}
+
 __device__ __forceinline__ void ctl_write_next(FrameCtl* c, int par, const CtlSlot& s)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
@@ -433,8 +448,8 @@ __device__ __forceinline__ bool ctl_begin(FrameCtl* c, int par, float& omega, in
 
 // After the residual of a check iteration has been summed: history, stop on
 // convergence (`conv`: every cloth meets the tolerance) or a non-finite
-// residual, and a Chebyshev restart when the residual grew since the previous
-// check. Every third restart in a frame doubles 1 - rho (a less aggressive
+// residual, and a Chebyshev restart when the restart rule (ctl_restart_due)
+// calls for one. Every third restart in a frame doubles 1 - rho (a less aggressive
 // sequence, against the oscillation the nonlinear local step can feed; a rho
 // below the Jacobi spectral radius is not repaired by it), down to rho = 0,
 // where every weight is 1 and the iteration is plain Jacobi. Without the
@@ -465,7 +480,8 @@ __device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, doubl
         c->stop = 1;
         c->iters = k + 1;
     } else {
-        if (c->use_cheb && c->last_r2 >= 0.0 && r2 > c->last_r2) {
+        if (c->use_cheb &&
+            ctl_restart_due(c->restart_rule, c->restart_factor, r2, c->last_r2, c->seq_min_r2)) {
             CtlSlot n;
             n.ck = 0;
             n.omega = 1.0f;
@@ -474,6 +490,9 @@ __device__ __forceinline__ void ctl_check(FrameCtl* c, int par, double r2, doubl
             c->slot[par ^ 1] = n;
             c->restarts += 1;
             if (c->restarts % 3 == 0) c->rho = fmaxf(0.0f, 1.0f - 2.0f * (1.0f - c->rho));
+            c->seq_min_r2 = r2;
+        } else if (c->seq_min_r2 < 0.0 || r2 < c->seq_min_r2) {
+            c->seq_min_r2 = r2;
         }
         c->last_r2 = r2;
     }
@@ -503,7 +522,8 @@ __device__ __forceinline__ void cloth_check(ClothCtl* q, const FrameCtl* c, int 
         q->state = CLH_CLOTH_STALLED;
         q->iters = k + 1;
     } else {
-        if (c->use_cheb && q->last_r2 >= 0.0 && r2 > q->last_r2) {
+        if (c->use_cheb &&
+            ctl_restart_due(c->restart_rule, c->restart_factor, r2, q->last_r2, q->seq_min_r2)) {
             CtlSlot n;
             n.ck = 0;
             n.omega = 1.0f;
@@ -512,6 +532,9 @@ __device__ __forceinline__ void cloth_check(ClothCtl* q, const FrameCtl* c, int 
             q->slot[par ^ 1] = n;
             q->restarts += 1;
             if (q->restarts % 3 == 0) q->rho = fmaxf(0.0f, 1.0f - 2.0f * (1.0f - q->rho));
+            q->seq_min_r2 = r2;
+        } else if (q->seq_min_r2 < 0.0 || r2 < q->seq_min_r2) {
+            q->seq_min_r2 = r2;
         }
         q->last_r2 = r2;
     }

@@ -231,6 +231,7 @@ struct VertexArgs {
     int   blend;          // apply the Chebyshev blend (PD)
     FrameCtl* ctl;        // when non-null, omega and blend come from ctl->slot[par]
     int   par;
+    float relax;          // PD: scale gamma of the Jacobi step (0 means 1)
 };
 
 // Reference mode: rhs = sum corner forces + sum k h^2 w p + m y,
@@ -347,7 +348,8 @@ __global__ void __launch_bounds__(kBlock) k_vertex_pd(VertexArgs a)
                             (double)ar.y + md * ((double)yv.y - xv.y),
                             (double)ar.z + md * ((double)yv.z - xv.z)};
             const float d = a.diag[v];
-            const double inv = (d > 1e-10f) ? 1.0 / (double)d : 0.0;
+            const float gamma = a.relax > 0.0f ? a.relax : 1.0f;
+            const double inv = (d > 1e-10f) ? (double)gamma / (double)d : 0.0;
             d3 xn = xv + r * inv;
             if (a.blend) {
                 const d3 xp = ld3d(static_cast<const double*>(a.xkm1), v);
@@ -537,12 +539,14 @@ k_cloth_ctl_init(ClothCtl* __restrict__ q, int C, float rho)
     z.stall_ref = 1.0e300;
     z.r2 = 0.0;
     z.b2 = 0.0;
+    z.seq_min_r2 = -1.0;
     q[c] = z;
 }
 
 // Starts the control state of a frame run to a tolerance.
 __global__ void k_ctl_frame_init(FrameCtl* __restrict__ c, float rho, int use_cheb, int delay,
-                                 double tol2, FrameCtl* mirror, int stall_checks)
+                                 double tol2, FrameCtl* mirror, int stall_checks,
+                                 int restart_rule, float restart_factor)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     FrameCtl f;
@@ -568,6 +572,9 @@ __global__ void k_ctl_frame_init(FrameCtl* __restrict__ c, float rho, int use_ch
     f.best_rel2 = 1.0e300;
     f.stall_ref = 1.0e300;
     f.stalled = 0;
+    f.restart_rule = restart_rule;
+    f.restart_factor = restart_factor;
+    f.seq_min_r2 = -1.0;
     f.mirror = mirror;
     *c = f;
 }
@@ -610,6 +617,50 @@ k_trace(const void* __restrict__ it, const float* __restrict__ xprev, float* __r
     st3(out, v,
         disp ? to_f(to_d(ld3(xprev, v)) + ld3d(static_cast<const double*>(it), v))
              : ld3(static_cast<const float*>(it), v));
+}
+
+// Gershgorin bound of the Jacobi-preconditioned global matrix D^-1 K, with
+// K = M + h^2 (sum_t k_t A_t G_t^T G_t + sum_e k_e w_e w_e^T) (G_t the 2 x 3
+// gradient rows of triangle t from Dm^-1): every eigenvalue of D^-1 K is at
+// most max_v (sum_j |K_vj|) / D_v, and each element's block bounds its share
+// of the row, |K_vj| <= sum over elements of |K^e_vj|. One thread per vertex
+// sums its triangle corners and hinge slots in ascending order, so the row is
+// the same on every run; the maximum of non-negative doubles is exact in any
+// order, taken with an integer atomic max on their bits.
+__global__ void __launch_bounds__(kBlock)
+k_jacobi_bound(const int* __restrict__ vt_ptr, const int* __restrict__ vt_idx,
+               const int* __restrict__ ve_ptr, const int* __restrict__ ve_idx,
+               const float* __restrict__ mass, const float* __restrict__ diag,
+               const float* __restrict__ Dm_inv, const float* __restrict__ rest_area,
+               const float* __restrict__ stretch_k, const float* __restrict__ bend_w,
+               const float* __restrict__ bend_k, int N, double h2, int use_stretch,
+               int use_bend, unsigned long long* __restrict__ out)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const double d = diag[v];
+    if (!(d > 1e-10)) return;
+    double row = fabs((double)mass[v]);
+    if (use_stretch)
+        for (int j = vt_ptr[v]; j < vt_ptr[v + 1]; ++j) {
+            const int t = vt_idx[j] / 3, c = vt_idx[j] % 3;
+            const double g00 = Dm_inv[4 * (size_t)t], g10 = Dm_inv[4 * (size_t)t + 1];
+            const double g01 = Dm_inv[4 * (size_t)t + 2], g11 = Dm_inv[4 * (size_t)t + 3];
+            const double D0[3] = {-g00 - g10, g00, g10}, D1[3] = {-g01 - g11, g01, g11};
+            double sum = 0.0;
+            for (int b = 0; b < 3; ++b) sum += fabs(D0[c] * D0[b] + D1[c] * D1[b]);
+            row += fabs(h2 * (double)stretch_k[t] * (double)rest_area[t]) * sum;
+        }
+    if (use_bend)
+        for (int j = ve_ptr[v]; j < ve_ptr[v + 1]; ++j) {
+            const int e = ve_idx[j] >> 2, i = ve_idx[j] & 3;
+            const float* w = bend_w + 4 * (size_t)e;
+            const double sw = fabs((double)w[0]) + fabs((double)w[1]) + fabs((double)w[2]) +
+                              fabs((double)w[3]);
+            row += fabs(h2 * (double)bend_k[e]) * fabs((double)w[i]) * sw;
+        }
+    const double ratio = row / d;
+    atomicMax(out, (unsigned long long)__double_as_longlong(ratio));
 }
 
 // ---------------------------------------------------------------------------
@@ -705,9 +756,18 @@ struct clh_solver {
     const int* pins_checked_ptr = nullptr;
     int pins_checked_n = -1;
 
+    // Jacobi step scale in use (clh_config.relax resolved; CLH_RELAX_AUTO:
+    // from the Gershgorin bound of D^-1 K of the constants in relax_args)
+    float relax_eff = 1.0f;
+    double jacobi_bound = 0.0;
+    bool relax_ready = false;
+    clh_step_args relax_args = {};
+
     // statistics of the last frame
     int last_iters = 0;
     int last_checks = 0;
+    int last_restarts = 0;
+    float last_rho = 0.0f;
     int converged = 0;
     int frames = 0;
 };
@@ -963,6 +1023,7 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
         f.ctl = ctl;
         f.par = par;
         f.cctl = (ctl && s->num_cloths > 1) ? s->cctl.ptr : nullptr;
+        f.relax = s->relax_eff;
         CLH_TRY(fused_launch_iteration(s->fused, f, c.mode, s->stream));
         if (check) CLH_TRY(enqueue_fused_reduce(s, check_slot, ctl, par, 0));
         return hip_status(hipGetLastError(), "fused iteration launch");
@@ -1017,6 +1078,7 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
     va.blend = blend ? 1 : 0;
     va.ctl = ctl;
     va.par = k & 1;
+    va.relax = s->relax_eff;
     if (s->N > 0) {
         if (pd)
             hipLaunchKernelGGL(k_vertex_pd, dim3(s->nblocks_v), dim3(kBlock), 0, s->stream, va);
@@ -1059,6 +1121,7 @@ int enqueue_pair(clh_solver* s, const clh_step_args& a, int k, float omega0, boo
     f.ctl = ctl;
     f.par = par;
     f.cctl = nullptr;  // batches to a tolerance run one iteration per launch
+    f.relax = s->relax_eff;
     const int blend1 = (c.use_chebyshev && blend1_in && omega1 != 1.0f) ? 1 : 0;
     CLH_TRY(fused_launch_pair(s->fused, f, s->it[(k + 2) % kIterBufs].ptr, omega1, blend1,
                               s->stream));
@@ -1145,10 +1208,57 @@ int validate_step_args(int N, const clh_step_args* a)
 
 int validate_args(const clh_solver* s, const clh_step_args* a) { return validate_step_args(s->N, a); }
 
+// Resolves the Jacobi step scale of the solver's frames. With CLH_RELAX_AUTO
+// in PD mode, gamma = min(1, (1 + rho) / mu) with mu the Gershgorin bound of
+// D^-1 K (k_jacobi_bound): the damped Jacobi iteration matrix
+// I - gamma D^-1 K then has every eigenvalue in [-rho, 1), inside the
+// interval [-rho, rho] the Chebyshev weights assume at the bottom, where stiff
+// bending otherwise puts eigenvalues below -1 that the weights amplify. The
+// bound is recomputed when the constants' pointers, dt or rho change.
+int ensure_relax(clh_solver* s, const clh_step_args& a)
+{
+    const clh_config& c = s->cfg;
+    if (!(c.relax < 0.0f && c.mode == CLH_MODE_PD)) {
+        s->relax_eff = c.relax > 0.0f ? c.relax : 1.0f;
+        return CLH_OK;
+    }
+    const clh_step_args& r = s->relax_args;
+    if (s->relax_ready && r.d_mass == a.d_mass && r.d_jacobi_diag == a.d_jacobi_diag &&
+        r.d_Dm_inv == a.d_Dm_inv && r.d_rest_area == a.d_rest_area &&
+        r.d_stretch_k == a.d_stretch_k && r.d_bend_w == a.d_bend_w && r.d_bend_k == a.d_bend_k &&
+        r.d_bend_quads == a.d_bend_quads && r.d_tris == a.d_tris)
+        return CLH_OK;
+    const bool use_stretch = s->T > 0 && a.d_stretch_k != nullptr;
+    const bool use_bend = s->E > 0 && a.d_bend_quads != nullptr;
+    DevBuf<unsigned long long> mx;
+    CLH_TRY(mx.resize(1));
+    CLH_HIP(hipMemsetAsync(mx.ptr, 0, sizeof(unsigned long long), s->stream));
+    if (s->N > 0)
+        hipLaunchKernelGGL(k_jacobi_bound, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream,
+                           s->vt_ptr.ptr, s->vt_idx.ptr, s->ve_ptr.ptr, s->ve_idx.ptr, a.d_mass,
+                           a.d_jacobi_diag, a.d_Dm_inv, a.d_rest_area, a.d_stretch_k, a.d_bend_w,
+                           a.d_bend_k, s->N, (double)c.dt * (double)c.dt, use_stretch ? 1 : 0,
+                           use_bend ? 1 : 0, mx.ptr);
+    CLH_HIP(hipGetLastError());
+    unsigned long long bits = 0;
+    CLH_HIP(hipMemcpyAsync(&bits, mx.ptr, sizeof(bits), hipMemcpyDeviceToHost, s->stream));
+    CLH_HIP(hipStreamSynchronize(s->stream));
+    double mu = 0.0;
+    std::memcpy(&mu, &bits, sizeof(mu));
+    if (!std::isfinite(mu)) return fail(CLH_ERR_NUMERIC, "clh_solver_step: non-finite constants");
+    s->jacobi_bound = mu;
+    const double rho = std::min(std::max((double)c.rho, 0.0), 1.0);
+    s->relax_eff = (mu > 1.0 + rho) ? (float)((1.0 + rho) / mu) : 1.0f;
+    s->relax_args = a;
+    s->relax_ready = true;
+    return CLH_OK;
+}
+
 int step_impl(clh_solver* s, const clh_step_args& a)
 {
     CLH_TRY(ensure_plan(s, a));
     CLH_TRY(ensure_scratch(s));
+    CLH_TRY(ensure_relax(s, a));
     if (a.num_pinned > 0 &&
         (s->pins_checked_ptr != a.d_pin_idx || s->pins_checked_n != a.num_pinned)) {
         // pin indices address the vertex arrays: check each new list once
@@ -1212,7 +1322,8 @@ int step_impl(clh_solver* s, const clh_step_args& a)
     hipLaunchKernelGGL(k_ctl_frame_init, dim3(1), dim3(64), 0, s->stream, s->ctl.ptr, c.rho,
                        c.use_chebyshev ? 1 : 0, c.cheb_delay,
                        (double)c.tolerance * (double)c.tolerance,
-                       mirrored ? s->d_ctl_mirror : nullptr, c.stall_checks);
+                       mirrored ? s->d_ctl_mirror : nullptr, c.stall_checks, c.restart_rule,
+                       c.restart_factor);
     CLH_HIP(hipGetLastError());
     // a batch: every cloth has its own control (fused variant)
     s->last_cloth_ctl = s->num_cloths > 1 && s->variant == CLH_KERNEL_FUSED;
@@ -1251,6 +1362,7 @@ int step_impl(clh_solver* s, const clh_step_args& a)
             f.pin_target = a.d_pin_target;
             f.use_stretch = (s->T > 0 && a.d_stretch_k != nullptr) ? 1 : 0;
             f.use_bend = (s->E > 0 && a.d_bend_quads != nullptr) ? 1 : 0;
+            f.relax = s->relax_eff;
             double* const its[kIterBufs] = {s->it[0].ptr, s->it[1].ptr, s->it[2].ptr, s->it[3].ptr};
             CLH_TRY(fused_launch_persist(s->fused, f, its, k0, k1, K, P, s->ctl.ptr,
                                          s->partials.ptr, s->hist.ptr, s->bar.ptr, s->stream));
@@ -1283,6 +1395,8 @@ int step_impl(clh_solver* s, const clh_step_args& a)
     const int nchunks = (K + P - 1) / P;
     int iters = K, checks = 0;
     s->converged = 0;
+    s->last_restarts = 0;
+    s->last_rho = c.rho;
     bool nonfinite = false;
     if (nchunks > 0) CLH_TRY(issue(0));
     if (nchunks > 1) CLH_TRY(issue(1));
@@ -1291,6 +1405,8 @@ int step_impl(clh_solver* s, const clh_step_args& a)
         const FrameCtl h = s->h_ctl[ci & 1];
         if (h.barrier_error) return fail(CLH_ERR_DEVICE, "persistent kernel: a grid barrier timed out");
         checks = h.checks;
+        s->last_restarts = h.restarts;
+        s->last_rho = h.rho;
         if (h.stop) {
             iters = h.iters;
             s->converged = h.converged;
@@ -1344,6 +1460,13 @@ extern "C" int clh_solver_create(const clh_config* cfg, const clh_mesh_desc* mes
     if (!(cfg->dt > 0.0f)) return fail(CLH_ERR_INVALID, "clh_solver_create: dt must be > 0");
     if (cfg->kernel_variant < CLH_KERNEL_AUTO || cfg->kernel_variant > CLH_KERNEL_FUSED)
         return fail(CLH_ERR_INVALID, "clh_solver_create: unknown kernel variant");
+    if (!((cfg->relax >= 0.0f && cfg->relax < 2.0f) || cfg->relax == CLH_RELAX_AUTO))
+        return fail(CLH_ERR_INVALID, "clh_solver_create: relax must be in [0, 2) or CLH_RELAX_AUTO");
+    if (cfg->restart_rule < CLH_RESTART_PREV || cfg->restart_rule > CLH_RESTART_NONE)
+        return fail(CLH_ERR_INVALID, "clh_solver_create: unknown restart rule");
+    if (cfg->restart_rule == CLH_RESTART_SEQ_MIN && !(cfg->restart_factor >= 1.0f &&
+                                                      cfg->restart_factor < 1e30f))
+        return fail(CLH_ERR_INVALID, "clh_solver_create: restart_factor must be finite and >= 1");
     clh_solver* s = new clh_solver();
     s->cfg = *cfg;
     if (s->cfg.check_every <= 0) s->cfg.check_every = 1;
@@ -1451,6 +1574,10 @@ extern "C" int clh_solver_get_stats(clh_solver* s, clh_step_stats* out)
     out->iterations = s->last_iters;
     out->converged = s->converged;
     out->frames = s->frames;
+    out->restarts = s->last_cloth_ctl ? 0 : s->last_restarts;
+    out->rho = s->last_cloth_ctl ? s->cfg.rho : s->last_rho;
+    out->relax = s->relax_eff;
+    out->jacobi_bound = s->jacobi_bound;
     if (s->cfg.mode == CLH_MODE_PD && s->last_checks > 0 && s->hist.ptr) {
         std::vector<double> h((size_t)s->last_checks * 2);
         CLH_HIP(hipMemcpy(h.data(), s->hist.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1962,6 +2089,10 @@ extern "C" int clh_step_noalloc(const clh_config* cfg, int num_verts, int num_tr
     if (cfg->max_iterations < 0 || cfg->max_iterations > kMaxIterations)
         return fail(CLH_ERR_INVALID, "clh_step_noalloc: max_iterations out of range");
     if (!(cfg->dt > 0.0f)) return fail(CLH_ERR_INVALID, "clh_step_noalloc: dt must be > 0");
+    if (!(cfg->relax >= 0.0f && cfg->relax < 2.0f))
+        return fail(CLH_ERR_INVALID,
+                    "clh_step_noalloc: relax must be in [0, 2) (no CLH_RELAX_AUTO: its bound "
+                    "takes a reduction buffer)");
     if (cfg->mode == CLH_MODE_PD && cfg->tolerance > 0.0f)
         return fail(CLH_ERR_UNSUPPORTED,
                     "clh_step_noalloc: fixed-length frames only (tolerance must be 0)");
@@ -2075,6 +2206,7 @@ extern "C" int clh_step_noalloc(const clh_config* cfg, int num_verts, int num_tr
         va.blend = blend ? 1 : 0;
         va.ctl = nullptr;
         va.par = k & 1;
+        va.relax = c.relax;
         if (pd)
             hipLaunchKernelGGL(k_vertex_pd, dim3(grid_for(N)), dim3(kBlock), 0, st, va);
         else

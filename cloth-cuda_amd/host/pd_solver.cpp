@@ -34,6 +34,9 @@ PDSolver::PDSolver(const PDSolverConfig& config, const ClothMesh& mesh,
     for (int k = 0; k < 3; ++k) c.ext_accel[k] = config.external_accel[k];
     c.kernel_variant = config.kernel_variant;
     c.use_graph      = config.use_graph ? 1 : 0;
+    c.relax          = config.relax;
+    c.restart_rule   = config.restart_rule;
+    c.restart_factor = config.restart_factor;
 
     clh_mesh_desc d;
     d.num_verts    = num_verts_;

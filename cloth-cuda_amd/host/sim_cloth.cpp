@@ -64,6 +64,9 @@ void print_usage(const char* prog)
     std::fprintf(stderr, "  --mode <m>       ref | pd                (default: ref)\n");
     std::fprintf(stderr, "  --tol <r>        pd: stop at this relative FP64 residual (default: 0 = fixed)\n");
     std::fprintf(stderr, "  --rho <r>        pd: Chebyshev spectral radius (default: 0.9)\n");
+    std::fprintf(stderr, "  --relax <g|auto> pd: Jacobi step scale; auto from the Gershgorin bound (default: auto)\n");
+    std::fprintf(stderr, "  --cheb-delay <n> pd: plain Jacobi iterations before the blend (default: 10)\n");
+    std::fprintf(stderr, "  --restart <r>    pd: prev | seqmin | none (default: seqmin, tenfold growth)\n");
     std::fprintf(stderr, "  --obj <path>     mesh from an OBJ file instead of the grid\n");
     std::fprintf(stderr, "  --energy         with --verbose: stretch, bend, kinetic, potential terms\n");
 }
@@ -96,6 +99,11 @@ int main(int argc, char* argv[])
     std::string stretch_backend = "gpu-current", export_dir, mode = "ref", obj;
     bool verbose = false, energy = false;
     float tol = 0.0f, rho = 0.9f;
+    // pd: the damped step from the Gershgorin bound of D^-1 K, the Chebyshev
+    // blend after 10 plain Jacobi iterations (Wang 2015), and a restart only
+    // when the residual has grown tenfold over the sequence's best
+    std::string relax = "auto", restart = "seqmin";
+    int cheb_delay = 10;
     for (int i = 4; i < argc; ++i) {
         const std::string a = argv[i];
         const bool more = i + 1 < argc;
@@ -114,6 +122,9 @@ int main(int argc, char* argv[])
         else if (a == "--mode" && more) mode = argv[++i];
         else if (a == "--tol" && more) tol = std::stof(argv[++i]);
         else if (a == "--rho" && more) rho = std::stof(argv[++i]);
+        else if (a == "--relax" && more) relax = argv[++i];
+        else if (a == "--cheb-delay" && more) cheb_delay = std::stoi(argv[++i]);
+        else if (a == "--restart" && more) restart = argv[++i];
         else if (a == "--obj" && more) obj = argv[++i];
         else if (a == "--energy") energy = true;
     }
@@ -123,6 +134,10 @@ int main(int argc, char* argv[])
         return 1;
     }
     const bool use_cpu_ref = stretch_backend == "cpu-ref";
+    if (restart != "prev" && restart != "seqmin" && restart != "none") {
+        std::fprintf(stderr, "unknown restart rule '%s' (prev | seqmin | none)\n", restart.c_str());
+        return 1;
+    }
     if (mode != "ref" && mode != "pd") {
         std::fprintf(stderr, "unknown mode '%s' (ref | pd)\n", mode.c_str());
         return 1;
@@ -175,7 +190,11 @@ int main(int argc, char* argv[])
     config.rho = rho;
     config.mode = mode == "pd" ? 1 : 0;
     config.tolerance = tol;
-    config.cheb_delay = 1;
+    config.cheb_delay = cheb_delay;
+    config.relax = relax == "auto" ? CLH_RELAX_AUTO : std::stof(relax);
+    config.restart_rule = restart == "prev" ? CLH_RESTART_PREV
+                        : restart == "none" ? CLH_RESTART_NONE : CLH_RESTART_SEQ_MIN;
+    config.restart_factor = 10.0f;
     PDSolver* gpu_solver = nullptr;
     CpuStretchReferenceSolver* cpu_ref_solver = nullptr;
     if (use_cpu_ref)

@@ -132,7 +132,32 @@ typedef struct clh_config {
                                (below 0.9 of its value at the last such drop) for this
                                many checks in a row stops, not converged, instead of
                                running to max_iterations; 0 = off */
+    int   restart_rule;     /* CLH_MODE_PD with tolerance > 0: when a residual check
+                               restarts the Chebyshev sequence (CLH_RESTART_*) */
+    float restart_factor;   /* CLH_RESTART_SEQ_MIN: restart when the residual exceeds this
+                               factor (> 1) times the smallest residual of the current
+                               sequence */
+    float relax;            /* CLH_MODE_PD: scale gamma of the Jacobi step,
+                               x + gamma D^-1 (b - K x), in (0, 2); 0 means 1 (plain
+                               Jacobi). Below 1 it damps the Jacobi modes with
+                               eigenvalues below -1 that stiff bending produces.
+                               CLH_RELAX_AUTO: gamma = min(1, (1 + rho) / mu), mu the
+                               Gershgorin bound of D^-1 K computed on the device from
+                               the constants, so every eigenvalue of the damped Jacobi
+                               iteration lies in [-rho, 1) */
 } clh_config;
+
+#define CLH_RELAX_AUTO (-1.0f)
+
+/* Chebyshev restart rules (clh_config.restart_rule). Every third restart of a
+ * frame (of a cloth, in a batch) doubles 1 - rho, down to rho = 0 (plain
+ * Jacobi), under each rule that restarts. */
+enum {
+    CLH_RESTART_PREV    = 0,  /* the residual grew since the previous check (default) */
+    CLH_RESTART_SEQ_MIN = 1,  /* the residual exceeds restart_factor times the smallest
+                                 residual since the sequence (re)started */
+    CLH_RESTART_NONE    = 2   /* no restarts */
+};
 
 /* Per-cloth state of a batch frame run to a tolerance (clh_batch_cloth_status). */
 enum {
@@ -211,6 +236,10 @@ typedef struct clh_step_stats {
     double residual0;    /* first FP64 ||b - K x|| of the frame              */
     double rhs_norm;     /* FP64 ||b|| at the last residual evaluation        */
     int    frames;       /* frames stepped since creation or reset           */
+    int    restarts;     /* Chebyshev restarts of the last frame (one solve; 0 in a batch) */
+    float  rho;          /* rho in use when the last frame ended (one solve) */
+    float  relax;        /* Jacobi step scale gamma of the last frame */
+    double jacobi_bound; /* CLH_RELAX_AUTO: the Gershgorin bound of D^-1 K it used (else 0) */
 } clh_step_stats;
 
 /* FP64 relative residual ||b - K x_0|| / ||b|| of the frame's first iterate

@@ -24,6 +24,10 @@ struct PDSolverConfig {
     int   cheb_delay         = 0;     // PD mode: iterations before Chebyshev starts
     int   kernel_variant     = 0;     // 0 auto, 1 unfused, 2 fused
     bool  use_graph          = true;  // capture fixed-length frames in a hipGraph
+    float relax              = 0.0f;  // PD mode: Jacobi step scale gamma (0 = 1; -1 = from the
+                                      // Gershgorin bound of D^-1 K, CLH_RELAX_AUTO)
+    int   restart_rule       = 0;     // PD mode with a tolerance: CLH_RESTART_* (0 = on any growth)
+    float restart_factor     = 10.0f; // CLH_RESTART_SEQ_MIN: growth factor over the sequence's best
 };
 
 struct ClothMesh;

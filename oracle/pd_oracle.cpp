@@ -525,6 +525,7 @@ int step(const or_mesh* mp, const or_params* pp, R* pos, R* vel, const R* init, 
         int ck = 0;               // iterations since the last Chebyshev (re)start
         int restarts = 0;
         double last_r2 = -1.0;    // residual at the previous check
+        double seq_min = -1.0;    // smallest residual of the current sequence
         or_params q = p;          // rho is lowered after every third restart
         for (int k = 0; k < p.iterations; ++k) {
             omf = cheb_omega<float>(q, ck, omf);
@@ -547,12 +548,22 @@ int step(const or_mesh* mp, const or_params* pp, R* pos, R* vel, const R* init, 
             ++iters;
             if (p.tolerance > 0 && ((k + 1) % every == 0)) {
                 if (r2 <= tol2 * b2) break;
-                // Chebyshev restart when the residual grew since the last
-                // test; every third restart doubles 1 - rho, down to rho = 0
-                // (plain Jacobi)
-                if (p.use_chebyshev && last_r2 >= 0.0 && r2 > last_r2) {
+                // Chebyshev restart when the rule calls for one (by default
+                // when the residual grew since the last test); every third
+                // restart doubles 1 - rho, down to rho = 0 (plain Jacobi)
+                bool restart = false;
+                if (p.restart_rule == OR_RESTART_SEQ_MIN) {
+                    const double f = (double)p.restart_factor;
+                    restart = seq_min >= 0.0 && r2 > f * f * seq_min;
+                } else if (p.restart_rule != OR_RESTART_NONE) {
+                    restart = last_r2 >= 0.0 && r2 > last_r2;
+                }
+                if (p.use_chebyshev && restart) {
                     ck = 0;
                     if (++restarts % 3 == 0) q.rho = std::max(0.0f, 1.0f - 2.0f * (1.0f - q.rho));
+                    seq_min = r2;
+                } else if (seq_min < 0.0 || r2 < seq_min) {
+                    seq_min = r2;
                 }
                 last_r2 = r2;
             }

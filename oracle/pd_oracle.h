@@ -66,9 +66,16 @@ typedef struct or_params {
     int   exact_pins;      /* OR_EXACT: eliminate pins correctly (reference: 0)   */
     int   threads;         /* OpenMP threads for OR_PD / OR_REF_CUDA (<= 0: 1)    */
     float relax;           /* OR_PD: under-relaxation gamma of the Jacobi step    */
-    /* OR_PD with a tolerance restarts the Chebyshev sequence whenever the
-     * residual at a test exceeds the one at the previous test. */
+    int   restart_rule;    /* OR_PD with a tolerance: OR_RESTART_*                 */
+    float restart_factor;  /* OR_RESTART_SEQ_MIN: growth factor over the minimum   */
+    /* OR_PD with a tolerance restarts the Chebyshev sequence at a test when the
+     * rule says so: OR_RESTART_PREV, the residual exceeds the one at the
+     * previous test; OR_RESTART_SEQ_MIN, it exceeds restart_factor times the
+     * smallest residual since the sequence (re)started; OR_RESTART_NONE, never.
+     * Every third restart doubles 1 - rho, down to rho = 0. */
 } or_params;
+
+enum { OR_RESTART_PREV = 0, OR_RESTART_SEQ_MIN = 1, OR_RESTART_NONE = 2 };
 
 void or_params_default(or_params* p);
 

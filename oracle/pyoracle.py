@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
 
 REF_CUDA, PD, EXACT = 0, 1, 2
+RESTART_PREV, RESTART_SEQ_MIN, RESTART_NONE = 0, 1, 2
 
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
@@ -32,7 +33,8 @@ class OrParams(C.Structure):
                 ("rho", C.c_float), ("cheb_delay", C.c_int), ("gravity", C.c_float),
                 ("dt", C.c_float), ("damping", C.c_float), ("ext_accel", C.c_float * 3),
                 ("tolerance", C.c_float), ("check_every", C.c_int), ("exact_bend", C.c_int),
-                ("exact_pins", C.c_int), ("threads", C.c_int), ("relax", C.c_float)]
+                ("exact_pins", C.c_int), ("threads", C.c_int), ("relax", C.c_float),
+                ("restart_rule", C.c_int), ("restart_factor", C.c_float)]
 
 
 _lib = None

@@ -78,3 +78,35 @@ def cfg5_specs(count=64, seed=5, n=256, cloth_seed=5000, wind_max=2.0):
         d = rng.normal(size=3)
         accel.append(d / np.linalg.norm(d) * rng.uniform(0.0, wind_max) / CFG2["density"])
     return specs, np.asarray(accel, np.float32)
+
+
+def jacobi_bound_np(arrays, dt, stretch=True, bend=True, signed=False):
+    """Per-element Gershgorin bound of D^-1 K (K = M + h^2 (sum_t k A G^T G +
+    sum_e k w w^T), D the Jacobi diagonal the solver divides by): max over
+    vertices of (m_v + sum over its elements of the absolute entries of the
+    element block's row v) / D_v. signed=True sums the entries with their
+    signs instead (a plausible mistake, for the tests)."""
+    f = (lambda z: z) if signed else np.abs
+    N = len(arrays["mass"])
+    h2 = float(np.float32(dt)) ** 2  # dt is an FP32 setting
+    row = np.abs(arrays["mass"].astype(np.float64))
+    if stretch and len(arrays["tris"]):
+        t = arrays["tris"].reshape(-1, 3)
+        G = arrays["Dm_inv"].reshape(-1, 4).astype(np.float64)
+        g00, g10, g01, g11 = G[:, 0], G[:, 1], G[:, 2], G[:, 3]
+        D0 = np.stack([-g00 - g10, g00, g10], 1)
+        D1 = np.stack([-g01 - g11, g01, g11], 1)
+        w = np.abs(h2 * arrays["stretch_k"].astype(np.float64) * arrays["rest_area"].astype(np.float64))
+        for c in range(3):
+            acc = sum(f(D0[:, c] * D0[:, b] + D1[:, c] * D1[:, b]) for b in range(3))
+            np.add.at(row, t[:, c], w * acc)
+    if bend and len(arrays["bend_quads"]):
+        q = arrays["bend_quads"].reshape(-1, 4)
+        bw = arrays["bend_w"].reshape(-1, 4).astype(np.float64)
+        kb = np.abs(h2 * arrays["bend_k"].astype(np.float64))
+        sw = f(bw).sum(1) if not signed else bw.sum(1)
+        for i in range(4):
+            np.add.at(row, q[:, i], kb * f(bw[:, i]) * sw)
+    d = arrays["jacobi_diag"].astype(np.float64)
+    ok = d > 1e-10
+    return float((row[ok] / d[ok]).max())
