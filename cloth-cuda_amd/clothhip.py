@@ -25,6 +25,7 @@ OK, ERR_INVALID, ERR_DEVICE, ERR_ALLOC, ERR_UNSUPPORTED, ERR_NUMERIC = range(6)
 CLOTH_RUNNING, CLOTH_CONVERGED, CLOTH_STALLED, CLOTH_NONFINITE = range(4)
 PERSIST_OFF, PERSIST_ON = 0, 1
 RESTART_PREV, RESTART_SEQ_MIN, RESTART_NONE = 0, 1, 2
+TUNE_MAX = 40  # CLH_TUNE_MAX_CANDIDATES
 RELAX_AUTO = -1.0
 PIN_NONE, PIN_TOP_ROW, PIN_CORNERS, PIN_RANDOM_BOUNDARY = 0, 1, 2, 3
 MESH_IRREGULAR = 4
@@ -90,9 +91,9 @@ class PlanInfo(C.Structure):
 
 
 class TuneReport(C.Structure):
-    _fields_ = [("num_candidates", C.c_int), ("tile_verts", C.c_int * 32),
-                ("block_threads", C.c_int * 32), ("iters_per_launch", C.c_int * 32),
-                ("element_regions", C.c_int * 32), ("us_per_iteration", C.c_float * 32),
+    _fields_ = [("num_candidates", C.c_int), ("tile_verts", C.c_int * TUNE_MAX),
+                ("block_threads", C.c_int * TUNE_MAX), ("iters_per_launch", C.c_int * TUNE_MAX),
+                ("element_regions", C.c_int * TUNE_MAX), ("us_per_iteration", C.c_float * TUNE_MAX),
                 ("best", C.c_int), ("from_cache", C.c_int), ("key", C.c_ulonglong)]
 
     def candidates(self):

@@ -2238,10 +2238,11 @@ int fused_launch_persist(const FusedPlan& p, const FusedIterArgs& a, double* con
 
 size_t fused_pair_lds_bytes(const FusedPlan& p, bool check) { return pair_lds_layout(p, check).total; }
 
-// the two-iteration kernel is built for 256- and 512-thread workgroups
+// the two-iteration kernel is built for 256-, 512- and 1024-thread
+// workgroups, which must hold a tile's own and ring-1 vertices
 bool fused_pair_usable(const FusedPlan& p)
 {
-    return p.pair && p.threads <= 512 && p.pcmax <= p.threads;
+    return p.pair && p.threads <= 1024 && p.pcmax <= p.threads;
 }
 
 int fused_launch_pair(const FusedPlan& p, const FusedIterArgs& a, void* out2, float omega1,
@@ -2274,7 +2275,10 @@ int fused_launch_pair(const FusedPlan& p, const FusedIterArgs& a, void* out2, fl
     P.ptmax = p.ptmax;
     P.phmax = p.phmax;
     const dim3 grid(p.num_tiles);
-    if (p.threads == 512) {
+    if (p.threads == 1024) {
+        if (check) hipLaunchKernelGGL((k_fused_pair<true, 1024>), grid, dim3(1024), L.total, stream, P);
+        else       hipLaunchKernelGGL((k_fused_pair<false, 1024>), grid, dim3(1024), L.total, stream, P);
+    } else if (p.threads == 512) {
         if (check) hipLaunchKernelGGL((k_fused_pair<true, 512>), grid, dim3(512), L.total, stream, P);
         else       hipLaunchKernelGGL((k_fused_pair<false, 512>), grid, dim3(512), L.total, stream, P);
     } else {
@@ -2385,6 +2389,8 @@ int fused_enable_lds(int& bytes)
     fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<false, 256>));
     fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<true, 512>));
     fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<false, 512>));
+    fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<true, 1024>));
+    fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<false, 1024>));
     int granted = bytes;
     for (const void* f : fns) {
         hipFuncAttributes fa;

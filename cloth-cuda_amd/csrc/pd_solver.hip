@@ -1844,8 +1844,7 @@ extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int nu
         CLH_TRY(replan(s, *a, tv, 256, got));
         if (got == tv) CLH_TRY(warm_up());
         for (int th : threads) {
-            // two-iteration launches use 256- or 512-thread workgroups
-            for (int ipl = 1; ipl <= (try_pairs && th <= 512 ? 2 : 1); ++ipl) {
+            for (int ipl = 1; ipl <= (try_pairs ? 2 : 1); ++ipl) {
                 for (int reg = 1; reg <= (pd && ipl == 1 ? 2 : 1); ++reg) {
                     out->tile_verts[n] = tv;
                     out->block_threads[n] = th;

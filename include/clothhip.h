@@ -112,7 +112,8 @@ typedef struct clh_config {
     int   use_graph;        /* capture fixed-length frames in a hipGraph        */
     int   tile_verts;       /* fused variant: vertices per tile (0 = default)   */
     int   block_threads;    /* fused variant: workgroup size, 256, 512 or 1024 (0 = default;
-                               two-iteration launches need 256 or 512) */
+                               two-iteration launches need a workgroup of at least a
+                               tile's own and ring-1 vertices) */
     int   persistent;       /* CLH_PERSIST_ON: frames to a tolerance in one cooperative
                                launch per chunk (grid barrier between iterations) when
                                every tile fits resident; off by default: on MI355X the
@@ -357,8 +358,9 @@ int clh_batch_cloth_status(clh_solver* s, int* state, int* iterations, int max_c
 /* Autotuner of the fused kernel's launch shape: times every (tile_verts in
  * {32, 64, 128, 256}, block_threads in {256, 512, 1024}; in CLH_MODE_PD
  * element regions in {1, 2} where separate regions fit, and with
- * iters_per_launch = 2 also two iterations per launch at 256 and 512
- * threads; at most 32 candidates) candidate on this
+ * iters_per_launch = 2 also two iterations per launch at every workgroup
+ * size that holds a tile's own and ring-1 vertices; at most 36 candidates)
+ * candidate on this
  * solver's mesh (the lower of two timings of num_launches steady-phase
  * launches each, as timed by clh_solver_time_iteration_kernel, after about
  * 20 ms of warm-up launches per tile size) and keeps the one with the
@@ -366,13 +368,16 @@ int clh_batch_cloth_status(clh_solver* s, int* state, int* iterations, int max_c
  * triangle, hinge and cloth counts, the mode, whether two-iteration launches
  * may be used and the histogram of vertex valences, so another solver on an
  * equally shaped mesh reuses the choice without timing. */
+#define CLH_TUNE_MAX_CANDIDATES 40
+
 typedef struct clh_tune_report {
     int   num_candidates;
-    int   tile_verts[32];
-    int   block_threads[32];
-    int   iters_per_launch[32];
-    int   element_regions[32];
-    float us_per_iteration[32]; /* launch time / iterations; 0 when the candidate did not fit */
+    int   tile_verts[CLH_TUNE_MAX_CANDIDATES];
+    int   block_threads[CLH_TUNE_MAX_CANDIDATES];
+    int   iters_per_launch[CLH_TUNE_MAX_CANDIDATES];
+    int   element_regions[CLH_TUNE_MAX_CANDIDATES];
+    float us_per_iteration[CLH_TUNE_MAX_CANDIDATES]; /* launch time / iterations; 0 when the
+                                                         candidate did not fit */
     int   best;               /* index of the chosen candidate */
     int   from_cache;         /* 1 when the choice came from the cache */
     unsigned long long key;   /* cache key of the mesh shape */

@@ -83,27 +83,28 @@ def test_reference_mode_lane_counts_agree():
 
 
 def test_two_iteration_setting_at_1024_threads():
-    """The two-iteration kernel is built for 256 and 512 threads: with 1024
-    the frames run one iteration per launch, with the same bits as the
-    two-iteration launches at 512."""
+    """The two-iteration kernel runs at 256, 512 and 1024 threads: at 1024 and
+    512 the frames run two iterations per launch, with the bits of one
+    iteration per launch."""
     spec = CASES["irregular24_top"]
     outs = []
-    for th in (1024, 512):
+    for th, ipl in ((1024, 2), (512, 2), (1024, 1)):
         s = ch.Scene(**spec)
         s.upload()
         sol = ch.Solver(ch.config(mode=ch.MODE_PD, gravity=-9.81, dt=spec["dt"], rho=0.99,
                                   cheb_delay=1, kernel_variant=ch.KERNEL_FUSED,
-                                  max_iterations=16, iters_per_launch=2, tile_verts=128,
+                                  max_iterations=16, iters_per_launch=ipl, tile_verts=128,
                                   block_threads=th), s.N, s.T, s.E)
         args = s.step_args()
         for _ in range(2):
             sol.step(args)
         info = sol.plan_info()
         assert info.block_threads == th
-        assert info.iters_per_launch == (1 if th == 1024 else 2)
+        assert info.iters_per_launch == ipl
         outs.append(s.get_state())
-    np.testing.assert_array_equal(outs[0][0], outs[1][0])
-    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    for pos, vel in outs[1:]:
+        np.testing.assert_array_equal(outs[0][0], pos)
+        np.testing.assert_array_equal(outs[0][1], vel)
 
 
 def test_block_threads_validated():

@@ -27,7 +27,7 @@ CASES = {
     "checker40_no_bend": dict(CFG1, nrows=40, ncols=40, size=1 / 39, type=1, bend_k=0.0),
 }
 
-SHAPES = [(64, 256), (128, 256), (128, 512), (256, 512)]
+SHAPES = [(64, 256), (128, 256), (128, 512), (256, 512), (128, 1024), (256, 1024)]
 
 
 def run(spec, ipl, frames, shape=None, accel=None, **kw):
@@ -150,14 +150,21 @@ def test_batched_cloths(tol):
         assert np.all(r2 <= tol)
 
 
-def test_workload_frame_256():
+@pytest.mark.parametrize("shape,schedule", [
+    ((128, 256), {}),
+    ((256, 1024), dict(relax=ch.RELAX_AUTO, cheb_delay=10, restart_rule=ch.RESTART_SEQ_MIN,
+                       restart_factor=10.0)),
+])
+def test_workload_frame_256(shape, schedule):
     """The benchmark's frame: 256x256, wind, rho 0.999999, relative residual
-    1e-5 checked every 50 iterations, about 2200 iterations."""
+    1e-5 checked every 50 iterations, over a thousand iterations; also at the
+    benchmark's damped schedule in 1024-thread workgroups."""
     spec = dict(CFG2)
     kw = dict(max_iterations=80000, tolerance=1e-5, check_every=50, rho=0.999999,
               ext_accel=CFG2_WIND, kernel_variant=ch.KERNEL_AUTO)
-    one = run(spec, 1, 1, (128, 256), **kw)
-    two = run(spec, 2, 1, (128, 256), **kw)
+    kw.update(schedule)
+    one = run(spec, 1, 1, shape, **kw)
+    two = run(spec, 2, 1, shape, **kw)
     assert two[3].iters_per_launch == 2
     assert one[1][0][1] == 1 and one[1][0][0] > 1000
     assert_same(one, two)
@@ -200,7 +207,9 @@ def test_timing_uses_the_launch_in_use():
 def test_autotune_chooses_between_one_and_two():
     """With iters_per_launch = 2 the autotuner also times the two-iteration
     kernel (per tile size: one iteration with one or two element regions at
-    256, 512 and 1024 threads, two iterations at 256 and 512; 32 candidates),
+    256, 512 and 1024 threads, two iterations at 256, 512 and 1024; 36
+    candidates; a pair shape runs where its workgroup holds a tile's own and
+    ring-1 vertices),
     keeps the shortest time per iteration, and the frames equal the
     single-iteration solver's whichever wins."""
     spec = CASES["irregular24_top"]
@@ -212,8 +221,9 @@ def test_autotune_chooses_between_one_and_two():
     args = s.step_args()
     rep = sol.autotune(args, 5)
     cands = rep.candidates()
-    assert rep.num_candidates == 32
-    assert all(c["block_threads"] <= 512 for c in rep.candidates() if c["iters_per_launch"] == 2)
+    assert rep.num_candidates == 36
+    assert sorted({c["block_threads"] for c in cands if c["iters_per_launch"] == 2}) == \
+        [256, 512, 1024]
     assert sorted({c["iters_per_launch"] for c in cands}) == [1, 2]
     timed = [c["us_per_iteration"] for c in cands if c["us_per_iteration"] > 0]
     assert rep.us_per_iteration[rep.best] == min(timed)
