@@ -72,6 +72,11 @@ struct FusedPlan {
     DevBuf<unsigned> gl_meta;                 // [N] start in tile | n_tri << 16 | n_all << 24
     DevBuf<int>   tile_cloth;                 // [num_tiles] cloth owning each tile
     DevBuf<int>   cloth_tiles;                // [num_cloths + 1] first tile of each cloth
+    // persistent launches: the tiles owning each tile's halo vertices, and
+    // one progress word per tile (its own 128-byte line)
+    DevBuf<int>   nbr_ptr, nbr;               // [num_tiles + 1], neighbour tile ids
+    DevBuf<unsigned> prog;                    // [num_tiles * kBarLine]
+    int nbr_max = 0;                          // most neighbours of a tile
 
     // Two-iteration launches (PD): each tile also updates its ring-1 halo
     // (the vertices sharing an element with an own vertex) in the first

@@ -1020,24 +1020,39 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
     if (kCheck) fused_block_sum2<kT>(rr, bb, a.partials, tile);
 }
 
+__global__ void __launch_bounds__(256) k_fill_u32(unsigned* __restrict__ p, int n, unsigned v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // ---- persistent PD iteration kernel ------------------------------------------
 //
 // One cooperative launch runs iterations [k0, k1) of a frame under device
 // control (FrameCtl): every tile's workgroup stays resident, keeps its element
 // records, gather list, halo ids and halo base positions in LDS and its own
-// vertices' state (x_k, x_{k-1}, constants) in registers, and meets the other
-// workgroups at a grid barrier after each iteration. Only the iterate goes
-// through memory: each iteration writes x_{k+1} of the own vertices and reads
-// x_k of the halo. A check iteration adds the fixed-order FP64 residual sum
-// and the control decision (workgroup 0) between two barriers. The arithmetic
-// and every summation order are those of k_fused_iter and k_reduce_partials,
-// so both paths give the same iterates bit for bit.
+// vertices' state (x_k, x_{k-1}, constants) in registers. Only the iterate
+// goes through memory: each iteration writes x_{k+1} of the own vertices and
+// reads x_k of the halo. The arithmetic and every summation order are those
+// of k_fused_iter and k_reduce_partials, so both paths give the same iterates
+// bit for bit.
 //
-// Measured on MI355X at 256x256 this is slower than a launch per iteration:
-// 68 us per iteration with 512 workgroups and 122 us with 1024, against 7-8
-// us, growing with the number of workgroups: the device-scope arrival
-// atomics on one word serialize across the XCDs, so the kernel boundary is
-// the cheaper global barrier here. The path is therefore opt-in.
+// Synchronization. A tile reads the halo of its neighbour tiles (those owning
+// a vertex of its halo; the relation is symmetric, as elements are shared),
+// so iteration k of a tile waits only until every neighbour has finished
+// iteration k-1, as counted by the neighbour's progress word. A tile is then
+// never more than one iteration ahead of a neighbour, so the four rotating
+// iterate buffers are never overwritten while read. Iterates and progress
+// words are written and read as relaxed device-scope atomics, which are
+// coherent across the XCDs without cache write-backs: a workgroup waits for
+// its iterate stores to complete before it publishes its progress. (A
+// device-scope release or acquire fence writes back or invalidates the
+// XCD's L2 and serializes over the XCD's workgroups: measured 21.8 us per
+// grid barrier with one per workgroup at 256 workgroups, 2.06 us without.)
+// The Chebyshev state advances in every workgroup by the same recurrence as
+// the device control's; a check iteration adds the fixed-order FP64 residual
+// sum and the control decision (workgroup 0, whose writes it releases once)
+// between two grid barriers, after which every workgroup reloads the state.
 
 struct PersistArgs {
     FusedKernelArgs A;   // plan and frame arrays (A.it.x / xkm1 / out unused)
@@ -1047,21 +1062,51 @@ struct PersistArgs {
     double* partials;    // [num_tiles * 2]
     double* hist;        // residual history
     unsigned* bar;       // barrier words (kBarWords unsigned, zero at rest)
+    unsigned* prog;      // [num_tiles * kBarLine] progress words (iterations done)
+    const int* nbr_ptr;  // [num_tiles + 1] neighbour tiles of each tile
+    const int* nbr;
     int off_rt, off_rh, off_hb;  // LDS byte offsets: tri records, hinge records, halo ids + base
 };
 
 constexpr long kSpinLimit = 1l << 24;
 
-// Grid-wide barrier of a cooperative launch: release of this workgroup's
-// writes at agent scope, a two-level arrival, and a wait on the generation
-// word; acquire on the way out. Workgroup b counts on the arrival word of its
-// group b % 8 (the XCD it runs on under round-robin dispatch, so the
-// group's arrivals meet in one L2); the group's last arrival counts on the
-// global word, whose last arrival resets it and advances the generation. Each
-// word has a 128-byte line of its own (kBarLine unsigned apart), so polls of
-// the generation do not share a line with the arrivals. A wait longer than
-// kSpinLimit sleeps raises the error flag `err`, which every later barrier
-// honours, so a stuck launch still ends. Returns false once the flag is up.
+__device__ __forceinline__ double ld_coh(const double* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ d3 ld3d_coh(const double* p, size_t i)
+{
+    return d3{ld_coh(p + 3 * i), ld_coh(p + 3 * i + 1), ld_coh(p + 3 * i + 2)};
+}
+__device__ __forceinline__ void st_coh(double* p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st3d_coh(double* p, size_t i, d3 v)
+{
+    st_coh(p + 3 * i, v.x);
+    st_coh(p + 3 * i + 1, v.y);
+    st_coh(p + 3 * i + 2, v.z);
+}
+__device__ __forceinline__ int ctl_load_int(const int* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ctl_load_float(const float* p)
+{
+    return __int_as_float(ctl_load_int(reinterpret_cast<const int*>(p)));
+}
+
+// Grid-wide barrier of a cooperative launch, at the check iterations: a
+// two-level arrival (workgroup b counts on the word of its group b % 8, the
+// group's last arrival on the global word, whose last arrival resets it and
+// advances the generation word that every workgroup waits on), all as relaxed
+// device-scope atomics, each word on a 128-byte line of its own. The data a
+// barrier orders is exchanged through coherent atomics (or released once by
+// its writer), so the barrier itself writes back and invalidates nothing. A
+// wait longer than kSpinLimit sleeps raises the error flag `err`, which every
+// later wait honours, so a stuck launch still ends. Returns false once the
+// flag is up.
 __device__ __forceinline__ bool grid_barrier(unsigned* bar, int* err, unsigned nblocks, int* sflag)
 {
     __syncthreads();
@@ -1072,44 +1117,97 @@ __device__ __forceinline__ bool grid_barrier(unsigned* bar, int* err, unsigned n
         const unsigned ngrp = nblocks < kBarGroups ? nblocks : kBarGroups;
         const unsigned in_grp = nblocks / kBarGroups + (grp < nblocks % kBarGroups ? 1u : 0u);
         unsigned* grp_w = bar + grp * kBarLine;
-        const unsigned gen = __hip_atomic_load(gen_w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
+        const unsigned gen = __hip_atomic_load(gen_w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // this workgroup's earlier stores (coherent atomics) have completed
+        __builtin_amdgcn_s_waitcnt(0);
         bool last = false;
         const unsigned a1 =
-            __hip_atomic_fetch_add(grp_w, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+            __hip_atomic_fetch_add(grp_w, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
         if (a1 == in_grp) {
             __hip_atomic_store(grp_w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const unsigned a2 =
-                __hip_atomic_fetch_add(all_w, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+                __hip_atomic_fetch_add(all_w, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
             last = a2 == ngrp;
         }
         if (last) {
             __hip_atomic_store(all_w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(gen_w, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_s_waitcnt(0);
+            __hip_atomic_store(gen_w, gen + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             long spins = 0;
-            while (__hip_atomic_load(gen_w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            while (__hip_atomic_load(gen_w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
                 if (++spins > kSpinLimit) {
                     __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(1);
             }
         }
-        __threadfence();
         *sflag = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 1 : 0;
     }
     __syncthreads();
     return *sflag == 0;
 }
 
-__device__ __forceinline__ int ctl_load_int(const int* p)
+// Waits until every neighbour tile of `tile` has finished `k` iterations
+// (thread i polls neighbour i; at most kT neighbours, checked on the host).
+// The same time-out as the grid barrier. Returns false once the flag is up.
+__device__ __forceinline__ bool wait_neighbours(const unsigned* prog, const int* nbr, int nb,
+                                                unsigned k, int* err, int* sflag)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) *sflag = 0;
+    __syncthreads();
+    if ((int)threadIdx.x < nb) {
+        const unsigned* w = prog + (size_t)nbr[threadIdx.x] * kBarLine;
+        long spins = 0;
+        while (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                *sflag = 1;
+                break;
+            }
+            if (++spins > kSpinLimit) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                *sflag = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+    return *sflag == 0;
 }
 
+// The persistent kernel's residual partials: the block sum of
+// fused_block_sum2, stored as coherent atomics.
 template <int kT>
+__device__ void persist_block_sum2(double r, double b, double* out, int slot)
+{
+    __shared__ double sr[kT / 64], sb[kT / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        r += __shfl_down(r, off, 64);
+        b += __shfl_down(b, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        sr[wave] = r;
+        sb[wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tr = 0.0, tb = 0.0;
+        for (int i = 0; i < kT / 64; ++i) {
+            tr += sr[i];
+            tb += sb[i];
+        }
+        st_coh(out + 2 * slot + 0, tr);
+        st_coh(out + 2 * slot + 1, tb);
+    }
+}
+
+// kG adjacent lanes share an own vertex in the gathers, as in k_fused_iter
+// (same bits for every kG); the group's first lane holds the vertex's state.
+template <int kT, int kG>
 __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
 {
     extern __shared__ double smem[];
@@ -1122,6 +1220,7 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
     const int tile = A.xcd_remap ? xcd_tile(blockIdx.x, A.num_tiles) : (int)blockIdx.x;
     const int vbeg = tile * A.tile_verts;
     const int nown = min(A.tile_verts, A.N - vbeg);
+    int* const err = &P.ctl->barrier_error;
 
     char* lds = reinterpret_cast<char*>(smem);
     double* sX = smem;
@@ -1145,6 +1244,8 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
     const int* ti = A.tile_info + (size_t)tile * kTileInfo;
     const int nhalo = ti[0], tbeg = ti[1], ntri = ti[2], ebeg = ti[3], nhin = ti[4];
     const int gbeg = ti[5], glen = ti[6];
+    const int nb0 = P.nbr_ptr[tile], nb = P.nbr_ptr[tile + 1] - nb0;
+    const int* nbr = P.nbr + nb0;
     for (int j = tid; j < ntri; j += kT) {
         rti[j] = A.tri_idx[tbeg + j];
         rtg[j] = A.tri_c0[tbeg + j];
@@ -1169,39 +1270,48 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
     int4* sov = reinterpret_cast<int4*>(shb + 3 * A.hstride);  // 16 * hstride bytes in
     float* sob = reinterpret_cast<float*>(sov + A.tile_verts);
     float* soy = sob + 3 * A.tile_verts;
-    const bool own = tid < nown;
-    const size_t g = (size_t)vbeg + (own ? tid : 0);
+    const int vl = tid / kG, lane_p = tid % kG;  // own vertex of the lane's group
+    const bool own = vl < nown;
+    const bool lead = own && lane_p == 0;
+    const size_t g = (size_t)vbeg + (own ? vl : 0);
     d3 od = d3{0.0, 0.0, 0.0}, oxp = d3{0.0, 0.0, 0.0};
-    if (own) {
-        sov[tid] = a.vinfo[g];
-        sts3(sob, tid, ld3(a.base, g));
-        sts3(soy, tid, ld3(a.y, g));
+    if (lead) {
+        sov[vl] = a.vinfo[g];
+        sts3(sob, vl, ld3(a.base, g));
+        sts3(soy, vl, ld3(a.y, g));
         od = ld3d(P.it[P.k0 % kIterBufs], g);
         oxp = ld3d(P.it[(P.k0 + kIterBufs - 1) % kIterBufs], g);
     }
+    // control state of iteration k0 (the launch boundary orders it)
+    if (ctl_load_int(&P.ctl->stop)) return;
+    CtlSlot cs;
+    cs.omega = ctl_load_float(&P.ctl->slot[P.k0 & 1].omega);
+    cs.blend = ctl_load_int(&P.ctl->slot[P.k0 & 1].blend);
+    cs.ck = ctl_load_int(&P.ctl->slot[P.k0 & 1].ck);
+    cs.k = ctl_load_int(&P.ctl->slot[P.k0 & 1].k);
+    float rho = ctl_load_float(&P.ctl->rho);
+    const int delay = ctl_load_int(&P.ctl->delay), use_cheb = ctl_load_int(&P.ctl->use_cheb);
     __syncthreads();
 
     for (int k = P.k0; k < P.k1; ++k) {
-        // ---- control state of iteration k (written before the last barrier)
-        if (ctl_load_int(&P.ctl->stop)) break;
         const int par = k & 1;
-        CtlSlot cs;
-        cs.omega = __int_as_float(ctl_load_int(reinterpret_cast<const int*>(&P.ctl->slot[par].omega)));
-        cs.blend = ctl_load_int(&P.ctl->slot[par].blend);
-        cs.ck = ctl_load_int(&P.ctl->slot[par].ck);
-        cs.k = ctl_load_int(&P.ctl->slot[par].k);
-        ctl_write_next(P.ctl, par, cs);
+        // block 0 keeps the device control's slots as the launches do
+        const CtlSlot cn = ctl_next_rho(rho, delay, use_cheb, cs);
+        if (blockIdx.x == 0 && tid == 0) P.ctl->slot[par ^ 1] = cn;
         const bool check = ((k + 1) % P.check_every == 0) || (k == P.K - 1);
         const double* xk = P.it[k % kIterBufs];
         double* out = P.it[(k + 1) % kIterBufs];
 
+        // ---- the neighbours have written x_k
+        if (k > P.k0 && !wait_neighbours(P.prog, nbr, nb, (unsigned)k, err, &s_flag)) break;
+
         // ---- positions of the tile: own from registers, halo from memory
         for (int i = tid; i < nhalo; i += kT) {
-            const d3 hx = ld3d(xk, shid[i]);
+            const d3 hx = ld3d_coh(xk, shid[i]);
             const d3 hb = d3{(double)shb[3 * i], (double)shb[3 * i + 1], (double)shb[3 * i + 2]};
             stsX(sX, L, nown + i, hb + hx);
         }
-        if (own) stsX(sX, L, tid, to_d(lds3(sob, tid)) + od);
+        if (lead) stsX(sX, L, vl, to_d(lds3(sob, vl)) + od);
         __syncthreads();
 
         // ---- triangles -> corner forces
@@ -1225,15 +1335,15 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
             }
         }
         __syncthreads();
-        const int4 vi = own ? sov[tid] : make_int4(0, 0, -1, 0);
+        const int4 vi = own ? sov[vl] : make_int4(0, 0, -1, 0);
         const unsigned meta = (unsigned)vi.w;
         const int j0 = (int)(meta & 0xFFFFu), jt = j0 + (int)((meta >> 16) & 0xFFu),
                   j1 = j0 + (int)(meta >> 24);
         const bool gather = own && vi.z < 0;
-        GatherAcc<1> acc;
+        GatherAcc<kG> acc;
         if (gather && a.use_stretch) {
-            if (check) gather_corners<true, 1>(sg, j0, jt, 0, st, st2, acc);
-            else       gather_corners<false, 1>(sg, j0, jt, 0, st, st2, acc);
+            if (check) gather_corners<true, kG>(sg, j0, jt, lane_p, st, st2, acc);
+            else       gather_corners<false, kG>(sg, j0, jt, lane_p, st, st2, acc);
         }
         if (a.use_bend) {
             __syncthreads();
@@ -1254,42 +1364,59 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
             }
             __syncthreads();
             if (gather) {
-                if (check) gather_hinges<true, true, 1>(sg, jt, j1, 0, skw, A.hmax, sq, sp, acc);
-                else       gather_hinges<true, false, 1>(sg, jt, j1, 0, skw, A.hmax, sq, sp, acc);
+                if (check) gather_hinges<true, true, kG>(sg, jt, j1, lane_p, skw, A.hmax, sq, sp, acc);
+                else       gather_hinges<true, false, kG>(sg, jt, j1, lane_p, skw, A.hmax, sq, sp, acc);
             }
         }
-        const f3 ar = gather_total<1>(acc.r);
-        const f3 ab = gather_total<1>(acc.b);
+        const f3 ar = gather_total<kG>(acc.r);
+        const f3 ab = gather_total<kG>(acc.b);
 
         // ---- own vertices: update, pins, residual
         double rr = 0.0, bb = 0.0;
-        if (own) {
-            const f3 ob = lds3(sob, tid);
+        if (lead) {
+            const f3 ob = lds3(sob, vl);
             const float om = __int_as_float(vi.x), odg = __int_as_float(vi.y);
             d3 xn;
             if (vi.z >= 0) {
                 xn = to_d(ld3(a.pin_target, vi.z)) - to_d(ob);
             } else {
-                const f3 oy = lds3(soy, tid);
+                const f3 oy = lds3(soy, vl);
                 d3 r;
                 xn = pd_update(ar, om, odg, oy, od, oxp, cs.omega, cs.blend, relax_gamma(a), r);
                 if (check) {
                     pd_residual_terms(r, ab, om, ob, oy, rr, bb);
                 }
             }
-            st3d(out, g, xn);
+            st3d_coh(out, g, xn);
             oxp = od;
             od = xn;
         }
-        if (check) fused_block_sum2<kT>(rr, bb, P.partials, tile);
-        if (!grid_barrier(P.bar, &P.ctl->barrier_error, gridDim.x, &s_flag)) break;
+        cs = cn;
+        // every lane's iterate stores have completed (a workgroup barrier
+        // orders execution, not the completion of other waves' stores)
+        __builtin_amdgcn_s_waitcnt(0);
         if (check) {
+            if (kG > 1) {
+                // vertex i's terms in thread i, as with one lane per vertex
+                // (the position table is free: its last reads came before
+                // the barrier ahead of the hinge sums)
+                __syncthreads();
+                if (lead) {
+                    sX[vl] = rr;
+                    sX[L + vl] = bb;
+                }
+                __syncthreads();
+                rr = tid < nown ? sX[tid] : 0.0;
+                bb = tid < nown ? sX[L + tid] : 0.0;
+            }
+            persist_block_sum2<kT>(rr, bb, P.partials, tile);
+            if (!grid_barrier(P.bar, err, gridDim.x, &s_flag)) break;
             if (blockIdx.x == 0) {
                 // fixed-order sum of the tile partials (k_reduce_partials)
                 double r = 0.0, b = 0.0;
                 for (int i = tid; i < A.num_tiles; i += kT) {
-                    r += P.partials[2 * i + 0];
-                    b += P.partials[2 * i + 1];
+                    r += ld_coh(P.partials + 2 * i + 0);
+                    b += ld_coh(P.partials + 2 * i + 1);
                 }
                 red_r[tid] = r;
                 red_b[tid] = b;
@@ -1301,10 +1428,28 @@ __global__ void __launch_bounds__(kT, 4) k_fused_persist(PersistArgs P)
                     }
                     __syncthreads();
                 }
-                if (tid == 0) ctl_check(P.ctl, par, red_r[0], red_b[0], P.hist, red_r[0] <= P.ctl->tol2 * red_b[0]);
+                if (tid == 0) {
+                    ctl_check(P.ctl, par, red_r[0], red_b[0], P.hist,
+                              red_r[0] <= P.ctl->tol2 * red_b[0]);
+                    // the control block and history, once, for every XCD
+                    __threadfence();
+                }
             }
-            if (!grid_barrier(P.bar, &P.ctl->barrier_error, gridDim.x, &s_flag)) break;
+            if (!grid_barrier(P.bar, err, gridDim.x, &s_flag)) break;
+            if (ctl_load_int(&P.ctl->stop)) break;
+            // a restart may have replaced the next state and lowered rho
+            cs.omega = ctl_load_float(&P.ctl->slot[par ^ 1].omega);
+            cs.blend = ctl_load_int(&P.ctl->slot[par ^ 1].blend);
+            cs.ck = ctl_load_int(&P.ctl->slot[par ^ 1].ck);
+            cs.k = ctl_load_int(&P.ctl->slot[par ^ 1].k);
+            rho = ctl_load_float(&P.ctl->rho);
+        } else {
+            __syncthreads();
         }
+        // publish x_{k+1} of this tile
+        if (tid == 0)
+            __hip_atomic_store(P.prog + (size_t)tile * kBarLine, (unsigned)(k + 1),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -2130,6 +2275,30 @@ int fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
     CLH_TRY(p.gl_meta.upload(h.gl_meta));
     CLH_TRY(p.tile_cloth.upload(h.tile_cloth));
     CLH_TRY(p.cloth_tiles.upload(h.cloth_tiles));
+    {
+        // neighbour tiles: owners of the halo vertices (tile t owns the
+        // tile-order range [t V, t V + V)), ascending, without t itself
+        std::vector<int> nptr(1, 0), nb;
+        std::vector<int> seen;
+        p.nbr_max = 0;
+        for (int t = 0; t < p.num_tiles; ++t) {
+            const int nh = h.info[(size_t)t * kTileInfo + 0];
+            seen.clear();
+            for (int i = 0; i < nh; ++i) {
+                const int o = h.halo[(size_t)t * p.hstride + i] / p.tile_verts;
+                if (o != t) seen.push_back(o);
+            }
+            std::sort(seen.begin(), seen.end());
+            seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
+            nb.insert(nb.end(), seen.begin(), seen.end());
+            nptr.push_back((int)nb.size());
+            p.nbr_max = std::max(p.nbr_max, (int)seen.size());
+        }
+        if (nb.empty()) nb.push_back(0);
+        CLH_TRY(p.nbr_ptr.upload(nptr));
+        CLH_TRY(p.nbr.upload(nb));
+        CLH_TRY(p.prog.resize((size_t)std::max(p.num_tiles, 1) * kBarLine));
+    }
     p.pair = m.pair;
     if (p.pair) {
         p.pstride = m.pstride; p.plmax = m.plmax; p.ptmax = m.ptmax; p.phmax = m.phmax;
@@ -2189,9 +2358,23 @@ int fused_max_blocks(const FusedPlan& p) { return p.num_tiles; }
 
 size_t fused_persist_lds_bytes(const FusedPlan& p) { return persist_layout(p).total; }
 
+int gather_lanes(const FusedPlan& p);
+
+// The persistent kernel of the plan's workgroup size and gather lanes.
+const void* persist_kernel(const FusedPlan& p)
+{
+    if (p.threads == 1024) return reinterpret_cast<const void*>(&k_fused_persist<1024, 4>);
+    switch (gather_lanes(p)) {
+    case 4: return reinterpret_cast<const void*>(&k_fused_persist<256, 4>);
+    case 2: return reinterpret_cast<const void*>(&k_fused_persist<256, 2>);
+    default: return reinterpret_cast<const void*>(&k_fused_persist<256, 1>);
+    }
+}
+
 bool fused_persist_fits(const FusedPlan& p)
 {
-    if (p.threads != 256 || p.num_tiles <= 0) return false;
+    if ((p.threads != 256 && p.threads != 1024) || p.num_tiles <= 0 || p.nbr_max > p.threads)
+        return false;
     int dev = 0, cus = 0, coop = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -2200,8 +2383,9 @@ bool fused_persist_fits(const FusedPlan& p)
         return false;
     }
     if (!coop) return false;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_fused_persist<256>, 256,
-                                                     fused_persist_lds_bytes(p)) != hipSuccess) {
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per, reinterpret_cast<const void*>(persist_kernel(p)), p.threads, fused_persist_lds_bytes(p));
+    if (e != hipSuccess) {
         (void)hipGetLastError();
         return false;
     }
@@ -2227,12 +2411,22 @@ int fused_launch_persist(const FusedPlan& p, const FusedIterArgs& a, double* con
     P.partials = partials;
     P.hist = hist;
     P.bar = bar;
+    P.prog = p.prog.ptr;
+    P.nbr_ptr = p.nbr_ptr.ptr;
+    P.nbr = p.nbr.ptr;
     P.off_rt = (int)L.off_rt;
     P.off_rh = (int)L.off_rh;
     P.off_hb = (int)L.off_hb;
+    if (k0 == 0) {
+        // a frame's progress counts start at 0 (a fill kernel: no memset
+        // node in captured work)
+        const int n = p.num_tiles * kBarLine;
+        hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, stream, p.prog.ptr, n, 0u);
+    }
     void* args[] = {&P};
-    return hip_status(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_fused_persist<256>),
-                                                 dim3(p.num_tiles), dim3(256), args, L.total, stream),
+    const void* fn = persist_kernel(p);
+    return hip_status(hipLaunchCooperativeKernel(fn, dim3(p.num_tiles), dim3(p.threads), args,
+                                                 L.total, stream),
                       "persistent iteration launch");
 }
 
@@ -2384,7 +2578,10 @@ int fused_enable_lds(int& bytes)
     push_iter_kernels<CLH_MODE_REF_CUDA, false, 1024, false>(fns);
     push_iter_kernels<CLH_MODE_PD, true, 1024, true>(fns);
     push_iter_kernels<CLH_MODE_PD, false, 1024, true>(fns);
-    fns.push_back(reinterpret_cast<const void*>(&k_fused_persist<256>));
+    fns.push_back(reinterpret_cast<const void*>(&k_fused_persist<256, 1>));
+    fns.push_back(reinterpret_cast<const void*>(&k_fused_persist<256, 2>));
+    fns.push_back(reinterpret_cast<const void*>(&k_fused_persist<256, 4>));
+    fns.push_back(reinterpret_cast<const void*>(&k_fused_persist<1024, 4>));
     fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<true, 256>));
     fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<false, 256>));
     fns.push_back(reinterpret_cast<const void*>(&k_fused_pair<true, 512>));
@@ -2520,6 +2717,10 @@ void fused_plan_release(FusedPlan& p)
     p.gl_meta.release();
     p.tile_cloth.release();
     p.cloth_tiles.release();
+    p.nbr_ptr.release();
+    p.nbr.release();
+    p.prog.release();
+    p.nbr_max = 0;
     p.pinfo.release();
     p.phalo.release();
     p.pmeta.release();

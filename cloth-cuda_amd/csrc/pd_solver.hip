@@ -1788,6 +1788,9 @@ void set_shape(clh_solver* s, int threads, int ipl, int regions)
     s->ipl = ipl;
     s->regions = regions;
     s->fused.split = regions == 2 ? 1 : 0;
+    // the persistent kernel is built for some workgroup sizes only
+    s->persist = s->cfg.persistent == CLH_PERSIST_ON && s->variant == CLH_KERNEL_FUSED &&
+                 s->cfg.mode == CLH_MODE_PD && s->num_cloths == 1 && fused_persist_fits(s->fused);
     s->graph_valid = false;
     drop_chunk_graphs(s);
 }

@@ -397,17 +397,27 @@ def test_fused_matches_unfused(mode, spec):
     assert rel_err(outs[1][1], outs[0][1], L / REF_CASES[spec]["dt"]) < 1e-6
 
 
+@pytest.mark.parametrize("shape,schedule", [
+    ({}, {}),
+    (dict(tile_verts=64, block_threads=256), {}),
+    (dict(tile_verts=256, block_threads=1024), {}),
+    (dict(tile_verts=256, block_threads=1024),
+     dict(relax=ch.RELAX_AUTO, cheb_delay=10, restart_rule=ch.RESTART_SEQ_MIN, restart_factor=10.0)),
+])
 @pytest.mark.parametrize("spec", [dict(CFG1, pin_mode=ch.PIN_TOP_ROW), CFG2])
-def test_persistent_frames_equal_per_iteration_frames(spec):
-    """Frames to a tolerance in persistent cooperative launches (grid barrier
-    between iterations, residual reduction and control in workgroup 0) equal
-    the per-iteration launches bit for bit: positions, velocities, iteration
-    counts and residual histories."""
+def test_persistent_frames_equal_per_iteration_frames(spec, shape, schedule):
+    """Frames to a tolerance in persistent cooperative launches (each tile
+    waits for its neighbour tiles' progress between iterations; a grid barrier,
+    the residual reduction and the control in workgroup 0 at the checks)
+    equal the per-iteration launches bit for bit: positions, velocities,
+    iteration counts and residual histories; with 256- and 1024-thread
+    workgroups, one, two and four lanes per vertex in the gathers, and
+    Chebyshev restarts (rho 0.999, a check every 25 iterations)."""
     out = {}
     for persist in (ch.PERSIST_ON, ch.PERSIST_OFF):
         s, _ = make(spec)
         cfg = pd_cfg(20000, spec["dt"], rho=0.999, tolerance=1e-5, check_every=25,
-                     persistent=persist)
+                     persistent=persist, **shape, **schedule)
         sol = solver_for(s, **cfg)
         args = s.step_args()
         its, hist = [], []
