@@ -251,6 +251,45 @@ def cpu_baseline(ch, s, sol, pos_w, vel_w, args):
             "ref_cpu_32x32": ref_cpu}
 
 
+def persistent_frames(ch, s, args_c, pos_w, vel_w, common, args):
+    """The same timed frames (from the same state, same schedule) in
+    persistent cooperative launches, one per chunk of check_every iterations
+    (tiles wait only for their neighbour tiles between iterations), at
+    256-vertex tiles and 1024-thread workgroups, against launches per
+    iteration at the same shape; both end on the same bits."""
+    import numpy as np
+    out = {}
+    ends = {}
+    for name, extra in (("launch", {}), ("persistent", dict(persistent=ch.PERSIST_ON))):
+        s.set_state(pos_w, vel_w)
+        sol = ch.Solver(ch.config(max_iterations=args.max_iters, tolerance=args.tol,
+                                  check_every=args.check_every, tile_verts=256,
+                                  block_threads=1024, **extra, **common), s.N, s.T, s.E)
+        # the plan, the step-scale bound and the chunk graphs are built in
+        # warm-up frames; the timed frames start again from the same state
+        for _ in range(2):
+            sol.step(args_c)
+        s.set_state(pos_w, vel_w)
+        ms_total, its = 0.0, []
+        for _ in range(args.steps):
+            ms_total += sol.run_frames(args_c, 1)
+            its.append(sol.stats().iterations)
+        info = sol.plan_info()
+        ends[name] = s.get_state()
+        out[name] = {"frames_per_s": args.steps / (ms_total / 1e3),
+                     "iterations_per_frame": float(np.mean(its)),
+                     "us_per_iteration": 1e3 * ms_total / sum(its),
+                     "persistent": int(info.persistent)}
+    out["same_bits"] = bool(np.array_equal(ends["launch"][0], ends["persistent"][0]) and
+                            np.array_equal(ends["launch"][1], ends["persistent"][1]))
+    out["shape"] = "256-vertex tiles, 1024-thread workgroups"
+    log(f"persistent: {out['persistent']['frames_per_s']:.2f} frames/s "
+        f"({out['persistent']['us_per_iteration']:.2f} us/iteration) against "
+        f"{out['launch']['frames_per_s']:.2f} launched at the same shape; same bits: "
+        f"{out['same_bits']}")
+    return out
+
+
 def warm_start_study(ch, n, rho, args, pos0=None, vel0=None, scene=None, gen_tol=None,
                      gen_mode=None):
     """Trains the warm start on frames of args.ws_cloths batched n x n cloths
@@ -623,6 +662,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-1024", action="store_true")
+    ap.add_argument("--no-persist", action="store_true",
+                    help="skip the persistent-kernel frames")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-ws", action="store_true", help="skip the learned warm start")
     ap.add_argument("--launch-timing", action="store_true",
@@ -706,6 +747,10 @@ def main():
     mean_iters = float(np.mean(iters))
     log(f"to tol {args.tol}: {fps_tol:.2f} frames/s, {mean_iters:.0f} iterations/frame, "
         f"{sum(conv)}/{len(conv)} converged")
+    persist = None
+    if not args.no_persist:
+        persist = persistent_frames(ch, s, args_c, pos_w, vel_w, common, args)
+
 
     # ---- 256x256: fixed 20 iterations per frame --------------------------------
     s.set_state(pos0, vel0)
@@ -867,6 +912,7 @@ def main():
                    "frames_converged": int(sum(conv)),
                    "final_rel_residual_mean": float(np.mean(resid)),
                    "frames_per_s_fixed_20_iterations": fps20,
+                   "persistent_kernel": persist,
                    "fp64_oracle_error_20_iterations": err64,
                    "fp64_oracle_error_to_tolerance": tol_err,
                    "kernel_variant": int(plan.variant), "iters_per_launch": int(ipl),

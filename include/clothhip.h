@@ -115,9 +115,11 @@ typedef struct clh_config {
                                two-iteration launches need a workgroup of at least a
                                tile's own and ring-1 vertices) */
     int   persistent;       /* CLH_PERSIST_ON: frames to a tolerance in one cooperative
-                               launch per chunk (grid barrier between iterations) when
-                               every tile fits resident; off by default: on MI355X the
-                               barrier costs more than a launch */
+                               launch per chunk when every tile fits resident (256- or
+                               1024-thread workgroups): each tile waits only for its
+                               neighbour tiles between iterations, a grid barrier joins
+                               them at the checks; same bits as launches; off by
+                               default */
     int   iters_per_launch; /* fused variant, CLH_MODE_PD: 2 runs two iterations per
                                launch (each tile also updates its ring-1 halo in the
                                first; same bits as one per launch) and lets the
