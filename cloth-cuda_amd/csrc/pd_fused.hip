@@ -111,6 +111,7 @@ void build_slots(const std::vector<int>& elems, int arity, int N, std::vector<in
 struct HostPlan {
     FusedPlan meta;  // sizes only; device buffers unused
     std::vector<int> perm, inv, info, halo, tri_gid, hin_gid, tile_cloth, cloth_tiles;
+    std::vector<int> nbr_ptr, nbr;  // neighbour tiles (owners of each tile's halo vertices)
     std::vector<uint2> tri_idx, hin_idx;
     std::vector<unsigned short> gl_idx;
     std::vector<unsigned> gl_meta;
@@ -2025,6 +2026,26 @@ int host_plan_build(HostPlan& h, int N, const std::vector<int>& tris,
     h.gl_meta.swap(gl_meta);
     h.tile_cloth.swap(tile_cloth);
     h.cloth_tiles.swap(cloth_tiles);
+    // neighbour tiles: owners of the halo vertices (tile t owns the
+    // tile-order range [t V, t V + V)), ascending, without t itself
+    h.nbr_ptr.assign(1, 0);
+    h.nbr.clear();
+    int nbr_max = 0;
+    std::vector<int> seen;
+    for (int t = 0; t < ntiles; ++t) {
+        const int nh = h.info[(size_t)t * kTileInfo + 0];
+        seen.clear();
+        for (int i = 0; i < nh; ++i) {
+            const int o = h.halo[(size_t)t * hstride + i] / V;
+            if (o != t) seen.push_back(o);
+        }
+        std::sort(seen.begin(), seen.end());
+        seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
+        h.nbr.insert(h.nbr.end(), seen.begin(), seen.end());
+        h.nbr_ptr.push_back((int)h.nbr.size());
+        nbr_max = std::max(nbr_max, (int)seen.size());
+    }
+    h.meta.nbr_max = nbr_max;
     return CLH_OK;
 }
 
@@ -2242,6 +2263,32 @@ int host_plan_check(const HostPlan& h, const std::vector<int>& tris, const std::
         }
         if (p.pair) CLH_TRY(pair_tile_check(h, t, tris, quads, vt_ptr, vt_ids, ve_ptr, ve_ids, cloth_of));
     }
+    // neighbour tiles (the persistent kernel's waits): ascending, unique,
+    // without the tile itself, every halo owner among them, no tile beyond
+    // them, and the relation symmetric, so a tile never runs more than one
+    // iteration ahead of a tile that reads its vertices
+    if ((int)h.nbr_ptr.size() != p.num_tiles + 1 || h.nbr_ptr[0] != 0 ||
+        h.nbr_ptr.back() != (int)h.nbr.size())
+        return fail(CLH_ERR_INVALID, "plan: neighbour table size");
+    std::vector<std::vector<int>> adj((size_t)p.num_tiles);
+    for (int t = 0; t < p.num_tiles; ++t) {
+        const int b = h.nbr_ptr[(size_t)t], e = h.nbr_ptr[(size_t)t + 1];
+        if (e < b || e - b > p.nbr_max) return fail(CLH_ERR_INVALID, "plan: neighbour range");
+        std::vector<int> want;
+        const int nh = h.info[(size_t)t * kTileInfo + 0];
+        for (int i = 0; i < nh; ++i) {
+            const int o = h.halo[(size_t)t * p.hstride + i] / V;
+            if (o != t) want.push_back(o);
+        }
+        std::sort(want.begin(), want.end());
+        want.erase(std::unique(want.begin(), want.end()), want.end());
+        adj[(size_t)t].assign(h.nbr.begin() + b, h.nbr.begin() + e);
+        if (adj[(size_t)t] != want) return fail(CLH_ERR_INVALID, "plan: neighbour tiles differ from halo owners");
+    }
+    for (int t = 0; t < p.num_tiles; ++t)
+        for (int o : adj[(size_t)t])
+            if (!std::binary_search(adj[(size_t)o].begin(), adj[(size_t)o].end(), t))
+                return fail(CLH_ERR_INVALID, "plan: neighbour relation not symmetric");
     return CLH_OK;
 }
 
@@ -2275,30 +2322,14 @@ int fused_plan_build(FusedPlan& p, int N, const std::vector<int>& tris,
     CLH_TRY(p.gl_meta.upload(h.gl_meta));
     CLH_TRY(p.tile_cloth.upload(h.tile_cloth));
     CLH_TRY(p.cloth_tiles.upload(h.cloth_tiles));
+    p.nbr_max = m.nbr_max;
+    CLH_TRY(p.nbr_ptr.upload(h.nbr_ptr));
     {
-        // neighbour tiles: owners of the halo vertices (tile t owns the
-        // tile-order range [t V, t V + V)), ascending, without t itself
-        std::vector<int> nptr(1, 0), nb;
-        std::vector<int> seen;
-        p.nbr_max = 0;
-        for (int t = 0; t < p.num_tiles; ++t) {
-            const int nh = h.info[(size_t)t * kTileInfo + 0];
-            seen.clear();
-            for (int i = 0; i < nh; ++i) {
-                const int o = h.halo[(size_t)t * p.hstride + i] / p.tile_verts;
-                if (o != t) seen.push_back(o);
-            }
-            std::sort(seen.begin(), seen.end());
-            seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
-            nb.insert(nb.end(), seen.begin(), seen.end());
-            nptr.push_back((int)nb.size());
-            p.nbr_max = std::max(p.nbr_max, (int)seen.size());
-        }
+        std::vector<int> nb = h.nbr;
         if (nb.empty()) nb.push_back(0);
-        CLH_TRY(p.nbr_ptr.upload(nptr));
         CLH_TRY(p.nbr.upload(nb));
-        CLH_TRY(p.prog.resize((size_t)std::max(p.num_tiles, 1) * kBarLine));
     }
+    CLH_TRY(p.prog.resize((size_t)std::max(p.num_tiles, 1) * kBarLine));
     p.pair = m.pair;
     if (p.pair) {
         p.pstride = m.pstride; p.plmax = m.plmax; p.ptmax = m.ptmax; p.phmax = m.phmax;

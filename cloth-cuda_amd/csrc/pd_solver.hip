@@ -1213,8 +1213,9 @@ int validate_args(const clh_solver* s, const clh_step_args* a) { return validate
 // D^-1 K (k_jacobi_bound): the damped Jacobi iteration matrix
 // I - gamma D^-1 K then has every eigenvalue in [-rho, 1), inside the
 // interval [-rho, rho] the Chebyshev weights assume at the bottom, where stiff
-// bending otherwise puts eigenvalues below -1 that the weights amplify. The
-// bound is recomputed when the constants' pointers, dt or rho change.
+// bending otherwise puts eigenvalues below -1 that the weights amplify. dt
+// and rho are fixed for the solver's lifetime, so the bound is recomputed
+// only when the constants' or topology's pointers change.
 int ensure_relax(clh_solver* s, const clh_step_args& a)
 {
     const clh_config& c = s->cfg;
@@ -1529,6 +1530,7 @@ extern "C" int clh_solver_reset(clh_solver* s)
     if (!s) return fail(CLH_ERR_INVALID, "clh_solver_reset: null solver");
     CLH_HIP(hipStreamSynchronize(s->stream));
     s->plan_ready = false;
+    s->relax_ready = false;  // constants may have changed in place
     s->graph_valid = false;
     drop_chunk_graphs(s);
     s->pins_checked_ptr = nullptr;
