@@ -42,6 +42,7 @@ struct FusedIterArgs {
     int   par;
     ClothCtl* cctl;        // batches run to a tolerance: per-cloth control (with ctl), or null
     float relax;           // PD: scale gamma of the Jacobi step (0 means 1)
+    const float* cloth_relax;  // batches: each cloth's gamma (overrides relax), or null
 };
 
 struct FusedPlan {

@@ -453,7 +453,7 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
     const int hid0 = hids[min(tid, A.hstride - 1)];
     const int4 vi = own ? vld : make_int4(0, 0, -1, 0);
 
-    const int cloth = a.cctl ? A.tile_cloth[tile] : 0;
+    const int cloth = (a.cctl || a.cloth_relax) ? A.tile_cloth[tile] : 0;
 
     const int nhalo = ti[0], tbeg = ti[1], ntri = ti[2], ebeg = ti[3], nhin = ti[4];
     const int gbeg = ti[5], glen = ti[6];
@@ -649,7 +649,8 @@ __global__ void __launch_bounds__(kT) k_fused_iter(FusedKernelArgs A)
             st3(static_cast<float*>(a.out), g, madd_nc(ar, om, oy) * inv);
         } else {
             d3 r;
-            const d3 xn = pd_update(ar, om, odg, oy, od, oxp, a.omega, a.blend, relax_gamma(a), r);
+            const float gamma = a.cloth_relax ? a.cloth_relax[cloth] : relax_gamma(a);
+            const d3 xn = pd_update(ar, om, odg, oy, od, oxp, a.omega, a.blend, gamma, r);
             st3d(static_cast<double*>(a.out), g, xn);
             if (kCheck) {
                 pd_residual_terms(r, ab, om, ob, oy, rr, bb);
@@ -730,6 +731,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
 
     const int L = P.plmax;
     const int tile = A.xcd_remap ? xcd_tile(blockIdx.x, A.num_tiles) : (int)blockIdx.x;
+    const float gamma = a.cloth_relax ? a.cloth_relax[A.tile_cloth[tile]] : relax_gamma(a);
     const int vbeg = tile * A.tile_verts;
     const int nown = min(A.tile_verts, A.N - vbeg);
 
@@ -928,7 +930,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             x1 = to_d(ld3(a.pin_target, opin)) - to_d(ob);
         } else {
             d3 r;
-            x1 = pd_update(ar1, om, odg, oy, od, oxp, c0.omega, c0.blend, relax_gamma(a), r);
+            x1 = pd_update(ar1, om, odg, oy, od, oxp, c0.omega, c0.blend, gamma, r);
         }
         if (own) st3d(static_cast<double*>(a.out), g, x1);
         stsX(sX, L, tid, to_d(ob) + x1);
@@ -1011,7 +1013,7 @@ __global__ void __launch_bounds__(kT) k_fused_pair(PairKernelArgs P)
             x2 = to_d(ld3(a.pin_target, opin)) - to_d(ob);
         } else {
             d3 r;
-            x2 = pd_update(ar, om, odg, oy, x1, od, c1.omega, c1.blend, relax_gamma(a), r);
+            x2 = pd_update(ar, om, odg, oy, x1, od, c1.omega, c1.blend, gamma, r);
             if (kCheck) {
                 pd_residual_terms(r, ab, om, ob, oy, rr, bb);
             }

@@ -626,7 +626,9 @@ k_trace(const void* __restrict__ it, const float* __restrict__ xprev, float* __r
 // of the row, |K_vj| <= sum over elements of |K^e_vj|. One thread per vertex
 // sums its triangle corners and hinge slots in ascending order, so the row is
 // the same on every run; the maximum of non-negative doubles is exact in any
-// order, taken with an integer atomic max on their bits.
+// order, taken with an integer atomic max on their bits. With cloth vertex
+// offsets seg[0..C] each cloth gets its own maximum in out[c] (the cloths
+// share no element, so a cloth's rows are those of its solo mesh).
 __global__ void __launch_bounds__(kBlock)
 k_jacobi_bound(const int* __restrict__ vt_ptr, const int* __restrict__ vt_idx,
                const int* __restrict__ ve_ptr, const int* __restrict__ ve_idx,
@@ -634,7 +636,8 @@ k_jacobi_bound(const int* __restrict__ vt_ptr, const int* __restrict__ vt_idx,
                const float* __restrict__ Dm_inv, const float* __restrict__ rest_area,
                const float* __restrict__ stretch_k, const float* __restrict__ bend_w,
                const float* __restrict__ bend_k, int N, double h2, int use_stretch,
-               int use_bend, unsigned long long* __restrict__ out)
+               int use_bend, const int* __restrict__ seg, int C,
+               unsigned long long* __restrict__ out)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
@@ -660,7 +663,16 @@ k_jacobi_bound(const int* __restrict__ vt_ptr, const int* __restrict__ vt_idx,
             row += fabs(h2 * (double)bend_k[e]) * fabs((double)w[i]) * sw;
         }
     const double ratio = row / d;
-    atomicMax(out, (unsigned long long)__double_as_longlong(ratio));
+    int lo = 0;
+    if (seg) {  // the cloth c with seg[c] <= v < seg[c + 1]
+        int hi = C;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (seg[mid] <= v) lo = mid;
+            else hi = mid;
+        }
+    }
+    atomicMax(out + lo, (unsigned long long)__double_as_longlong(ratio));
 }
 
 // ---------------------------------------------------------------------------
@@ -758,10 +770,15 @@ struct clh_solver {
 
     // Jacobi step scale in use (clh_config.relax resolved; CLH_RELAX_AUTO:
     // from the Gershgorin bound of D^-1 K of the constants in relax_args)
+    // (batches with CLH_RELAX_AUTO: each cloth's gamma in cloth_relax, from
+    // its own bound; relax_eff and jacobi_bound are then the smallest gamma
+    // and the largest bound)
     float relax_eff = 1.0f;
     double jacobi_bound = 0.0;
     bool relax_ready = false;
     clh_step_args relax_args = {};
+    DevBuf<float> cloth_relax;
+    bool cloth_relax_on = false;
 
     // statistics of the last frame
     int last_iters = 0;
@@ -1024,6 +1041,7 @@ int enqueue_iteration(clh_solver* s, const clh_step_args& a, int k, float omega,
         f.par = par;
         f.cctl = (ctl && s->num_cloths > 1) ? s->cctl.ptr : nullptr;
         f.relax = s->relax_eff;
+        f.cloth_relax = s->cloth_relax_on ? s->cloth_relax.ptr : nullptr;
         CLH_TRY(fused_launch_iteration(s->fused, f, c.mode, s->stream));
         if (check) CLH_TRY(enqueue_fused_reduce(s, check_slot, ctl, par, 0));
         return hip_status(hipGetLastError(), "fused iteration launch");
@@ -1122,6 +1140,7 @@ int enqueue_pair(clh_solver* s, const clh_step_args& a, int k, float omega0, boo
     f.par = par;
     f.cctl = nullptr;  // batches to a tolerance run one iteration per launch
     f.relax = s->relax_eff;
+    f.cloth_relax = s->cloth_relax_on ? s->cloth_relax.ptr : nullptr;
     const int blend1 = (c.use_chebyshev && blend1_in && omega1 != 1.0f) ? 1 : 0;
     CLH_TRY(fused_launch_pair(s->fused, f, s->it[(k + 2) % kIterBufs].ptr, omega1, blend1,
                               s->stream));
@@ -1221,6 +1240,7 @@ int ensure_relax(clh_solver* s, const clh_step_args& a)
     const clh_config& c = s->cfg;
     if (!(c.relax < 0.0f && c.mode == CLH_MODE_PD)) {
         s->relax_eff = c.relax > 0.0f ? c.relax : 1.0f;
+        s->cloth_relax_on = false;
         return CLH_OK;
     }
     const clh_step_args& r = s->relax_args;
@@ -1231,25 +1251,40 @@ int ensure_relax(clh_solver* s, const clh_step_args& a)
         return CLH_OK;
     const bool use_stretch = s->T > 0 && a.d_stretch_k != nullptr;
     const bool use_bend = s->E > 0 && a.d_bend_quads != nullptr;
+    const int C = s->num_cloths;
     DevBuf<unsigned long long> mx;
-    CLH_TRY(mx.resize(1));
-    CLH_HIP(hipMemsetAsync(mx.ptr, 0, sizeof(unsigned long long), s->stream));
+    DevBuf<int> seg;
+    CLH_TRY(mx.resize((size_t)C));
+    if (C > 1) CLH_TRY(seg.upload(s->seg));
+    CLH_HIP(hipMemsetAsync(mx.ptr, 0, sizeof(unsigned long long) * (size_t)C, s->stream));
     if (s->N > 0)
         hipLaunchKernelGGL(k_jacobi_bound, dim3(grid_for(s->N)), dim3(kBlock), 0, s->stream,
                            s->vt_ptr.ptr, s->vt_idx.ptr, s->ve_ptr.ptr, s->ve_idx.ptr, a.d_mass,
                            a.d_jacobi_diag, a.d_Dm_inv, a.d_rest_area, a.d_stretch_k, a.d_bend_w,
                            a.d_bend_k, s->N, (double)c.dt * (double)c.dt, use_stretch ? 1 : 0,
-                           use_bend ? 1 : 0, mx.ptr);
+                           use_bend ? 1 : 0, C > 1 ? seg.ptr : nullptr, C, mx.ptr);
     CLH_HIP(hipGetLastError());
-    unsigned long long bits = 0;
-    CLH_HIP(hipMemcpyAsync(&bits, mx.ptr, sizeof(bits), hipMemcpyDeviceToHost, s->stream));
+    std::vector<unsigned long long> bits((size_t)C, 0);
+    CLH_HIP(hipMemcpyAsync(bits.data(), mx.ptr, sizeof(unsigned long long) * (size_t)C,
+                           hipMemcpyDeviceToHost, s->stream));
     CLH_HIP(hipStreamSynchronize(s->stream));
-    double mu = 0.0;
-    std::memcpy(&mu, &bits, sizeof(mu));
-    if (!std::isfinite(mu)) return fail(CLH_ERR_NUMERIC, "clh_solver_step: non-finite constants");
-    s->jacobi_bound = mu;
     const double rho = std::min(std::max((double)c.rho, 0.0), 1.0);
-    s->relax_eff = (mu > 1.0 + rho) ? (float)((1.0 + rho) / mu) : 1.0f;
+    std::vector<float> gamma((size_t)C, 1.0f);
+    double mu_max = 0.0;
+    float g_min = 1.0f;
+    for (int i = 0; i < C; ++i) {
+        double mu = 0.0;
+        std::memcpy(&mu, &bits[(size_t)i], sizeof(mu));
+        if (!std::isfinite(mu))
+            return fail(CLH_ERR_NUMERIC, "clh_solver_step: non-finite constants");
+        gamma[(size_t)i] = (mu > 1.0 + rho) ? (float)((1.0 + rho) / mu) : 1.0f;
+        mu_max = std::max(mu_max, mu);
+        g_min = std::min(g_min, gamma[(size_t)i]);
+    }
+    if (C > 1) CLH_TRY(s->cloth_relax.upload(gamma));
+    s->cloth_relax_on = C > 1;
+    s->jacobi_bound = mu_max;
+    s->relax_eff = g_min;
     s->relax_args = a;
     s->relax_ready = true;
     return CLH_OK;
@@ -1363,7 +1398,7 @@ int step_impl(clh_solver* s, const clh_step_args& a)
             f.pin_target = a.d_pin_target;
             f.use_stretch = (s->T > 0 && a.d_stretch_k != nullptr) ? 1 : 0;
             f.use_bend = (s->E > 0 && a.d_bend_quads != nullptr) ? 1 : 0;
-            f.relax = s->relax_eff;
+            f.relax = s->relax_eff;  // single cloth: cloth_relax stays null
             double* const its[kIterBufs] = {s->it[0].ptr, s->it[1].ptr, s->it[2].ptr, s->it[3].ptr};
             CLH_TRY(fused_launch_persist(s->fused, f, its, k0, k1, K, P, s->ctl.ptr,
                                          s->partials.ptr, s->hist.ptr, s->bar.ptr, s->stream));

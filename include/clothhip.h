@@ -147,7 +147,8 @@ typedef struct clh_config {
                                CLH_RELAX_AUTO: gamma = min(1, (1 + rho) / mu), mu the
                                Gershgorin bound of D^-1 K computed on the device from
                                the constants, so every eigenvalue of the damped Jacobi
-                               iteration lies in [-rho, 1) */
+                               iteration lies in [-rho, 1). In a batch each cloth takes
+                               gamma from its own bound, as it would alone */
 } clh_config;
 
 #define CLH_RELAX_AUTO (-1.0f)
@@ -241,8 +242,10 @@ typedef struct clh_step_stats {
     int    frames;       /* frames stepped since creation or reset           */
     int    restarts;     /* Chebyshev restarts of the last frame (one solve; 0 in a batch) */
     float  rho;          /* rho in use when the last frame ended (one solve) */
-    float  relax;        /* Jacobi step scale gamma of the last frame */
-    double jacobi_bound; /* CLH_RELAX_AUTO: the Gershgorin bound of D^-1 K it used (else 0) */
+    float  relax;        /* Jacobi step scale gamma of the last frame (a batch under
+                            CLH_RELAX_AUTO: the smallest over its cloths) */
+    double jacobi_bound; /* CLH_RELAX_AUTO: the Gershgorin bound of D^-1 K it used, the
+                            largest over a batch's cloths (else 0) */
 } clh_step_stats;
 
 /* FP64 relative residual ||b - K x_0|| / ||b|| of the frame's first iterate

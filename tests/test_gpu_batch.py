@@ -8,7 +8,7 @@ import pytest
 
 import clothhip as ch
 import pyoracle as po
-from helpers import cfg4_specs, extent, oracle_mesh, scene_arrays
+from helpers import cfg4_specs, extent, jacobi_bound_np, oracle_mesh, scene_arrays
 from test_batch import SPECS
 
 pytestmark = pytest.mark.gpu
@@ -58,13 +58,22 @@ def run_solo(spec, c, frames, accel=None, variant=ch.KERNEL_AUTO):
     return s.get_state()
 
 
-@pytest.mark.parametrize("mode", [ch.MODE_PD, ch.MODE_REF_CUDA])
-def test_batch_equals_solo_bitwise(mode):
+@pytest.mark.parametrize("mode,kw", [
+    (ch.MODE_PD, {}),
+    (ch.MODE_REF_CUDA, {}),
+    (ch.MODE_PD, dict(relax=ch.RELAX_AUTO, iters_per_launch=1)),
+    (ch.MODE_PD, dict(relax=ch.RELAX_AUTO, iters_per_launch=2)),
+], ids=["pd", "ref_cuda", "pd_relaxed", "pd_relaxed_pairs"])
+def test_batch_equals_solo_bitwise(mode, kw):
+    """With CLH_RELAX_AUTO each cloth of a batch takes the step scale of its
+    own bound (here from 1 for the unbent cloth to 0.63 for the folding
+    one), so it still equals its solo frame bit for bit, in one- and
+    two-iteration launches."""
     b = batch_scene()
     K, F = 12, 3
-    pos, vel, _ = run_batch(b, cfg(mode, K), F, ACCEL)
+    pos, vel, _ = run_batch(b, cfg(mode, K, **kw), F, ACCEL)
     for c, spec in enumerate(SPECS):
-        sp, sv = run_solo(spec, cfg(mode, K), F, ACCEL[c])
+        sp, sv = run_solo(spec, cfg(mode, K, **kw), F, ACCEL[c])
         lo, hi = b.vert_off[c], b.vert_off[c + 1]
         np.testing.assert_array_equal(pos[lo:hi], sp, err_msg=f"positions of cloth {c}")
         np.testing.assert_array_equal(vel[lo:hi], sv, err_msg=f"velocities of cloth {c}")
@@ -110,8 +119,9 @@ def test_batch_against_oracle_fp64():
 
 def test_batch_tolerance_every_cloth_converges():
     """Frames to a relative residual of 1e-5: the batch stops only when every
-    cloth meets it. (The single-pin irregular cloth is left out: Jacobi with
-    Chebyshev stalls on it near 4e-2 after it folds, in the oracle too.)"""
+    cloth meets it. (The single-pin irregular cloth is left out here: with
+    the plain Jacobi step, Chebyshev stalls on it near 4e-2 after it folds,
+    in the oracle too; the damped step brings it back in the next test.)"""
     specs = SPECS[:3]
     b = batch_scene(specs)
     tol = 1e-5
@@ -128,6 +138,41 @@ def test_batch_tolerance_every_cloth_converges():
         assert np.all(r <= tol), r
         # the batch stops at the first check where the slowest cloth converged
         assert st.iterations % 20 == 0 or st.iterations == 20000
+
+
+def test_batch_relaxed_every_cloth_converges():
+    """The damped Jacobi step with each cloth's own bound (CLH_RELAX_AUTO):
+    all four cloths, the single-pin irregular one included (Gershgorin bound
+    3.145 of D^-1 K, step scale 0.636), reach a relative residual of 1e-5 in
+    every one of four frames, as in the FP32 oracle with that scale (500,
+    760, 920 and 1,040 iterations for the folding cloth). The stats report
+    the largest bound and the smallest scale of the batch."""
+    b = batch_scene(SPECS)
+    tol = 1e-5
+    c = cfg(ch.MODE_PD, 20000, tolerance=tol, check_every=20, rho=0.999,
+            relax=ch.RELAX_AUTO)
+    sol = ch.Solver(c, b.N, b.T, b.E, cloth_offsets=b.vert_off)
+    args = b.step_args()
+    acc = ch.DeviceArray.from_host(ACCEL)
+    args.d_cloth_accel = acc.ptr
+    for f in range(4):
+        sol.step(args)
+        st, it = sol.cloth_status()
+        r = sol.cloth_residuals()
+        print(f"frame {f}: states {st.tolist()}, iterations {it.tolist()}, "
+              f"residuals {np.array2string(r, precision=2)}", flush=True)
+        assert np.all(st == ch.CLOTH_CONVERGED), st
+        assert np.all(r <= tol), r
+        assert sol.stats().converged == 1
+        assert it[3] <= 2000, it
+    pos, vel = b.get_state()
+    assert np.all(np.isfinite(pos)) and np.all(np.isfinite(vel))
+    bounds = [jacobi_bound_np(scene_arrays(ch.Scene(**spec)), DT) for spec in SPECS]
+    stats = sol.stats()
+    assert abs(stats.jacobi_bound - max(bounds)) <= 1e-12 * max(bounds)
+    rho = float(np.float32(0.999))
+    want = min(np.float32((1.0 + rho) / m) if m > 1.0 + rho else np.float32(1.0) for m in bounds)
+    assert stats.relax == want
 
 
 def test_batch_tolerance_cloth_follows_its_solo_frame():
