@@ -30,6 +30,14 @@ kernel), read from the committed profile summary for the launch shape the
 autotuner chose (null when that shape was not profiled). The same figures for
 the 1024x1024 irregular mesh and the 512-cloth batch are reported under
 roofline.mesh_1024 and roofline.batched_512.
+
+cpu_baseline times the C++ oracle (FP32 storage, FP64 iterates) on the
+host's cores: frames of the workload to the tolerance (`value`), one thread
+and all threads at 20 iterations per frame, the reference's CPU backend on
+the 32x32 workload, and under `plan` the 32x32 workload for 200 frames in
+both modes, the 1024x1024 mesh per frame, and one cloth of the 64-cloth
+random-wind workload, each beside the GPU's figure for the same frames and
+the GPU's error against the FP64 oracle.
 """
 import argparse
 import json
@@ -186,7 +194,154 @@ def tolerance_frame_error(ch, s, sol, args_c, pos_w, vel_w, args):
     return out
 
 
-def cpu_baseline(ch, s, sol, pos_w, vel_w, args):
+def cpu_plan_32(ch, threads):
+    """The 32x32 workload (10 iterations/frame, Chebyshev rho 0.99, 200
+    frames from rest) in reference mode and PD mode: the C++ oracle in FP32
+    on one thread and on `threads`, the GPU solver on the same frames, and
+    the GPU trajectory's error after 200 frames against the oracle's FP32
+    and FP64 trajectories (max |x_gpu - x_oracle| / extent). In reference
+    mode any two of these trajectories part by about 5e-2 of the extent
+    within 20 frames, the FP32 and FP64 oracles too: the reference's
+    rotation takes its V = I branch on a discriminant below 1e-10, which
+    rounding decides for the nearly undeformed triangles, so reference mode
+    is pinned one iteration at a time from identical inputs instead."""
+    import numpy as np
+    import pyoracle as po
+    from helpers import CFG1, extent, oracle_mesh, scene_arrays
+    F = 200
+    out = {"workload": "32x32 workload, 10 iterations/frame, rho 0.99, 200 frames from rest"}
+    for name, mode, omode in (("ref_cuda", ch.MODE_REF_CUDA, po.REF_CUDA),
+                              ("pd", ch.MODE_PD, po.PD)):
+        s = build_scene(CFG1)
+        pos0, vel0 = s.get_state()
+        m = oracle_mesh(scene_arrays(s))
+        kw = dict(mode=omode, iterations=10, gravity=-9.81, dt=CFG1["dt"], rho=0.99, cheb_delay=1)
+        r = {}
+        for th in sorted({1, threads}):
+            p = po.params(threads=th, **kw)
+            x, v = pos0.copy(), vel0.copy()
+            t0 = time.time()
+            for _ in range(F):
+                x, v, *_ = po.step(m, p, x, v, "f32")
+            r[f"cpu_frames_per_s_{th}_threads"] = F / (time.time() - t0)
+        p = po.params(threads=threads, **kw)
+        x64, v64 = pos0.astype(np.float64), vel0.astype(np.float64)
+        for _ in range(F):
+            x64, v64, *_ = po.step(m, p, x64, v64, "f64")
+        sol = ch.Solver(ch.config(mode=mode, max_iterations=10, gravity=-9.81, dt=CFG1["dt"],
+                                  rho=0.99, cheb_delay=1), s.N, s.T, s.E)
+        a = s.step_args()
+        sol.step(a)  # plan and graph, then the timed trajectory from rest
+        s.set_state(pos0, vel0)
+        ms = sol.run_frames(a, F)
+        xg, _ = s.get_state()
+        r["gpu_frames_per_s"] = F * 1e3 / ms
+        r["fp32_oracle_error_200_frames"] = float(np.abs(xg - x).max() / extent(pos0))
+        r["fp64_oracle_error_200_frames"] = float(np.abs(xg.astype(np.float64) - x64).max()
+                                                  / extent(pos0))
+        out[name] = r
+        log(f"cpu plan 32x32 {name}: " + json.dumps(r))
+        del sol, s
+    return out
+
+
+def cpu_plan_1024(ch, s, sol, a, threads, frames=2):
+    """The 1024x1024 irregular workload at 30 iterations/frame: the C++
+    oracle in FP32 on `threads`, seconds per frame over `frames` frames from
+    rest, and the GPU's first frame from rest against the oracle's FP64
+    frame (max |x_gpu - x_fp64| / extent)."""
+    import numpy as np
+    import pyoracle as po
+    from helpers import CFG3, extent, oracle_mesh, scene_arrays
+    pos0 = s.get("pos0").reshape(-1, 3).astype(np.float32)
+    vel0 = np.zeros_like(pos0)
+    t0 = time.time()
+    m = oracle_mesh(scene_arrays(s))
+    setup = time.time() - t0
+    kw = dict(mode=po.PD, iterations=30, gravity=-9.81, dt=CFG3["dt"], rho=0.99, cheb_delay=1,
+              threads=threads, relax=sol.stats().relax)
+    p = po.params(**kw)
+    x, v = pos0.copy(), vel0.copy()
+    t0 = time.time()
+    for _ in range(frames):
+        x, v, *_ = po.step(m, p, x, v, "f32")
+    per_frame = (time.time() - t0) / frames
+    x64, *_ = po.step(m, p, pos0, vel0, "f64")
+    s.set_state(pos0, vel0)
+    sol.step(a)
+    xg, _ = s.get_state()
+    out = {"workload": "1024x1024 irregular mesh, 30 iterations/frame, rho 0.99, from rest",
+           "cpu_seconds_per_frame": per_frame, "cpu_frames_per_s": 1.0 / per_frame,
+           "cpu_frames": frames, "cores": threads, "oracle_setup_s": setup,
+           "fp64_oracle_error_one_frame": float(np.abs(xg.astype(np.float64) - x64).max()
+                                                / extent(pos0))}
+    log("cpu plan 1024x1024: " + json.dumps(out))
+    return out
+
+
+def cpu_plan_cfg5(ch, threads, args, frames=2):
+    """One cloth of the 64-cloth random-wind workload (held-out cloth 0),
+    frames to the tolerance from rest at the benchmark's schedule: the GPU
+    solving it alone and the C++ oracle (FP32) on `threads`, and the GPU's
+    first frame against the FP64 oracle run for the GPU's iteration count
+    with the same checks (two runs stopped by the tolerance stop at
+    different checks when their arithmetic differs, and at a relative
+    residual of 1e-5 the positions of such a stiff cloth are bounded only to
+    about 1e-2 of the extent). The learned start is not deployed at 256x256
+    (its validation picks the plain start), so plain against plain is the
+    comparison the deployed solver makes."""
+    import numpy as np
+    import pyoracle as po
+    from helpers import CFG2, cfg5_specs, extent, oracle_mesh, scene_arrays
+    especs, eacc = cfg5_specs(64, seed=9, cloth_seed=9000)
+    s = build_scene(especs[0])
+    pos0, vel0 = s.get_state()
+    acc = ch.DeviceArray.from_host(np.ascontiguousarray(eacc[:1], np.float32))
+    a = s.step_args()
+    a.d_cloth_accel = acc.ptr
+    sol = ch.Solver(ch.config(mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
+                              tolerance=args.tol, check_every=args.check_every, stall_checks=40,
+                              max_iterations=8000, cheb_delay=args.cheb_delay, **schedule(args)),
+                    s.N, s.T, s.E)
+    sol.step(a)  # plan, bound and graphs
+    s.set_state(pos0, vel0)
+    gi = []
+    xg0 = None
+    t0 = time.time()
+    for f in range(frames):
+        sol.step(a)
+        gi.append(int(sol.stats().iterations))
+        if f == 0:
+            t_get = time.time()
+            xg0, _ = s.get_state()
+            t0 += time.time() - t_get
+    gpu_s = time.time() - t0
+    m = oracle_mesh(scene_arrays(s))
+    kw = dict(mode=po.PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho, cheb_delay=args.cheb_delay,
+              ext_accel=tuple(float(z) for z in eacc[0]), iterations=8000, tolerance=args.tol,
+              check_every=args.check_every, threads=threads, **oracle_schedule(args, sol))
+    p = po.params(**kw)
+    x, v = pos0.copy(), vel0.copy()
+    ci = []
+    t0 = time.time()
+    for _ in range(frames):
+        x, v, _, _, n = po.step(m, p, x, v, "f32")
+        ci.append(int(n))
+    cpu_s = time.time() - t0
+    kw.update(iterations=gi[0], tolerance=1e-30)
+    x64, *_ = po.step(m, po.params(**kw), pos0, vel0, "f64")
+    out = {"workload": f"held-out cloth 0 of the 64 x 256x256 random-wind workload, {frames} "
+                       f"frames from rest to relative residual {args.tol}, plain start",
+           "gpu_frames_per_s": frames / gpu_s, "gpu_iterations": gi,
+           "cpu_frames_per_s": frames / cpu_s, "cpu_iterations": ci, "cores": threads,
+           "fp64_oracle_error_frame_0": float(np.abs(xg0.astype(np.float64) - x64).max()
+                                              / extent(pos0)),
+           "fp64_oracle_iterations_frame_0": gi[0]}
+    log("cpu plan 64 x 256x256, one cloth: " + json.dumps(out))
+    return out
+
+
+def cpu_baseline(ch, s, sol, pos_w, vel_w, args, plan_1024=None):
     """The oracle (C++, FP32 storage and FP64 iterates, as the GPU) timed on
     the host: frames of the benchmark's workload to the tolerance at the
     benchmark's settings, from the state the GPU's timed frames start from,
@@ -223,6 +378,15 @@ def cpu_baseline(ch, s, sol, pos_w, vel_w, args):
         frames1 += 1
     dt1 = time.time() - t1
     it1 = frames1 * 20 / dt1
+    # all threads: frames/s at the workload's own 20 iterations per frame
+    p20 = po.params(iterations=20, threads=threads, **kw)
+    pos, vel = pos_w.copy(), vel_w.copy()
+    t1 = time.time()
+    frames20 = 0
+    while time.time() - t1 < args.cpu_seconds / 3:
+        pos, vel, *_ = po.step(m, p20, pos, vel, "f32")
+        frames20 += 1
+    fps20 = frames20 / (time.time() - t1)
     # the CPU reference backend on the 32x32 workload, 50 iterations a frame
     c1 = ch.Scene(**CFG1)
     ref_cpu = {"workload": "32x32 workload, CpuStretchReferenceSolver (stretch only, 50 "
@@ -248,7 +412,11 @@ def cpu_baseline(ch, s, sol, pos_w, vel_w, args):
             "host": host,
             "single_core": {"iterations_per_s": it1, "cores": 1,
                             "sample": f"{frames1} frames x 20 iterations in {dt1:.1f} s"},
-            "ref_cpu_32x32": ref_cpu}
+            "fixed_20_iterations": {"frames_per_s": fps20, "cores": threads,
+                                    "frames": frames20},
+            "ref_cpu_32x32": ref_cpu,
+            "plan": {"32x32": cpu_plan_32(ch, threads), "1024x1024": plan_1024,
+                     "cfg5_one_cloth": cpu_plan_cfg5(ch, threads, args)}}
 
 
 def persistent_frames(ch, s, args_c, pos_w, vel_w, common, args):
@@ -615,7 +783,7 @@ def launch_timing(ch, args):
         a = sc.step_args()
         dt = spec.get("dt", 1 / 60)
         kw = dict(mode=ch.MODE_PD, gravity=-9.81, dt=dt, rho=0.99, cheb_delay=1,
-                  max_iterations=20, **extra)
+                  max_iterations=20, relax=args.relax, **extra)
         offs = dict(cloth_offsets=sc.vert_off) if "specs" in spec else {}
         sol = ch.Solver(ch.config(**kw), sc.N, sc.T, sc.E, **offs)
         sol.autotune(a, n_tune)
@@ -794,11 +962,15 @@ def main():
         log("frame to the tolerance vs FP64 oracle: " + json.dumps(tol_err))
 
     roof_1024 = None
+    plan_1024 = None
     if not args.no_1024:
         s3 = build_scene(CFG3)
         a3 = s3.step_args()
+        # the damped step: the irregular mesh's obtuse triangles put the
+        # Gershgorin bound of D^-1 K near 4.6, and plain Jacobi steps under the
+        # Chebyshev weights diverge within one frame
         sol3 = ch.Solver(ch.config(max_iterations=30, mode=ch.MODE_PD, gravity=-9.81,
-                                   dt=CFG3["dt"], rho=0.99, cheb_delay=1,
+                                   dt=CFG3["dt"], rho=0.99, cheb_delay=1, relax=args.relax,
                                    kernel_variant=args.variant), s3.N, s3.T, s3.E)
         rep3 = sol3.autotune(a3, 20)
         tune1024 = rep3.candidates()
@@ -817,11 +989,15 @@ def main():
                      "us_per_iteration": t3 * 1e3 / pi3.iters_per_launch,
                      "bytes_per_launch": b3, "achieved": bw3, "frac": bw3 / HBM_PEAK_GBS,
                      "traffic": tr3, "frames_per_s_30_iterations": 1e3 / ms3,
+                     "relax": float(sol3.stats().relax),
+                     "jacobi_bound": float(sol3.stats().jacobi_bound),
                      "tile_verts": pi3.tile_verts, "block_threads": pi3.block_threads,
                      "autotune": tune1024}
         log(f"1024x1024: {t3 * 1e3:.1f} us per launch of {pi3.iters_per_launch} iteration(s), "
             f"{bw3:.0f} GB/s ({100 * bw3 / HBM_PEAK_GBS:.1f}% of peak); "
             f"{1e3 / ms3:.1f} frames/s at 30 iterations")
+        if not args.no_cpu:
+            plan_1024 = cpu_plan_1024(ch, s3, sol3, a3, host_cpu()["omp_threads"])
         del sol3, s3
 
     # ---- 512 batched cloths of uneven size, one launch per iteration -----------
@@ -834,8 +1010,8 @@ def main():
         ab = sb.step_args()
         log(f"batch: {sb.C} cloths, N={sb.N} T={sb.T} E={sb.E} built in {time.time() - t0:.1f} s")
         solb = ch.Solver(ch.config(max_iterations=15, mode=ch.MODE_PD, gravity=-9.81,
-                                   dt=specs[0]["dt"], rho=0.99, cheb_delay=1), sb.N, sb.T, sb.E,
-                         cloth_offsets=sb.vert_off)
+                                   dt=specs[0]["dt"], rho=0.99, cheb_delay=1, relax=args.relax),
+                         sb.N, sb.T, sb.E, cloth_offsets=sb.vert_off)
         t0 = time.time()
         rep = solb.autotune(ab, 10)
         cands = rep.candidates()
@@ -859,7 +1035,9 @@ def main():
                  "traffic": profiled_traffic(TRAFFIC_FILE_BATCH, pib),
                  "tile_verts": pib.tile_verts, "block_threads": pib.block_threads,
                  "autotune": {"best": cands[rep.best], "candidates": cands},
-                 "padded_verts": int(pib.padded_verts)}
+                 "padded_verts": int(pib.padded_verts),
+                 "relax_min": float(solb.stats().relax),
+                 "jacobi_bound_max": float(solb.stats().jacobi_bound)}
         log(f"batch: {1e3 / msb:.1f} frames/s ({sb.C * 1e3 / msb:.0f} cloth-frames/s), launch of "
             f"{pib.iters_per_launch} iteration(s) {tb * 1e3:.0f} us = {bwb:.0f} GB/s "
             f"({100 * bwb / HBM_PEAK_GBS:.1f}% of peak)")
@@ -879,7 +1057,7 @@ def main():
     # ---- CPU baseline: the oracle's FP32 PD frames on the host cores -------------
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(ch, s, sol_tol, pos_w, vel_w, args)
+        cpu = cpu_baseline(ch, s, sol_tol, pos_w, vel_w, args, plan_1024)
 
     value = fps_tol
     line = {

@@ -1702,6 +1702,7 @@ extern "C" int clh_solver_time_iteration_kernel(clh_solver* s, const clh_step_ar
     CLH_TRY(validate_args(s, a));
     CLH_TRY(ensure_plan(s, *a));
     CLH_TRY(ensure_scratch(s));
+    CLH_TRY(ensure_relax(s, *a));  // the timed launches use the frames' step scale
     CLH_TRY(time_iterations(s, *a, num_launches, avg_ms));
     return CLH_OK;
 }
@@ -1845,6 +1846,7 @@ extern "C" int clh_solver_autotune(clh_solver* s, const clh_step_args* a, int nu
     CLH_TRY(ensure_plan(s, *a));
     if (s->variant != CLH_KERNEL_FUSED)
         return fail(CLH_ERR_UNSUPPORTED, "clh_solver_autotune: the mesh has no fused plan");
+    CLH_TRY(ensure_relax(s, *a));  // candidates run at the frames' step scale
     out->key = s->shape_key;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);

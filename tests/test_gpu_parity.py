@@ -360,6 +360,42 @@ def test_cfg3_full_size_frame():
     assert rel_err(xr, orr, extent(pos)) < 1e-5
 
 
+def test_cfg3_damped_frame():
+    """The 1024x1024 workload's own frame (30 iterations, rho 0.99) with the
+    damped Jacobi step (CLH_RELAX_AUTO). The irregular mesh's obtuse
+    triangles put the Gershgorin bound of D^-1 K at 4.63 (the device's bound
+    equals the numpy restatement); with the plain step the frame diverges
+    (the FP64 relative residual reaches about 3e8 in the oracle). With
+    gamma = (1 + rho) / bound the FP64 relative residuals at iterations 10,
+    20 and 30 equal the FP32 oracle's run with that gamma to 1e-5 relative
+    (measured: 1e-7) and fall from their start. (The frame moves vertices by about 5e-8 m,
+    the size of an FP32 ulp of the positions, so the residuals, not the
+    positions, carry the comparison.)"""
+    from helpers import jacobi_bound_np
+    s, arr = make(CFG3)
+    pos, vel = s.get_state()
+    sol = solver_for(s, **pd_cfg(30, CFG3["dt"], relax=ch.RELAX_AUTO, tolerance=1e-30,
+                                 check_every=10))
+    sol.step(s.step_args())
+    st = sol.stats()
+    want = jacobi_bound_np(arr, CFG3["dt"])
+    print(f"  bound {st.jacobi_bound:.6f} (numpy {want:.6f}), gamma {st.relax:.6f}", flush=True)
+    assert abs(st.jacobi_bound - want) <= 1e-12 * want
+    assert 4.0 < st.jacobi_bound < 5.0
+    hist = sol.residual_history()
+    assert st.iterations == 30 and len(hist) == 3
+    p = po.params(mode=po.PD, iterations=30, gravity=-9.81, dt=CFG3["dt"], rho=0.99, cheb_delay=1,
+                  tolerance=1e-30, check_every=10, threads=16, relax=st.relax)
+    _, _, _, rs, n = po.step(oracle_mesh(arr), p, pos, vel, "f32", resid=True)
+    assert n == 30
+    want_h = np.sqrt(rs[[9, 19, 29], 0] / rs[[9, 19, 29], 1])
+    print(f"  residuals {hist} (oracle {want_h})", flush=True)
+    np.testing.assert_allclose(hist, want_h, rtol=1e-5)
+    assert hist[2] < hist[0]
+    xg, _ = s.get_state()
+    assert np.abs(xg - pos).max() < 1e-3
+
+
 def test_reference_bounds_through_cpp_api():
     """The reference's Test 1 (test_pd_constraints.cpp:151-187) through the C++
     PDSolver in reference mode on the GPU."""

@@ -55,3 +55,31 @@ def test_plain_jacobi_diverges_where_the_bound_says():
         assert (mu > 2.0) == above, (bend_k, mu)
         assert mu <= jacobi_bound_np(arr, 1 / 60) * (1 + 1e-9)
     assert jacobi_bound_np(arr, 1 / 60, bend=False) < 2.0
+
+
+def test_irregular_workload_needs_the_damped_step():
+    """The 1024x1024 workload's mesh family (interior vertices jittered by
+    0.3 of the spacing, a random diagonal per quad, stretch log-uniform in
+    [5e2, 5e3]) at 64x64: obtuse triangles put the bound of D^-1 K near 4,
+    and one 30-iteration frame from rest with the plain Jacobi step under
+    Chebyshev weights (rho 0.99) grows its residual to above 1e3 and moves
+    vertices by hundreds of metres (the 1024x1024 mesh's bound is 4.63 and
+    its frame reaches 8e4 m); the damped step gamma = (1 + rho) / bound
+    keeps the frame's residual below 1e-4 and its motion below a millimetre."""
+    import pyoracle as po
+    from helpers import CFG3, oracle_mesh
+    spec = dict(CFG3, nrows=64, ncols=64, size=1 / 63)
+    arr = scene_arrays(ch.Scene(**spec))
+    m = oracle_mesh(arr)
+    mu = jacobi_bound_np(arr, spec["dt"])
+    assert mu > 3.5, mu
+    rho = float(np.float32(0.99))
+    pos0 = arr["pos0"]
+    out = {}
+    for name, relax in (("plain", 1.0), ("damped", (1.0 + rho) / mu)):
+        p = po.params(mode=po.PD, iterations=30, gravity=-9.81, dt=spec["dt"], rho=0.99,
+                      cheb_delay=1, relax=relax)
+        x, _, _, rs, n = po.step(m, p, pos0, np.zeros_like(pos0), "f32", resid=True)
+        out[name] = (np.sqrt(rs[n - 1, 0] / rs[n - 1, 1]), np.abs(x - pos0).max())
+    assert out["plain"][0] > 1e3 and out["plain"][1] > 1.0, out
+    assert out["damped"][0] < 1e-4 and out["damped"][1] < 1e-3, out
