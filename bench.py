@@ -62,6 +62,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# hipGraph capture of the frames of every solver the benchmark creates
+# (--no-graph clears it: chunks and fixed-length frames are then enqueued
+# directly, with the same kernels and bits)
+USE_GRAPH = 1
+
+
+def solver_config(ch, **kw):
+    """clh_config with the benchmark's graph setting unless kw names one."""
+    kw.setdefault("use_graph", USE_GRAPH)
+    return ch.config(**kw)
+
+
 def schedule(args):
     """The frame schedule's damping and restart settings (clh_config)."""
     return dict(relax=args.relax, restart_rule=args.restart_rule,
@@ -136,7 +148,7 @@ def fp64_error(ch, s, pos0, vel0, common, iterations):
     import pyoracle as po
     from helpers import CFG2, CFG2_WIND, extent, oracle_mesh, scene_arrays
     s.set_state(pos0, vel0)
-    sol = ch.Solver(ch.config(max_iterations=iterations, **common), s.N, s.T, s.E)
+    sol = ch.Solver(solver_config(ch, max_iterations=iterations, **common), s.N, s.T, s.E)
     sol.step(s.step_args())
     x_gpu, _ = s.get_state()
     m = oracle_mesh(scene_arrays(s))
@@ -228,8 +240,8 @@ def cpu_plan_32(ch, threads):
         x64, v64 = pos0.astype(np.float64), vel0.astype(np.float64)
         for _ in range(F):
             x64, v64, *_ = po.step(m, p, x64, v64, "f64")
-        sol = ch.Solver(ch.config(mode=mode, max_iterations=10, gravity=-9.81, dt=CFG1["dt"],
-                                  rho=0.99, cheb_delay=1), s.N, s.T, s.E)
+        sol = ch.Solver(solver_config(ch, mode=mode, max_iterations=10, gravity=-9.81,
+                                      dt=CFG1["dt"], rho=0.99, cheb_delay=1), s.N, s.T, s.E)
         a = s.step_args()
         sol.step(a)  # plan and graph, then the timed trajectory from rest
         s.set_state(pos0, vel0)
@@ -299,10 +311,10 @@ def cpu_plan_cfg5(ch, threads, args, frames=2):
     acc = ch.DeviceArray.from_host(np.ascontiguousarray(eacc[:1], np.float32))
     a = s.step_args()
     a.d_cloth_accel = acc.ptr
-    sol = ch.Solver(ch.config(mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
-                              tolerance=args.tol, check_every=args.check_every, stall_checks=40,
-                              max_iterations=8000, cheb_delay=args.cheb_delay, **schedule(args)),
-                    s.N, s.T, s.E)
+    cfg = solver_config(ch, mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
+                        tolerance=args.tol, check_every=args.check_every, stall_checks=40,
+                        max_iterations=8000, cheb_delay=args.cheb_delay, **schedule(args))
+    sol = ch.Solver(cfg, s.N, s.T, s.E)
     sol.step(a)  # plan, bound and graphs
     s.set_state(pos0, vel0)
     gi = []
@@ -430,9 +442,9 @@ def persistent_frames(ch, s, args_c, pos_w, vel_w, common, args):
     ends = {}
     for name, extra in (("launch", {}), ("persistent", dict(persistent=ch.PERSIST_ON))):
         s.set_state(pos_w, vel_w)
-        sol = ch.Solver(ch.config(max_iterations=args.max_iters, tolerance=args.tol,
-                                  check_every=args.check_every, tile_verts=256,
-                                  block_threads=1024, **extra, **common), s.N, s.T, s.E)
+        sol = ch.Solver(solver_config(ch, max_iterations=args.max_iters, tolerance=args.tol,
+                                      check_every=args.check_every, tile_verts=256,
+                                      block_threads=1024, **extra, **common), s.N, s.T, s.E)
         # the plan, the step-scale bound and the chunk graphs are built in
         # warm-up frames; the timed frames start again from the same state
         for _ in range(2):
@@ -475,9 +487,9 @@ def warm_start_study(ch, n, rho, args, pos0=None, vel0=None, scene=None, gen_tol
     # cloth-frames that miss it are left out
     gen_tol = gen_tol or args.ws_gen_tol
     gen_mode = gen_mode or args.ws_gen_mode
-    gen = ch.Solver(ch.config(max_iterations=args.ws_gen_cap, tolerance=gen_tol,
-                              check_every=50,
-                              mode=ch.MODE_PD, gravity=-9.81, dt=dt, rho=rho, cheb_delay=1),
+    gen = ch.Solver(solver_config(ch, max_iterations=args.ws_gen_cap, tolerance=gen_tol,
+                                  check_every=50,
+                                  mode=ch.MODE_PD, gravity=-9.81, dt=dt, rho=rho, cheb_delay=1),
                     b.N, b.T, b.E, cloth_offsets=b.vert_off)
     wsm = ch.WarmStart.for_scene(b, dt, -9.81, seed=7)
     wsm.dataset_reserve(args.ws_frames * (b.N + 16 * b.C))
@@ -505,9 +517,9 @@ def warm_start_study(ch, n, rho, args, pos0=None, vel0=None, scene=None, gen_tol
     t2 = time.time()
     vspecs, vacc = cfg5_specs(8, seed=9, n=n, cloth_seed=9000, wind_max=0.5)
     vb = build_scene(dict(specs=vspecs))
-    vsol = ch.Solver(ch.config(max_iterations=3 * args.ws_gen_cap, tolerance=args.tol,
-                               check_every=args.check_every, mode=ch.MODE_PD, gravity=-9.81,
-                               dt=dt, rho=rho, cheb_delay=1), vb.N, vb.T, vb.E,
+    vsol = ch.Solver(solver_config(ch, max_iterations=3 * args.ws_gen_cap, tolerance=args.tol,
+                                   check_every=args.check_every, mode=ch.MODE_PD, gravity=-9.81,
+                                   dt=dt, rho=rho, cheb_delay=1), vb.N, vb.T, vb.E,
                      cloth_offsets=vb.vert_off)
     vws = ch.WarmStart.for_scene(vb, dt, -9.81)
     vws.set_params(params, scale)
@@ -528,11 +540,12 @@ def warm_start_study(ch, n, rho, args, pos0=None, vel0=None, scene=None, gen_tol
     cap = None
     for mode in ("plain", "extrap", "learned"):
         scene.set_state(pos0, vel0)
-        full = ch.Solver(ch.config(max_iterations=args.max_iters, **common), scene.N, scene.T,
-                         scene.E)
+        full = ch.Solver(solver_config(ch, max_iterations=args.max_iters, **common), scene.N,
+                         scene.T, scene.E)
         warm = None
         if mode != "plain":
-            warm = ch.Solver(ch.config(max_iterations=cap, **common), scene.N, scene.T, scene.E)
+            warm = ch.Solver(solver_config(ch, max_iterations=cap, **common), scene.N, scene.T,
+                             scene.E)
         for sol_m in (full, warm):
             if sol_m is not None:
                 sol_m.autotune(scene.step_args(), 50)
@@ -624,7 +637,7 @@ def warm_start_cfg5(ch, args):
                   **schedule(args))
 
     def solver_for(b, delay):
-        return ch.Solver(ch.config(cheb_delay=delay, **common), b.N, b.T, b.E,
+        return ch.Solver(solver_config(ch, cheb_delay=delay, **common), b.N, b.T, b.E,
                          cloth_offsets=b.vert_off)
 
     especs, eacc = cfg5_specs(64, seed=9, cloth_seed=9000)
@@ -739,10 +752,10 @@ def ws_kernels(ch, args):
     from warmstart import FrameLoop
     specs, acc = cfg5_specs(64)
     b = build_scene(dict(specs=specs))
-    sol = ch.Solver(ch.config(mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
-                              cheb_delay=args.cheb_delay, tolerance=args.tol,
-                              check_every=args.check_every, max_iterations=3000,
-                              stall_checks=40, use_graph=0),
+    sol = ch.Solver(solver_config(ch, mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
+                                  cheb_delay=args.cheb_delay, tolerance=args.tol,
+                                  check_every=args.check_every, max_iterations=3000,
+                                  stall_checks=40, use_graph=0),
                     b.N, b.T, b.E, cloth_offsets=b.vert_off)
     ws = ch.WarmStart.for_scene(b, CFG2["dt"], -9.81, seed=7)
     ws.dataset_reserve(6 * (b.N + 16 * b.C))
@@ -785,7 +798,7 @@ def launch_timing(ch, args):
         kw = dict(mode=ch.MODE_PD, gravity=-9.81, dt=dt, rho=0.99, cheb_delay=1,
                   max_iterations=20, relax=args.relax, **extra)
         offs = dict(cloth_offsets=sc.vert_off) if "specs" in spec else {}
-        sol = ch.Solver(ch.config(**kw), sc.N, sc.T, sc.E, **offs)
+        sol = ch.Solver(solver_config(ch, **kw), sc.N, sc.T, sc.E, **offs)
         sol.autotune(a, n_tune)
         t = launch_ms(sol, a, n_time)
         pi = sol.plan_info()
@@ -825,7 +838,7 @@ def main():
     ap.add_argument("--restart-factor", type=float, default=10.0)
     ap.add_argument("--variant", type=int, default=0, help="0 auto, 1 unfused, 2 fused")
     ap.add_argument("--no-graph", action="store_true",
-                    help="enqueue the 256x256 frames' launches directly instead of replaying "
+                    help="enqueue every solver's launches directly instead of replaying "
                          "captured hipGraphs (the kernels and their bits are the same)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -858,6 +871,8 @@ def main():
                     help="iteration cap of a training frame (cloths missing the tolerance "
                          "are left out of the training set)")
     args = ap.parse_args()
+    global USE_GRAPH
+    USE_GRAPH = 0 if args.no_graph else 1
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this benchmark measures one GPU; run it with --gpus 1"}))
@@ -885,8 +900,8 @@ def main():
     common = dict(mode=ch.MODE_PD, gravity=-9.81, dt=CFG2["dt"], rho=args.rho,
                   cheb_delay=args.cheb_delay, ext_accel=CFG2_WIND, kernel_variant=args.variant,
                   use_graph=0 if args.no_graph else 1, **schedule(args))
-    sol_tol = ch.Solver(ch.config(max_iterations=args.max_iters, tolerance=args.tol,
-                                  check_every=args.check_every, **common), N, T, E)
+    sol_tol = ch.Solver(solver_config(ch, max_iterations=args.max_iters, tolerance=args.tol,
+                                      check_every=args.check_every, **common), N, T, E)
     rep256 = sol_tol.autotune(args_c, 200)
     tune256 = rep256.candidates()
     log("256x256 autotune: " + tune_line(tune256, 2))
@@ -922,7 +937,7 @@ def main():
 
     # ---- 256x256: fixed 20 iterations per frame --------------------------------
     s.set_state(pos0, vel0)
-    sol20 = ch.Solver(ch.config(max_iterations=20, **common), N, T, E)
+    sol20 = ch.Solver(solver_config(ch, max_iterations=20, **common), N, T, E)
     sol20.autotune(args_c, 200)  # the cached choice
     s.set_state(pos0, vel0)
     for _ in range(3):
@@ -969,9 +984,9 @@ def main():
         # the damped step: the irregular mesh's obtuse triangles put the
         # Gershgorin bound of D^-1 K near 4.6, and plain Jacobi steps under the
         # Chebyshev weights diverge within one frame
-        sol3 = ch.Solver(ch.config(max_iterations=30, mode=ch.MODE_PD, gravity=-9.81,
-                                   dt=CFG3["dt"], rho=0.99, cheb_delay=1, relax=args.relax,
-                                   kernel_variant=args.variant), s3.N, s3.T, s3.E)
+        sol3 = ch.Solver(solver_config(ch, max_iterations=30, mode=ch.MODE_PD, gravity=-9.81,
+                                       dt=CFG3["dt"], rho=0.99, cheb_delay=1, relax=args.relax,
+                                       kernel_variant=args.variant), s3.N, s3.T, s3.E)
         rep3 = sol3.autotune(a3, 20)
         tune1024 = rep3.candidates()
         log("1024x1024 autotune: " + tune_line(tune1024, 1))
@@ -1009,8 +1024,8 @@ def main():
         sb = build_scene(dict(specs=specs))
         ab = sb.step_args()
         log(f"batch: {sb.C} cloths, N={sb.N} T={sb.T} E={sb.E} built in {time.time() - t0:.1f} s")
-        solb = ch.Solver(ch.config(max_iterations=15, mode=ch.MODE_PD, gravity=-9.81,
-                                   dt=specs[0]["dt"], rho=0.99, cheb_delay=1, relax=args.relax),
+        solb = ch.Solver(solver_config(ch, max_iterations=15, mode=ch.MODE_PD, gravity=-9.81,
+                                       dt=specs[0]["dt"], rho=0.99, cheb_delay=1, relax=args.relax),
                          sb.N, sb.T, sb.E, cloth_offsets=sb.vert_off)
         t0 = time.time()
         rep = solb.autotune(ab, 10)
