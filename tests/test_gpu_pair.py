@@ -116,13 +116,14 @@ def test_cloth_accel_and_pins():
                 run(spec, 2, 3, accel=accel, max_iterations=16))
 
 
-def run_batch(specs, ipl, tol):
+def run_batch(specs, ipl, tol, relax=0.0):
     b = ch.Scene(specs=specs)
     b.upload()
     acc = ch.DeviceArray.from_host(
         np.random.default_rng(7).uniform(-3.0, 3.0, size=(len(specs), 3)).astype(np.float32))
     c = ch.config(mode=ch.MODE_PD, max_iterations=4000 if tol else 14, gravity=-9.81, dt=1 / 60,
-                  rho=0.999, cheb_delay=1, tolerance=tol, check_every=20, iters_per_launch=ipl)
+                  rho=0.999, cheb_delay=1, tolerance=tol, check_every=20, iters_per_launch=ipl,
+                  relax=relax)
     sol = ch.Solver(c, b.N, b.T, b.E, cloth_offsets=b.vert_off)
     args = b.step_args()
     args.d_cloth_accel = acc.ptr
@@ -134,14 +135,16 @@ def run_batch(specs, ipl, tol):
     return b.get_state(), its, sol.cloth_residuals() if tol else None
 
 
-@pytest.mark.parametrize("tol", [0.0, 1e-5])
-def test_batched_cloths(tol):
-    """The batch of the batching tests, per-cloth wind; to a tolerance without
-    the single-pin irregular cloth, which folds and stalls near 4e-2 (in the
-    oracle too)."""
-    specs = SPECS if tol == 0.0 else SPECS[:3]
-    (p1, v1), it1, r1 = run_batch(specs, 1, tol)
-    (p2, v2), it2, r2 = run_batch(specs, 2, tol)
+@pytest.mark.parametrize("tol,relax", [(0.0, 0.0), (1e-5, 0.0), (0.0, ch.RELAX_AUTO),
+                                       (1e-5, ch.RELAX_AUTO)])
+def test_batched_cloths(tol, relax):
+    """The batch of the batching tests, per-cloth wind. To a tolerance with
+    the plain Jacobi step the single-pin irregular cloth is left out: it
+    folds and stalls near 4e-2 (in the oracle too). With each cloth's own
+    damped step (CLH_RELAX_AUTO) it is kept, and every cloth converges."""
+    specs = SPECS if (tol == 0.0 or relax != 0.0) else SPECS[:3]
+    (p1, v1), it1, r1 = run_batch(specs, 1, tol, relax)
+    (p2, v2), it2, r2 = run_batch(specs, 2, tol, relax)
     assert it1 == it2
     np.testing.assert_array_equal(p1, p2)
     np.testing.assert_array_equal(v1, v2)
